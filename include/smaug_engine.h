@@ -1,0 +1,89 @@
+/*
+ * smaug_engine.h -- C-ABI of the MI355X stepping engine behind the
+ * reference's cu* entry points (declared in smaug_hip.h).
+ *
+ * The reference has no counterpart of this handle API: its state lives in
+ * globals created by cuinit (dev/smaug/src/init.cu:816-1068).  The engine
+ * is the same state made explicit, so that one process can hold several
+ * grids (the ensemble) and so tests can drive the path with plain arrays.
+ *
+ * Array conventions (same as the reference's wmod slot, iotypes.h:20-43):
+ *   cell c = i + n0*(j + n1*k), i fastest; n[d] counts 2 ghost layers per side.
+ *   evolved w  : (2+2*ndim) fields  rho', m_1..m_D, e', b'_1..b'_D
+ *   background : (2+ndim)   fields  e_b, rho_b, b_b1..b_bD
+ *   field f occupies [f*ncell, (f+1)*ncell).  Host arrays are always FP64.
+ * Every function returns 0 on success and a nonzero SMAUG_E* code on
+ * failure; smaug_last_error() gives the message.
+ */
+#ifndef SMAUG_ENGINE_H
+#define SMAUG_ENGINE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SMAUG_OK = 0,
+    SMAUG_EINVAL = 1,   /* bad argument or unsupported configuration */
+    SMAUG_EHIP = 2,     /* a HIP runtime call failed */
+    SMAUG_ENODEV = 3,   /* no usable gfx950 device */
+    SMAUG_ENOMEM = 4    /* device allocation failed */
+};
+
+enum { SMAUG_FP64 = 0, SMAUG_MIXED = 1 };
+
+typedef struct {
+    int ndim;             /* 2 or 3 */
+    int n[3];             /* cells per dim incl. ghosts, each >= 8; n[2] ignored for ndim 2 */
+    int bc[3];            /* boundtype per dim: 0 periodic, 4 contcd4 */
+    int hyperdif;         /* p->hyperdifmom */
+    int driver;           /* 0 none, 1 vertical Gaussian velocity driver */
+    int precision;        /* SMAUG_FP64 or SMAUG_MIXED */
+    double gamma;
+    double g[3];
+    double chyp[13];      /* indexed by the reference field enum */
+    double drv[8];        /* A, period, sigma_h, z0, sigma_z, xc, yc */
+    const double *dx[3];  /* cell metric per dim, length n[d] (copied) */
+    const double *pos[3]; /* cell centres per dim, length n[d] (copied) */
+} smaug_cfg;
+
+typedef struct smaug_engine smaug_engine;
+
+const char *smaug_last_error(void);
+
+int smaug_create(const smaug_cfg *cfg, smaug_engine **out);
+int smaug_destroy(smaug_engine *e);
+
+/* upload state (FP64 host arrays; rounded to FP32 for evolved fields in
+ * mixed mode) and fill the ghosts */
+int smaug_set_state(smaug_engine *e, const double *w, const double *bg);
+/* download the evolved state of the current step (FP64 host array) */
+int smaug_get_state(smaug_engine *e, double *w);
+
+/* L(w) = dU/dt of the current state on interior cells, 0 on ghosts */
+int smaug_rhs(smaug_engine *e, double t, double *rhs);
+/* cmax[d] over all cells of the current state */
+int smaug_cmax(smaug_engine *e, double cmax[3]);
+/* max over dims and cells of cfast_d/dx_d (Courant limit) */
+int smaug_maxcourant(smaug_engine *e, double *out);
+
+/* advance nsteps with fixed dt: classical RK4 / forward Euler.  Steps are
+ * replayed from a captured hipGraph; the call returns after the last step
+ * completes. */
+int smaug_rk4(smaug_engine *e, double t0, double dt, int nsteps);
+int smaug_euler(smaug_engine *e, double t0, double dt, int nsteps);
+
+/* time nsteps of RK4: total milliseconds of the step loop (HIP events on the
+ * engine stream) and the mean duration of one stage-update kernel launch
+ * (HIP events around each launch, eager launches) */
+int smaug_time_rk4(smaug_engine *e, double t0, double dt, int nsteps,
+                   double *total_ms, double *stage_kernel_ms);
+
+/* sizes */
+long long smaug_ncell(const smaug_engine *e);
+long long smaug_ninterior(const smaug_engine *e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

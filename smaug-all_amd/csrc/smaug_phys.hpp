@@ -1,0 +1,176 @@
+// smaug_phys.hpp -- device-side layout and pointwise physics shared by the
+// fused stage kernels and the per-entry-point kernels.
+//
+// Field layout follows the reference's enum order (dev/smaug/include/iotypes.h:213-221):
+//   evolved    0: rho'   1..D: m_d   1+D: e'   2+D..1+2D: b'_d
+//   background 0: e_b    1: rho_b    2..1+D: b_bd
+// T is the storage/arithmetic type of the evolved (perturbation) fields:
+// double for the plain FP64 path, float for the mixed path.  Background
+// fields are always double; pressure recovery and internal energy are
+// evaluated in double in both paths.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace smaug {
+
+typedef long long idx_t;
+
+struct Geo {
+    int n[3];         // cells incl. ghosts per dim (n[2] == 1 in 2-D)
+    int per[3];       // 1 if periodic along d (boundtype 0)
+    idx_t s1, s2, nc; // strides of dims 1 and 2, cell count
+    const double *dx[3];
+    const double *idx[3];  // 1/dx per cell
+    const double *prof[3]; // driver profile factors per dim
+};
+
+struct Phys {
+    double gamma;
+    double g[3];
+    double chyp[13];
+    double drv[8];
+    int hyperdif;
+    int driver;
+};
+
+template <int D>
+struct Lay {
+    static constexpr int NVE = 2 + 2 * D;
+    static constexpr int NB = 2 + D;
+    static constexpr int RHO = 0;
+    static constexpr int EN = 1 + D;
+    static constexpr int M0 = 1;
+    static constexpr int B0 = 2 + D;
+    // (q, dim) pairs for which a hyperviscosity ratio is formed:
+    // rho' x D, e_int x D, v_f x D*D, b'_f (f != dim) x D*(D-1)
+    static constexpr int NPAIR = 2 * D + D * D + D * (D - 1);
+    __host__ __device__ static constexpr int prho(int d) { return d; }
+    __host__ __device__ static constexpr int pe(int d) { return D + d; }
+    __host__ __device__ static constexpr int pv(int f, int d) { return 2 * D + f * D + d; }
+    __host__ __device__ static constexpr int pb(int f, int d)
+    {
+        return 2 * D + D * D + f * (D - 1) + (d < f ? d : d - 1);
+    }
+};
+
+template <int D>
+__device__ __forceinline__ idx_t stride(const Geo &g, int d)
+{
+    return d == 0 ? (idx_t)1 : (d == 1 ? g.s1 : g.s2);
+}
+
+// one cell's state
+template <int D, typename T>
+struct St {
+    T r, e, m[D], b[D];   // perturbations
+    double rb, eb, bb[D]; // background
+};
+
+template <int D, typename T>
+__device__ __forceinline__ void load_st(St<D, T> &s, const T *__restrict__ u,
+                                        const double *__restrict__ bg, idx_t nc, idx_t c)
+{
+    s.r = u[c];
+#pragma unroll
+    for (int d = 0; d < D; d++) s.m[d] = u[(idx_t)(1 + d) * nc + c];
+    s.e = u[(idx_t)(1 + D) * nc + c];
+#pragma unroll
+    for (int d = 0; d < D; d++) s.b[d] = u[(idx_t)(2 + D + d) * nc + c];
+    s.eb = bg[c];
+    s.rb = bg[nc + c];
+#pragma unroll
+    for (int d = 0; d < D; d++) s.bb[d] = bg[(idx_t)(2 + d) * nc + c];
+}
+
+// full density rho' + rho_b (formed in double)
+template <int D, typename T>
+__device__ __forceinline__ double rho_full(const St<D, T> &s)
+{
+    return (double)s.r + s.rb;
+}
+
+// total-pressure perturbation, computept3 (dervfields.cu:284-330):
+// (g-1)(e' - m^2/2rho) - (g-2)(b'.b_b + b'^2/2), in double
+template <int D, typename T>
+__device__ __forceinline__ double ptot_pert(const St<D, T> &s, double rho, double gam)
+{
+    double mm = 0, bgb = 0, bb = 0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        double m = s.m[d], b = s.b[d];
+        mm += m * m;
+        bgb += b * s.bb[d];
+        bb += b * b;
+    }
+    double ptmp = (gam - 2.0) * (bgb + 0.5 * bb);
+    return (gam - 1.0) * ((double)s.e - 0.5 * mm / rho) - ptmp;
+}
+
+// background total pressure, computepbg3 (dervfields.cu:332-368)
+template <int D, typename T>
+__device__ __forceinline__ double ptot_bg(const St<D, T> &s, double gam)
+{
+    double bbb = 0;
+#pragma unroll
+    for (int d = 0; d < D; d++) bbb += s.bb[d] * s.bb[d];
+    return (gam - 1) * s.eb - 0.5 * (gam - 2) * bbb;
+}
+
+// internal-energy perturbation used by the energy hyperdiffusion,
+// e' - (b'^2 + m^2/rho)/2 (hyperdifesource1.cu:300-330), in double
+template <int D, typename T>
+__device__ __forceinline__ double e_int(const St<D, T> &s, double rho)
+{
+    double bb = 0, mm = 0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        double b = s.b[d], m = s.m[d];
+        bb += b * b;
+        mm += m * m;
+    }
+    return (double)s.e - 0.5 * (bb + mm / rho);
+}
+
+// fast magnetosonic speed along d plus |v_d|, computec3 (dervfields.cu:418-486), in double
+template <int D, typename T>
+__device__ __forceinline__ double cfast(const St<D, T> &s, int d, double gam)
+{
+    double rho = rho_full(s);
+    double mm = 0, bb = 0, bgb = 0, bbb = 0;
+#pragma unroll
+    for (int t = 0; t < D; t++) {
+        double m = s.m[t], b = s.b[t], B = s.bb[t];
+        mm += m * m;
+        bb += b * b;
+        bgb += B * b;
+        bbb += B * B;
+    }
+    double pk = (gam - 1) * ((double)s.e - 0.5 * (mm / rho) - 0.5 * bb - bgb);
+    double cs2 = (gam * (pk + (gam - 1) * (s.eb - 0.5 * bbb))) / rho;
+    double cf2 = ((bb + bbb + 2.0 * bgb) / rho) + cs2;
+    double btd = 0, md = 0;
+#pragma unroll
+    for (int t = 0; t < D; t++)
+        if (t == d) {
+            btd = s.bb[t] + (double)s.b[t];
+            md = s.m[t];
+        }
+    double cfi = 0.5 * (cf2 + sqrt(cf2 * cf2 - 4.0 * cs2 * (btd * btd) / rho));
+    return sqrt(cfi) + fabs(md / rho);
+}
+
+// edge rules of the reference stencils along a non-periodic direction:
+// the 4th-order divergence is zero at x in {2,3,n-4,n-3} (grad3dn,
+// gradops.cu:437-602); the one-cell differences are zero at
+// x in {1,2,n-3,n-2} (grad1l3n/grad1r3n/grad13n, gradops.cu:608-830)
+__device__ __forceinline__ bool zero4(int per, int x, int n)
+{
+    return !per && (x == 2 || x == 3 || x == n - 4 || x == n - 3);
+}
+__device__ __forceinline__ bool zero1(int per, int x, int n)
+{
+    return !per && (x == 1 || x == 2 || x == n - 3 || x == n - 2);
+}
+
+} // namespace smaug

@@ -1,0 +1,173 @@
+"""ctypes binding of the MI355X engine (include/smaug_engine.h).
+
+The library is loaded from the package's in-tree build
+(``smaug-all_amd/lib/libsmaug_engine.so``).  There is no fallback: if the
+library is missing or no HIP device is present the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .configs import Setup
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libsmaug_engine.so")
+
+FP64 = 0
+MIXED = 1
+
+EXPORTS = [
+    "smaug_last_error", "smaug_create", "smaug_destroy", "smaug_set_state", "smaug_get_state",
+    "smaug_rhs", "smaug_cmax", "smaug_maxcourant", "smaug_rk4", "smaug_euler",
+    "smaug_time_rk4", "smaug_ncell", "smaug_ninterior",
+]
+
+
+class SmaugCfg(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int),
+        ("n", ctypes.c_int * 3),
+        ("bc", ctypes.c_int * 3),
+        ("hyperdif", ctypes.c_int),
+        ("driver", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("gamma", ctypes.c_double),
+        ("g", ctypes.c_double * 3),
+        ("chyp", ctypes.c_double * 13),
+        ("drv", ctypes.c_double * 8),
+        ("dx", ctypes.POINTER(ctypes.c_double) * 3),
+        ("pos", ctypes.POINTER(ctypes.c_double) * 3),
+    ]
+
+
+class SmaugError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise SmaugError(f"HIP engine library not built: {path} (run __graft_entry__.build())")
+        lib = ctypes.CDLL(path)
+        lib.smaug_last_error.restype = ctypes.c_char_p
+        lib.smaug_create.argtypes = [ctypes.POINTER(SmaugCfg), ctypes.POINTER(ctypes.c_void_p)]
+        lib.smaug_destroy.argtypes = [ctypes.c_void_p]
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.smaug_set_state.argtypes = [ctypes.c_void_p, dp, dp]
+        lib.smaug_get_state.argtypes = [ctypes.c_void_p, dp]
+        lib.smaug_rhs.argtypes = [ctypes.c_void_p, ctypes.c_double, dp]
+        lib.smaug_cmax.argtypes = [ctypes.c_void_p, dp]
+        lib.smaug_maxcourant.argtypes = [ctypes.c_void_p, dp]
+        lib.smaug_rk4.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        lib.smaug_euler.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        lib.smaug_time_rk4.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int, dp, dp]
+        lib.smaug_ncell.argtypes = [ctypes.c_void_p]
+        lib.smaug_ncell.restype = ctypes.c_longlong
+        lib.smaug_ninterior.argtypes = [ctypes.c_void_p]
+        lib.smaug_ninterior.restype = ctypes.c_longlong
+        _lib = lib
+    return _lib
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def make_cfg(s: Setup, precision: int):
+    """the smaug_cfg for a set-up; returns (cfg, keepalive arrays)"""
+    cfg = SmaugCfg()
+    cfg.ndim = s.ndim
+    keep = []
+    for d in range(3):
+        cfg.n[d] = s.n[d] if d < len(s.n) else 1
+        cfg.bc[d] = s.bc[d] if d < s.ndim else 0
+        cfg.g[d] = s.g[d]
+    cfg.hyperdif = s.hyperdif
+    cfg.driver = s.driver
+    cfg.precision = precision
+    cfg.gamma = s.gamma
+    for f in range(13):
+        cfg.chyp[f] = float(s.chyp[f])
+    for k in range(8):
+        cfg.drv[k] = float(s.drv[k])
+    for d in range(s.ndim):
+        dx = np.ascontiguousarray(s.dx[d], dtype=np.float64)
+        pos = np.ascontiguousarray(s.pos[d], dtype=np.float64)
+        keep += [dx, pos]
+        cfg.dx[d] = _dptr(dx)
+        cfg.pos[d] = _dptr(pos)
+    return cfg, keep
+
+
+class Engine:
+    """One grid on the GPU.  All state transfer is FP64 on the host side."""
+
+    def __init__(self, setup: Setup, precision: int = FP64):
+        self.lib = load_library()
+        self.setup = setup
+        self.precision = precision
+        cfg, self._keep = make_cfg(setup, precision)
+        h = ctypes.c_void_p()
+        self._check(self.lib.smaug_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise SmaugError(f"smaug error {rc}: {self.lib.smaug_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.smaug_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, w=None, bg=None):
+        s = self.setup
+        w = np.ascontiguousarray(s.w if w is None else w, dtype=np.float64)
+        bg = np.ascontiguousarray(s.bg if bg is None else bg, dtype=np.float64)
+        assert w.shape == (s.nve, s.ncell) and bg.shape == (s.nbg, s.ncell)
+        self._check(self.lib.smaug_set_state(self.h, _dptr(w), _dptr(bg)))
+
+    def get_state(self):
+        s = self.setup
+        w = np.empty((s.nve, s.ncell))
+        self._check(self.lib.smaug_get_state(self.h, _dptr(w)))
+        return w
+
+    def rhs(self, t=0.0):
+        s = self.setup
+        r = np.empty((s.nve, s.ncell))
+        self._check(self.lib.smaug_rhs(self.h, t, _dptr(r)))
+        return r
+
+    def cmax(self):
+        c = np.zeros(3)
+        self._check(self.lib.smaug_cmax(self.h, _dptr(c)))
+        return c
+
+    def rk4(self, t0, dt, nsteps):
+        self._check(self.lib.smaug_rk4(self.h, t0, dt, nsteps))
+
+    def euler(self, t0, dt, nsteps):
+        self._check(self.lib.smaug_euler(self.h, t0, dt, nsteps))
+
+    def time_rk4(self, t0, dt, nsteps):
+        tot = ctypes.c_double()
+        st = ctypes.c_double()
+        self._check(self.lib.smaug_time_rk4(self.h, t0, dt, nsteps, ctypes.byref(tot),
+                                            ctypes.byref(st)))
+        return tot.value, st.value

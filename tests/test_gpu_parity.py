@@ -1,0 +1,102 @@
+"""Parity of the HIP path (through the C-ABI of smaug_engine.h) with the CPU
+oracle on identical seeded inputs."""
+import numpy as np
+import pytest
+
+from helpers import interior_mask, perturbed, rel_err, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+from smaugamd import configs  # noqa: E402
+from smaugamd.engine import FP64, MIXED, Engine  # noqa: E402
+from oracle.oracle import Oracle  # noqa: E402
+
+# FP64 path vs FP64 oracle: same arithmetic up to operation order and FMA
+# contraction, so agreement to a few ulps times the operator's condition
+TOL_RHS_FP64 = 1e-10
+# mixed path (FP32 perturbations) vs FP64 oracle, one evaluation of L(w)
+TOL_RHS_MIXED = 2e-4
+
+
+def small_setups():
+    out = []
+    out.append(("ot32", configs.orszag_tang(nx=32)))
+    out.append(("bw64", configs.brio_wu(ncells=64)))
+    s = configs.stratified(nz=20, nx=12, ny=16, lz=6e6 * 20 / 256 * 4, lx=4e6 * 12 / 256,
+                           ly=4e6 * 16 / 256)
+    out.append(("strat20x12x16", s))
+    return out
+
+
+@pytest.mark.parametrize("name,setup", small_setups(), ids=lambda x: x if isinstance(x, str) else "")
+def test_rhs_fp64(name, setup):
+    print(f"[case] rhs fp64 {name} n={setup.n}", flush=True)
+    o = Oracle(setup)
+    w = o.boundary(perturbed(setup, seed=1))
+    ref = o.rhs(w, t=3.0)
+    e = Engine(setup, FP64)
+    e.set_state(w)
+    got = e.rhs(t=3.0)
+    err = rel_err(got, ref, interior_mask(setup))
+    print(name, "rhs rel err per field", err)
+    assert np.all(err < TOL_RHS_FP64), err
+
+
+@pytest.mark.parametrize("name,setup", small_setups(), ids=lambda x: x if isinstance(x, str) else "")
+def test_cmax_fp64(name, setup):
+    print(f"[case] cmax {name}", flush=True)
+    o = Oracle(setup)
+    w = o.boundary(perturbed(setup, seed=2))
+    e = Engine(setup, FP64)
+    e.set_state(w)
+    np.testing.assert_allclose(e.cmax()[:setup.ndim], o.cmax(w)[:setup.ndim], rtol=1e-13)
+
+
+@pytest.mark.parametrize("name,setup", small_setups(), ids=lambda x: x if isinstance(x, str) else "")
+def test_rhs_mixed(name, setup):
+    print(f"[case] rhs mixed {name}", flush=True)
+    o = Oracle(setup)
+    w = o.boundary(perturbed(setup, seed=3))
+    # the mixed path stores the perturbations in FP32: feed the oracle the
+    # same rounded state so the comparison measures the arithmetic only
+    w32 = o.boundary(w.astype(np.float32).astype(np.float64))
+    ref = o.rhs(w32, t=1.0)
+    e = Engine(setup, MIXED)
+    e.set_state(w32)
+    got = e.rhs(t=1.0)
+    err = rel_err(got, ref, interior_mask(setup))
+    print(name, "mixed rhs rel err per field", err)
+    assert np.all(err < TOL_RHS_MIXED), err
+
+
+@pytest.mark.parametrize("name,setup", small_setups(), ids=lambda x: x if isinstance(x, str) else "")
+def test_rk4_steps_fp64(name, setup):
+    print(f"[case] rk4 fp64 {name}", flush=True)
+    o = Oracle(setup)
+    w0 = o.boundary(perturbed(setup, amp=1e-3, seed=4))
+    dt = 0.3 / configs.initial_max_courant(setup) if setup.dt == 0 else setup.dt * 0.5
+    if name.startswith("ot"):
+        dt = 1e-3
+    ref, _ = o.run(w0, 0.0, dt, 5, rk4=True)
+    e = Engine(setup, FP64)
+    e.set_state(w0)
+    e.rk4(0.0, dt, 5)
+    got = e.get_state()
+    err = rel_err(got, ref)
+    print(name, "rk4 5 steps rel err", err)
+    assert np.all(err < 1e-11), err
+
+
+def test_euler_steps_fp64():
+    setup = configs.brio_wu(ncells=64)
+    print("[case] euler bw64", flush=True)
+    o = Oracle(setup)
+    w0 = o.boundary(setup.w)
+    ref, _ = o.run(w0, 0.0, 2e-4, 10, rk4=False)
+    e = Engine(setup, FP64)
+    e.set_state(w0)
+    e.euler(0.0, 2e-4, 10)
+    got = e.get_state()
+    err = rel_err(got, ref)
+    print("euler 10 steps rel err", err)
+    assert np.all(err < 1e-11), err
