@@ -1,0 +1,177 @@
+"""Benchmark of the SMAUG per-step MHD update on one MI355X.
+
+Workload: the 3-D gravitationally stratified atmosphere, 256^3 interior
+cells over 6 x 4 x 4 Mm (vertical axis first), FP32-perturbation /
+FP64-background mixed precision, classical RK4 with the fixed CFL-0.3 dt of
+the initial state.  A step is one RK4 step of the whole grid (four fused
+stage updates plus their cmax, hyperviscosity-ratio and ghost-fill passes),
+replayed from a captured hipGraph; inputs are resident in HBM.
+
+Printed JSON line (rank 0): the contract fields plus
+  fp64        : the plain FP64 GPU path on the same workload and steps
+  accuracy    : per-field relative L2 / Linf of the mixed result against the
+                FP64 GPU result after the timed steps (normalised by each
+                field's amplitude)
+  roofline    : the stage-update kernel against HBM peak
+  cpu_baseline: the CPU oracle (OpenMP) on a bounded sample of the workload
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "smaug-all_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic HBM bytes per cell per RK4 step (low-storage RK4: 72 + 4*N_B
+# reads and 56 writes of field values, N_B = 4 background fields):
+#   FP64  (128 + 16) * 8        = 1152
+#   mixed 128 * 4 + 16 * 8      = 640
+BYTES_PER_CELL_STEP = {"mixed": 640.0, "fp64": 1152.0}
+STAGES_PER_STEP = 4
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256, help="interior cells per axis")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    return ap.parse_args()
+
+
+def run_path(setup, precision, warmup, steps):
+    from smaugamd.engine import Engine
+
+    e = Engine(setup, precision)
+    e.set_state()
+    e.rk4(0.0, setup.dt, warmup)
+    t0 = warmup * setup.dt
+    tot_ms, stage_ms = e.time_rk4(t0, setup.dt, steps)
+    w = e.get_state()
+    e.close()
+    return tot_ms, stage_ms, w
+
+
+def cpu_baseline(n):
+    """oracle RK4 steps on a bounded sample of the workload: the same vertical
+    column structure with a reduced horizontal extent"""
+    from oracle.oracle import Oracle
+    from smaugamd import configs
+
+    nh = 48
+    s = configs.stratified(nz=n, nx=nh, ny=nh, lx=4.0e6 * nh / 256, ly=4.0e6 * nh / 256)
+    o = Oracle(s)
+    w0 = o.boundary(s.w)
+    o.run(w0, 0.0, s.dt, 1)  # warm caches and page in
+    steps = 2
+    t = time.perf_counter()
+    o.run(w0, 0.0, s.dt, steps)
+    el = time.perf_counter() - t
+    return {
+        "value": s.ninterior * steps / el / 1e6,
+        "unit": "Mcell-updates/s",
+        "cores": Oracle.threads(),
+        "kind": "port",
+        "sample": f"{steps} RK4 steps of the stratified workload on {n}x{nh}x{nh} interior "
+                  f"cells (full vertical column, 48x48 horizontal), FP64, "
+                  f"{el:.1f} s wall",
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    import numpy as np
+
+    from smaugamd import configs
+    from smaugamd.engine import FP64, MIXED
+
+    n = a.n
+    setup = configs.stratified(nz=n, nx=n, ny=n)
+    cells = setup.ninterior
+
+    print(f"[bench] set-up done: {setup.n} dt={setup.dt:.4e}", file=sys.stderr, flush=True)
+    tot, stage_ms, wm = run_path(setup, MIXED, a.warmup, a.steps)
+    print(f"[bench] mixed path: {tot / a.steps:.3f} ms/step", file=sys.stderr, flush=True)
+    ms_step = tot / a.steps
+    value = cells * a.steps / (tot / 1e3) / 1e6
+
+    out_fp64 = None
+    acc = None
+    if not a.no_fp64:
+        tot64, stage64, wd = run_path(setup, FP64, a.warmup, a.steps)
+        print(f"[bench] fp64 path: {tot64 / a.steps:.3f} ms/step", file=sys.stderr, flush=True)
+        out_fp64 = {
+            "value": cells * a.steps / (tot64 / 1e3) / 1e6,
+            "ms_per_step": tot64 / a.steps,
+            "stage_kernel_ms": stage64,
+            "speedup_mixed_over_fp64": (tot64 / tot),
+        }
+        n0, n1, n2 = setup.n
+        m = np.zeros((n2, n1, n0), dtype=bool)
+        m[2:-2, 2:-2, 2:-2] = True
+        m = m.reshape(-1)
+        names = ["rho", "m1", "m2", "m3", "e", "b1", "b2", "b3"]
+        l2, linf = {}, {}
+        for f in range(setup.nve):
+            ref = wd[f][m]
+            d = wm[f][m] - ref
+            amp = np.abs(ref).max()
+            rms = np.sqrt(np.mean(ref ** 2))
+            l2[names[f]] = float(np.sqrt(np.mean(d ** 2)) / rms) if rms > 0 else 0.0
+            linf[names[f]] = float(np.abs(d).max() / amp) if amp > 0 else 0.0
+        acc = {"reference": "FP64 GPU path (parity-tested against the FP64 oracle)",
+               "steps": a.warmup + a.steps + 1, "rel_l2": l2, "rel_linf": linf}
+
+    bytes_launch = BYTES_PER_CELL_STEP["mixed"] / STAGES_PER_STEP * cells
+    achieved = bytes_launch / (stage_ms / 1e3) / 1e9
+    roof = {
+        "bound": "hbm",
+        "kernel": "k_update<3,float> (fused RK4 stage update)",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": None,
+        "step_level_frac": (BYTES_PER_CELL_STEP["mixed"] * cells / (ms_step / 1e3) / 1e9)
+        / HBM_PEAK_GBS,
+    }
+    from smaugamd.engine import copy_probe
+
+    roof["measured_copy_gbs"] = copy_probe()
+    roof["frac_of_measured_copy"] = achieved / roof["measured_copy_gbs"]
+    cpu = None if a.no_cpu else cpu_baseline(n)
+    rec = {
+        "metric": "Mcell-updates/s per RK4 step, mixed vs FP64 at stated L2 error; % HBM peak",
+        "value": value,
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-perturbation/fp64-background (mixed)",
+        "data": "synthetic",
+        "config": {"workload": f"3-D stratified solar atmosphere {n}x{n}x{n} cells over "
+                               f"6x4x4 Mm, RK4, dt={setup.dt:.4e} s (CFL 0.3)",
+                   "cells": cells, "precision": "mixed", "parallelism": "single GPU"},
+        "fp64": out_fp64,
+        "accuracy": acc,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

@@ -79,6 +79,10 @@ int smaug_euler(smaug_engine *e, double t0, double dt, int nsteps);
 int smaug_time_rk4(smaug_engine *e, double t0, double dt, int nsteps,
                    double *total_ms, double *stage_kernel_ms);
 
+/* measured copy-bandwidth ceiling: GB/s (read + write bytes) of a plain
+ * streaming copy of nbytes, mean of reps launches */
+int smaug_copy_probe(long long nbytes, int reps, double *gbs);
+
 /* sizes */
 long long smaug_ncell(const smaug_engine *e);
 long long smaug_ninterior(const smaug_engine *e);

@@ -537,3 +537,49 @@ extern "C" int smaug_time_rk4(smaug_engine *e, double t0, double dt, int nsteps,
     if (!e || nsteps < 1 || !total_ms || !stage_kernel_ms) return fail(SMAUG_EINVAL, "bad argument");
     return DISPATCH(e, time_rk4, e, t0, dt, nsteps, total_ms, stage_kernel_ms);
 }
+
+// ---------------------------------------------------------------------------
+// measured copy-bandwidth ceiling: a plain streaming copy of nbytes
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_copy_probe(const float4 *__restrict__ a,
+                                                     float4 *__restrict__ b, idx_t n)
+{
+    for (idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (idx_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+extern "C" int smaug_copy_probe(long long nbytes, int reps, double *gbs)
+{
+    if (nbytes < 16 || reps < 1 || !gbs) return fail(SMAUG_EINVAL, "bad argument");
+    idx_t n = nbytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc((void **)&a, n * 16));
+    hipError_t he = hipMalloc((void **)&b, n * 16);
+    if (he != hipSuccess) {
+        hipFree(a);
+        return fail(SMAUG_ENOMEM, "hipMalloc: %s", hipGetErrorString(he));
+    }
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIPCHK(hipMemsetAsync(a, 0, n * 16, st));
+    unsigned nb = 256 * 32; // 32 workgroups per CU
+    hipLaunchKernelGGL(k_copy_probe, dim3(nb), dim3(256), 0, st, a, b, n);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(k_copy_probe, dim3(nb), dim3(256), 0, st, a, b, n);
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *gbs = 2.0 * (double)n * 16 * reps / (ms / 1e3) / 1e9;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipStreamDestroy(st);
+    hipFree(a);
+    hipFree(b);
+    return 0;
+}

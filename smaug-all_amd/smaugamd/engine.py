@@ -23,7 +23,7 @@ MIXED = 1
 EXPORTS = [
     "smaug_last_error", "smaug_create", "smaug_destroy", "smaug_set_state", "smaug_get_state",
     "smaug_rhs", "smaug_cmax", "smaug_maxcourant", "smaug_rk4", "smaug_euler",
-    "smaug_time_rk4", "smaug_ncell", "smaug_ninterior",
+    "smaug_time_rk4", "smaug_ncell", "smaug_ninterior", "smaug_copy_probe",
 ]
 
 
@@ -74,6 +74,7 @@ def load_library(path: str = LIB_PATH):
         lib.smaug_ncell.restype = ctypes.c_longlong
         lib.smaug_ninterior.argtypes = [ctypes.c_void_p]
         lib.smaug_ninterior.restype = ctypes.c_longlong
+        lib.smaug_copy_probe.argtypes = [ctypes.c_longlong, ctypes.c_int, dp]
         _lib = lib
     return _lib
 
@@ -171,3 +172,13 @@ class Engine:
         self._check(self.lib.smaug_time_rk4(self.h, t0, dt, nsteps, ctypes.byref(tot),
                                             ctypes.byref(st)))
         return tot.value, st.value
+
+
+def copy_probe(nbytes=2 << 30, reps=10):
+    """measured streaming-copy bandwidth (GB/s, read + write) of the device"""
+    lib = load_library()
+    g = ctypes.c_double()
+    rc = lib.smaug_copy_probe(nbytes, reps, ctypes.byref(g))
+    if rc:
+        raise SmaugError(lib.smaug_last_error().decode())
+    return g.value
