@@ -71,13 +71,30 @@ static int launch_cmax(smaug_engine *e, const void *u)
     return 0;
 }
 
+// tiles of 64 x 4 cells over an extent of e0 x e1 x e2 cells, grouped in
+// panels so that each XCD sweeps its own sub-domain (see map_block)
+static TileMap make_map(int e0, int e1, int e2, unsigned *nblocks)
+{
+    TileMap m;
+    m.ntx = (e0 + 63) / 64;
+    m.nty = (e1 + 3) / 4;
+    m.nk = e2;
+    m.tpx = m.ntx < 8 ? m.ntx : 8;
+    m.tpy = 16 / m.tpx > 0 ? 16 / m.tpx : 1;
+    m.npx = (m.ntx + m.tpx - 1) / m.tpx;
+    int npy = (m.nty + m.tpy - 1) / m.tpy;
+    m.npanels = m.npx * npy;
+    int per_xcd = (m.npanels + 7) / 8;
+    *nblocks = (unsigned)(8 * per_xcd * m.tpx * m.tpy * m.nk);
+    return m;
+}
+
 template <int D, typename T>
 static int launch_ratio(smaug_engine *e, const void *u)
 {
-    dim3 blk(64, 4, 1);
-    int e0 = e->n[0] - 3, e1 = e->n[1] - 3; // box [1, n-3]
-    dim3 grd((e0 + 63) / 64, (e1 + 3) / 4, D == 3 ? e->n[2] - 3 : 1);
-    hipLaunchKernelGGL((k_ratio<D, T>), grd, blk, 0, e->stream, e->geo, (const T *)u,
+    unsigned nb;
+    TileMap m = make_map(e->n[0] - 3, e->n[1] - 3, D == 3 ? e->n[2] - 3 : 1, &nb); // box [1, n-3]
+    hipLaunchKernelGGL((k_ratio<D, T>), dim3(nb), dim3(256), 0, e->stream, e->geo, m, (const T *)u,
                        (const double *)e->bg, (T *)e->R);
     HIPCHK(hipGetLastError());
     return 0;
@@ -87,10 +104,9 @@ template <int D, typename T>
 static int launch_update(smaug_engine *e, StageArgs sa, const void *uin, const void *un,
                          void *uout)
 {
-    dim3 blk(64, 4, 1);
-    int i0 = e->n[0] - 4, i1 = e->n[1] - 4;
-    dim3 grd((i0 + 63) / 64, (i1 + 3) / 4, D == 3 ? e->n[2] - 4 : 1);
-    hipLaunchKernelGGL((k_update<D, T>), grd, blk, 0, e->stream, e->geo, e->phys, sa,
+    unsigned nb;
+    TileMap m = make_map(e->n[0] - 4, e->n[1] - 4, D == 3 ? e->n[2] - 4 : 1, &nb);
+    hipLaunchKernelGGL((k_update<D, T>), dim3(nb), dim3(256), 0, e->stream, e->geo, m, e->phys, sa,
                        (const T *)uin, (const T *)un, (T *)uout, (T *)e->ACC,
                        (const T *)e->R, (const double *)e->bg,
                        (const unsigned long long *)e->d_cmax);

@@ -97,13 +97,15 @@ __device__ __forceinline__ T ratio6(const T q[6])
 }
 
 template <int D, typename T>
-__global__ __launch_bounds__(256) void k_ratio(Geo g, const T *__restrict__ u,
+__global__ __launch_bounds__(256) void k_ratio(Geo g, TileMap tm, const T *__restrict__ u,
                                                 const double *__restrict__ bg, T *__restrict__ R)
 {
     typedef Lay<D> LY;
-    int x0 = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    int x1 = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-    int x2 = (D == 3) ? 1 + (int)blockIdx.z : 0;
+    int tx, ty, tk;
+    if (!map_block(tm, tx, ty, tk)) return;
+    int x0 = 1 + tx * 64 + (threadIdx.x & 63);
+    int x1 = 1 + ty * 4 + (threadIdx.x >> 6);
+    int x2 = (D == 3) ? 1 + tk : 0;
     if (x0 > g.n[0] - 3 || x1 > g.n[1] - 3) return;
     int xs[3] = {x0, x1, x2};
     idx_t c = (idx_t)x0 + g.s1 * x1 + g.s2 * x2;
@@ -112,31 +114,34 @@ __global__ __launch_bounds__(256) void k_ratio(Geo g, const T *__restrict__ u,
         int x = xs[d], n = g.n[d];
         idx_t sd = stride<D>(g, d);
         idx_t c0 = c - (idx_t)x * sd;
-        T qr[6], qe[6], qv[D][6], qb[D][6];
+        // the differences and the ratio are formed in double in both paths:
+        // third differences of nearly smooth FP32 data sit near the FP32
+        // rounding level, and the ratio is the quantity that amplifies it
+        double qr[6], qe[6], qv[D][6], qb[D][6];
 #pragma unroll
         for (int p = 0; p < 6; p++) {
             int xx = x - 2 + p;
             if (xx < 0) xx = 5;
             if (xx >= n) xx = n - 6;
             St<D, T> s;
-            load_st<D, T>(s, u, bg, g.nc, c0 + (idx_t)xx * sd);
+            load_light<D, T>(s, u, bg, g.nc, c0 + (idx_t)xx * sd);
             double rho = rho_full(s);
             qr[p] = s.r;
-            qe[p] = (T)e_int<D, T>(s, rho);
-            T ir = (T)(1.0 / rho);
+            qe[p] = e_int<D, T>(s, rho);
+            double ir = 1.0 / rho;
 #pragma unroll
             for (int f = 0; f < D; f++) {
-                qv[f][p] = s.m[f] * ir;
+                qv[f][p] = (double)s.m[f] * ir;
                 qb[f][p] = s.b[f];
             }
         }
-        R[(idx_t)LY::prho(d) * g.nc + c] = ratio6<T>(qr);
-        R[(idx_t)LY::pe(d) * g.nc + c] = ratio6<T>(qe);
+        R[(idx_t)LY::prho(d) * g.nc + c] = (T)ratio6<double>(qr);
+        R[(idx_t)LY::pe(d) * g.nc + c] = (T)ratio6<double>(qe);
 #pragma unroll
-        for (int f = 0; f < D; f++) R[(idx_t)LY::pv(f, d) * g.nc + c] = ratio6<T>(qv[f]);
+        for (int f = 0; f < D; f++) R[(idx_t)LY::pv(f, d) * g.nc + c] = (T)ratio6<double>(qv[f]);
 #pragma unroll
         for (int f = 0; f < D; f++)
-            if (f != d) R[(idx_t)LY::pb(f, d) * g.nc + c] = ratio6<T>(qb[f]);
+            if (f != d) R[(idx_t)LY::pb(f, d) * g.nc + c] = (T)ratio6<double>(qb[f]);
     }
 }
 
@@ -213,7 +218,7 @@ __device__ __forceinline__ T div4(T qm2, T qm1, T q1, T q2, T i12dx)
 }
 
 template <int D, typename T>
-__global__ __launch_bounds__(256) void k_update(Geo g, Phys ph, StageArgs sa,
+__global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, StageArgs sa,
                                                  const T *__restrict__ uin,
                                                  const T *__restrict__ un,
                                                  T *__restrict__ uout, T *__restrict__ acc,
@@ -222,9 +227,11 @@ __global__ __launch_bounds__(256) void k_update(Geo g, Phys ph, StageArgs sa,
                                                  const unsigned long long *__restrict__ cmb)
 {
     typedef Lay<D> LY;
-    const int x0 = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int x1 = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-    const int x2 = (D == 3) ? 2 + (int)blockIdx.z : 0;
+    int tx, ty, tk;
+    if (!map_block(tm, tx, ty, tk)) return;
+    const int x0 = 2 + tx * 64 + (threadIdx.x & 63);
+    const int x1 = 2 + ty * 4 + (threadIdx.x >> 6);
+    const int x2 = (D == 3) ? 2 + tk : 0;
     if (x0 > g.n[0] - 3 || x1 > g.n[1] - 3) return;
     const int xs[3] = {x0, x1, x2};
     const idx_t nc = g.nc;
@@ -247,11 +254,17 @@ __global__ __launch_bounds__(256) void k_update(Geo g, Phys ph, StageArgs sa,
     for (int a = 0; a < D; a++) {
         const idx_t sa_ = stride<D>(g, a);
         const int xa = xs[a], na = g.n[a];
-        T F[LY::NVE][4];
-        T V[D][4];
+        if (zero4(g.per[a], xa, na)) continue;
+        // d/dx_a q = (q[-2] - 8 q[-1] + 8 q[+1] - q[+2]) / (12 dx), accumulated
+        // neighbour by neighbour
+        const T i12 = (T)(1.0 / (12.0 * g.dx[a][xa]));
+        T gv[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) gv[d] = 0;
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             const int off = (p < 2) ? p - 2 : p - 1; // -2,-1,+1,+2
+            const T cw = (p == 0 ? (T)1 : (p == 1 ? (T)-8 : (p == 2 ? (T)8 : (T)-1))) * i12;
             St<D, T> s;
             load_st<D, T>(s, uin, bg, nc, c + (idx_t)off * sa_);
             const double rho = rho_full(s);
@@ -264,37 +277,29 @@ __global__ __launch_bounds__(256) void k_update(Geo g, Phys ph, StageArgs sa,
                 bbm += (T)s.bb[t] * s.m[t];
                 bm += s.b[t] * s.m[t];
             }
-            F[LY::RHO][p] = ma; // m_a rho'/rho + rho_b m_a/rho
+            // F_rho = m_a rho'/rho + rho_b m_a/rho = m_a
+            dU[LY::RHO] -= cw * ma;
 #pragma unroll
             for (int i = 0; i < D; i++) {
                 T mag = -(s.b[i] * (T)s.bb[a] + (T)s.bb[i] * s.b[a]) - s.b[i] * s.b[a];
-                F[LY::M0 + i][p] = ma * s.m[i] * ir + ((i == a) ? pt + mag : mag);
+                dU[LY::M0 + i] -= cw * (ma * s.m[i] * ir + ((i == a) ? pt + mag : mag));
             }
-            F[LY::EN][p] = (ma * (s.e + (T)s.eb + pt) - s.b[a] * bbm - (T)s.bb[a] * bm -
-                            s.b[a] * bm) * ir;
+            dU[LY::EN] -= cw * ((ma * (s.e + (T)s.eb + pt) - s.b[a] * bbm - (T)s.bb[a] * bm -
+                                 s.b[a] * bm) * ir);
 #pragma unroll
             for (int i = 0; i < D; i++) {
-                F[LY::B0 + i][p] = (i == a) ? (T)0
-                                            : (ma * s.b[i] - (s.b[a] + (T)s.bb[a]) * s.m[i] +
-                                               (T)s.bb[i] * ma) * ir;
+                if (i == a) continue;
+                dU[LY::B0 + i] -= cw * ((ma * s.b[i] - (s.b[a] + (T)s.bb[a]) * s.m[i] +
+                                         (T)s.bb[i] * ma) * ir);
             }
 #pragma unroll
-            for (int d = 0; d < D; d++) V[d][p] = s.m[d] * ir;
+            for (int d = 0; d < D; d++) gv[d] += cw * (s.m[d] * ir);
         }
-        if (!zero4(g.per[a], xa, na)) {
-            const T i12 = (T)(1.0 / (12.0 * g.dx[a][xa]));
+        // -p_Tb dv_a/dx_a + sum_d b_ba b_bd dv_d/dx_a  (addenergyterms_cd2)
+        T w = -ptb * gv[a];
 #pragma unroll
-            for (int f = 0; f < LY::NVE; f++)
-                dU[f] -= div4<T>(F[f][0], F[f][1], F[f][2], F[f][3], i12);
-            T gv[D];
-#pragma unroll
-            for (int d = 0; d < D; d++) gv[d] = div4<T>(V[d][0], V[d][1], V[d][2], V[d][3], i12);
-            // -p_Tb dv_a/dx_a + sum_d b_ba b_bd dv_d/dx_a  (addenergyterms_cd2)
-            T w = -ptb * gv[a];
-#pragma unroll
-            for (int d = 0; d < D; d++) w += (T)sc.bb[a] * (T)sc.bb[d] * gv[d];
-            dU[LY::EN] += w;
-        }
+        for (int d = 0; d < D; d++) w += (T)sc.bb[a] * (T)sc.bb[d] * gv[d];
+        dU[LY::EN] += w;
     }
 
     // ---- gravity (addgrav_cd2, centdiff2.cu:118-154)
@@ -327,8 +332,8 @@ __global__ __launch_bounds__(256) void k_update(Geo g, Phys ph, StageArgs sa,
             const T idxd = (T)g.idx[d][xd];
             const bool z1 = zero1(g.per[d], xd, nd);
             St<D, T> sl, sr;
-            load_st<D, T>(sl, uin, bg, nc, c - sd);
-            load_st<D, T>(sr, uin, bg, nc, c + sd);
+            load_light<D, T>(sl, uin, bg, nc, c - sd);
+            load_light<D, T>(sr, uin, bg, nc, c + sd);
             const double rhol = rho_full(sl), rhor = rho_full(sr);
             const T irl = (T)(1.0 / rhol), irr = (T)(1.0 / rhor);
 
@@ -394,7 +399,7 @@ __global__ __launch_bounds__(256) void k_update(Geo g, Phys ph, StageArgs sa,
                     const int xfn = xf + sg;
                     if (xfn < 2 || xfn > nf - 3) continue; // nu vanishes off the interior
                     St<D, T> sn;
-                    load_st<D, T>(sn, uin, bg, nc, cn);
+                    load_light<D, T>(sn, uin, bg, nc, cn);
                     const double rhon = rho_full(sn);
                     // v_f and b_f at cn -/+ e_d
                     const idx_t ca = cn - sd, cb = cn + sd;

@@ -83,6 +83,53 @@ __device__ __forceinline__ void load_st(St<D, T> &s, const T *__restrict__ u,
     for (int d = 0; d < D; d++) s.bb[d] = bg[(idx_t)(2 + d) * nc + c];
 }
 
+// the perturbations and rho_b only (what the hyperdiffusion variables need)
+template <int D, typename T>
+__device__ __forceinline__ void load_light(St<D, T> &s, const T *__restrict__ u,
+                                           const double *__restrict__ bg, idx_t nc, idx_t c)
+{
+    s.r = u[c];
+#pragma unroll
+    for (int d = 0; d < D; d++) s.m[d] = u[(idx_t)(1 + d) * nc + c];
+    s.e = u[(idx_t)(1 + D) * nc + c];
+#pragma unroll
+    for (int d = 0; d < D; d++) s.b[d] = u[(idx_t)(2 + D + d) * nc + c];
+    s.rb = bg[nc + c];
+    s.eb = 0;
+#pragma unroll
+    for (int d = 0; d < D; d++) s.bb[d] = 0;
+}
+
+// Block-to-tile map that keeps the L2 of each XCD on one sub-domain.
+// Workgroups are handed to the 8 XCDs round-robin in launch order, so
+// block b runs on XCD b % 8.  The (tile_x, tile_y) tiles of a plane are
+// grouped into panels of tpx x tpy tiles; panel p belongs to XCD p % 8,
+// and an XCD walks each of its panels plane by plane (k innermost over
+// panels), so the k-1..k+2 planes a stencil touches are still in its L2.
+struct TileMap {
+    int ntx, nty, nk;  // tiles along x, y and planes
+    int tpx, tpy;      // panel size in tiles
+    int npx, npanels;  // panels along x, total panels
+};
+
+__device__ __forceinline__ bool map_block(const TileMap &m, int &tx, int &ty, int &k)
+{
+    const int b = blockIdx.x;
+    const int xcd = b & 7;
+    const int local = b >> 3;
+    const int tpp = m.tpx * m.tpy;
+    const int per_panel = tpp * m.nk;
+    const int pidx = local / per_panel;
+    const int panel = xcd + 8 * pidx;
+    if (panel >= m.npanels) return false;
+    const int rem = local - pidx * per_panel;
+    k = rem / tpp;
+    const int t = rem - k * tpp;
+    tx = (panel % m.npx) * m.tpx + t % m.tpx;
+    ty = (panel / m.npx) * m.tpy + t / m.tpx;
+    return tx < m.ntx && ty < m.nty;
+}
+
 // full density rho' + rho_b (formed in double)
 template <int D, typename T>
 __device__ __forceinline__ double rho_full(const St<D, T> &s)

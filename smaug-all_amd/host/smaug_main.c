@@ -1,0 +1,280 @@
+/*
+ * smaug_main.c -- host driver of the drop-in path, built once per
+ * dimensionality (-DUSE_SAC / -DUSE_SAC_3D) against libsmaug_hip2d/3d.so.
+ *
+ * The onestep (rkon = 0) time loop is the reference's, call for call
+ * (dev/smaug/src/iosmaug.cpp:786-1168): slot copy, optional Courant and
+ * hyperviscous dt control, advection per direction and field, gravity,
+ * hyperdiffusion per field/direction with the (ii, ii0) and (jj, mm, sb)
+ * patterns, user source and the ghost fill of every field and direction.
+ * rkon = 1 advances with classical RK4 on the fused engine (cuadvancerk4).
+ *
+ * The parameters the reference sets in iosmaugparams.h come from the
+ * configuration file header (grid, gamma, gravity) and the command line:
+ *
+ *   smaug_main -i in.ini [-o out.ini] [-n steps] [-dt dt] [-rk4] [-mixed]
+ *              [-bc b0,b1[,b2]] [-hyperdif 0|1] [-chyp rho,e,mom,b]
+ *              [-courant c] [-moddton] [-driver A,P,sh,z0,sz,xc,yc] [-t0 t]
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/smaug_hip.h"
+#include "vacio.h"
+
+#define CALL(x)                                                                            \
+    do {                                                                                   \
+        int rc_ = (x);                                                                     \
+        if (rc_) {                                                                         \
+            fprintf(stderr, "%s failed (%d): %s\n", #x, rc_, cuerrorstring());             \
+            exit(1);                                                                       \
+        }                                                                                  \
+    } while (0)
+
+static int parse_list(const char *s, double *v, int maxn)
+{
+    int k = 0;
+    size_t l = strlen(s);
+    char *dup = (char *)malloc(l + 1), *tok;
+    memcpy(dup, s, l + 1);
+    tok = strtok(dup, ",");
+    while (tok && k < maxn) {
+        v[k++] = atof(tok);
+        tok = strtok(NULL, ",");
+    }
+    free(dup);
+    return k;
+}
+
+int main(int argc, char **argv)
+{
+    const char *in = NULL, *out = NULL;
+    int nt = 1, rk4 = 0, mixed = 0, hyperdif = 1, moddton = 0, bcset = 0, driver = 0;
+    double dtarg = 0, courant = 0.3, t0 = 0;
+    double bcv[3] = {4, 4, 4}, chv[4] = {0.02, 0.02, 0.4, 0.02}, drv[7] = {0};
+    for (int a = 1; a < argc; a++) {
+        if (!strcmp(argv[a], "-i") && a + 1 < argc) in = argv[++a];
+        else if (!strcmp(argv[a], "-o") && a + 1 < argc) out = argv[++a];
+        else if (!strcmp(argv[a], "-n") && a + 1 < argc) nt = atoi(argv[++a]);
+        else if (!strcmp(argv[a], "-dt") && a + 1 < argc) dtarg = atof(argv[++a]);
+        else if (!strcmp(argv[a], "-t0") && a + 1 < argc) t0 = atof(argv[++a]);
+        else if (!strcmp(argv[a], "-rk4")) rk4 = 1;
+        else if (!strcmp(argv[a], "-mixed")) mixed = 1;
+        else if (!strcmp(argv[a], "-moddton")) moddton = 1;
+        else if (!strcmp(argv[a], "-courant") && a + 1 < argc) courant = atof(argv[++a]);
+        else if (!strcmp(argv[a], "-hyperdif") && a + 1 < argc) hyperdif = atoi(argv[++a]);
+        else if (!strcmp(argv[a], "-bc") && a + 1 < argc) bcset = parse_list(argv[++a], bcv, 3);
+        else if (!strcmp(argv[a], "-chyp") && a + 1 < argc) parse_list(argv[++a], chv, 4);
+        else if (!strcmp(argv[a], "-driver") && a + 1 < argc) driver = parse_list(argv[++a], drv, 7) == 7;
+        else {
+            fprintf(stderr, "unknown or incomplete option %s\n", argv[a]);
+            return 2;
+        }
+    }
+    if (!in) {
+        fprintf(stderr, "usage: %s -i config.ini [-o out.ini] [-n steps] [-dt dt] [-rk4] ...\n", argv[0]);
+        return 2;
+    }
+
+    /* grid and equation parameters from the configuration header */
+    FILE *f = fopen(in, "r");
+    if (!f) {
+        fprintf(stderr, "cannot open %s\n", in);
+        return 1;
+    }
+    char line[4096];
+    int dims[3] = {1, 1, 1};
+    double eq[8] = {0};
+    if (!fgets(line, sizeof line, f) || !fgets(line, sizeof line, f) || !fgets(line, sizeof line, f)) {
+        fprintf(stderr, "%s: short header\n", in);
+        return 1;
+    }
+    sscanf(line, "%d %d %d", &dims[0], &dims[1], &dims[2]);
+    if (!fgets(line, sizeof line, f)) return 1;
+    sscanf(line, "%lf %lf %lf %lf %lf", &eq[0], &eq[1], &eq[2], &eq[3], &eq[4]);
+    fclose(f);
+
+    struct params *p = (struct params *)calloc(1, sizeof(struct params));
+    struct state *st = (struct state *)calloc(1, sizeof(struct state));
+    struct bparams *bp = (struct bparams *)calloc(1, sizeof(struct bparams));
+    long dimp = 1;
+    for (int d = 0; d < NDIM; d++) {
+        p->n[d] = dims[d];
+        p->ng[d] = 2;
+        dimp *= dims[d];
+        p->g[d] = eq[2 + d];
+    }
+    p->gamma = eq[0];
+    p->eta = eq[1];
+    p->mu = 1.0;
+    p->rkon = rk4;
+    p->moddton = moddton;
+    p->courant = courant;
+    p->hyperdifmom = hyperdif;
+    p->divbon = 0;
+    for (int i = 0; i < NVAR; i++) p->chyp[i] = 0.0;
+    p->chyp[rho] = chv[0];
+    p->chyp[energy] = chv[1];
+    for (int d = 0; d < NDIM; d++) {
+        p->chyp[mom1 + d] = chv[2];
+        p->chyp[b1 + d] = chv[3];
+    }
+    for (int ii = 0; ii < NVAR; ii++)
+        for (int idir = 0; idir < NDIM; idir++)
+            for (int ib = 0; ib < 2; ib++)
+                p->boundtype[ii][idir][ib] = bcset ? (int)bcv[idir < bcset ? idir : bcset - 1] : 4;
+
+    real *wmod = (real *)calloc((size_t)NVAR * dimp, sizeof(real));
+    real *wd = (real *)calloc((size_t)NDERV * dimp, sizeof(real));
+    char *hlines[5] = {0};
+    struct Meta meta;
+    memset(&meta, 0, sizeof(meta));
+    meta.name = (char *)"smaug";
+    CALL(readasciivacconfig((char *)in, *p, meta, st, wmod, wd, hlines, 0));
+    for (int d = 0; d < NDIM; d++) {
+        long stride = d == 0 ? 1 : (d == 1 ? dims[0] : (long)dims[0] * dims[1]);
+        p->dx[d] = 0.5 * (wd[(long)(pos1 + d) * dimp + 2 * stride] - wd[(long)(pos1 + d) * dimp]);
+        p->xmin[d] = wd[(long)(pos1 + d) * dimp];
+        p->xmax[d] = wd[(long)(pos1 + d) * dimp + (dims[d] - 1) * stride];
+    }
+    p->dt = dtarg > 0 ? dtarg : 1e-4;
+    p->qt = t0;
+    real dt = p->dt;
+
+    struct params *d_p;
+    struct bparams *d_bp;
+    struct state *d_state;
+    real *d_wnew, *d_wmod, *d_dwn1, *d_wd, *d_wtemp, *d_wtemp1, *d_wtemp2, *wnew = NULL;
+    CALL(cusetdriver(driver, drv));
+    CALL(cuinit(&p, &bp, &wmod, &wnew, &wd, &st, &d_p, &d_bp, &d_wnew, &d_wmod, &d_dwn1, &d_wd,
+                &d_state, &d_wtemp, &d_wtemp1, &d_wtemp2));
+    CALL(initgrid(&p, &st, &wd, &d_p, &d_dwn1, &d_wd, &d_state, &d_wtemp, &d_wtemp1, &d_wtemp2));
+
+    /* ghost fill of the initial state (iosmaug.cpp:783-791) */
+    for (int ii = 0; ii <= (b1 + (NDIM - 1)); ii++)
+        for (int idir = 0; idir < NDIM; idir++)
+            CALL(cuboundary(&p, &bp, &d_p, &d_bp, &d_state, &d_wmod, 1, idir, ii));
+
+    if (p->rkon == 1) {
+        CALL(cucomputedervfields(&p, &d_p, &d_wmod, &d_wd, 0));
+        CALL(cuadvancerk4(&p, &d_p, &d_wmod, nt, mixed));
+        p->it = nt;
+    }
+    for (int n = 1; p->rkon == 0 && n <= nt; n++) {
+        int order = 0, ordero = 1;
+        p->it = n;
+        CALL(cucomputedervfields(&p, &d_p, &d_wmod, &d_wd, 0));
+        if (p->moddton == 1) {
+            /* Courant and hyperviscous dt (iosmaug.cpp:855-898) */
+            p->maxcourant = 0.0;
+            for (int dim = 0; dim <= (NDIM - 1); dim++) {
+                CALL(cucomputec(&p, &d_p, &d_wmod, &d_wd, order, dim));
+                CALL(cucomputemaxc(&p, &d_p, &d_wmod, &d_wd, order, dim, &wd, &d_wtemp));
+                CALL(cucomputemaxcourant(&p, &d_p, &d_wmod, &d_wd, order, dim, &wd, &d_wtemp));
+            }
+            if (((p->courant) / (p->maxcourant)) > 1.0e-8) p->dt = (p->courant) / (p->maxcourant);
+            if (n > 1) CALL(cugetdtvisc1(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2));
+            if ((p->dtdiffvisc) > 1.0e-8 && (p->dt) > (p->dtdiffvisc)) p->dt = (p->dtdiffvisc);
+        }
+        p->qt = (p->qt) + dt;
+        for (int dir = 0; dir < NDIM; dir++) CALL(cucomputevels(&p, &d_p, &d_wmod, &d_wd, order, dir));
+        for (int dir = 0; dir < NDIM; dir++) {
+            for (int fld = rho; fld <= (mom1 + NDIM - 1); fld++) {
+                if (fld == mom1 + dir) {
+                    CALL(cucomputept(&p, &d_p, &d_wmod, &d_wd, order, dir));
+                    CALL(cucomputepbg(&p, &d_p, &d_wmod, &d_wd, order, dir));
+                }
+                CALL(cucentdiff1(&p, &d_p, &d_state, &d_wmod, &d_wmod, &d_dwn1, &d_wd, order, ordero,
+                                 p->dt, fld, dir));
+            }
+            for (int fld = energy; fld <= (b1 + (NDIM - 1)); fld++) {
+                if (fld == energy) {
+                    CALL(cucomputevels(&p, &d_p, &d_wmod, &d_wd, order, dir));
+                    CALL(cucomputepbg(&p, &d_p, &d_wmod, &d_wd, ordero, dir));
+                    CALL(cucomputept(&p, &d_p, &d_wmod, &d_wd, order, dir));
+                }
+                CALL(cucentdiff2(&p, &d_p, &d_state, &d_wmod, &d_wmod, &d_dwn1, &d_wd, order, ordero,
+                                 p->dt, fld, dir));
+            }
+        }
+        CALL(cugrav(&p, &d_p, &d_state, &d_wmod, &d_wmod, &d_dwn1, &d_wd, order, ordero, p->dt));
+        if (p->divbon == 1) CALL(cudivb(&p, &d_p, &d_wmod, &d_wmod, &d_dwn1, &d_wd, order, ordero, p->dt));
+        if (p->hyperdifmom == 1) {
+            real cmax[NDIM];
+            p->maxviscoef = 0.0;
+            for (int dim = 0; dim <= (NDIM - 1); dim++) {
+                CALL(cucomputec(&p, &d_p, &d_wmod, &d_wd, order, dim));
+                CALL(cucomputemaxc(&p, &d_p, &d_wmod, &d_wd, order, dim, &wd, &d_wtemp));
+                cmax[dim] = p->cmax;
+                CALL(cuhyperdifvisc1ir(&p, &d_p, &d_wmod, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, rho, dim));
+                CALL(cuhyperdifvisc1r(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, rho, dim));
+                CALL(cuhyperdifvisc1l(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, rho, dim));
+                CALL(cuhyperdifrhosource1(&p, &d_p, &d_wmod, &d_dwn1, &d_wd, order, ordero, &d_wtemp, rho, dim, p->dt));
+            }
+            for (int dim = 0; dim <= (NDIM - 1); dim++) {
+                p->cmax = cmax[dim];
+                CALL(cuhyperdifvisc1ir(&p, &d_p, &d_wmod, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, energy, dim));
+                CALL(cuhyperdifvisc1r(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, energy, dim));
+                CALL(cuhyperdifvisc1l(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, energy, dim));
+                CALL(cuhyperdifesource1(&p, &d_p, &d_wmod, &d_dwn1, &d_wd, order, ordero, &d_wtemp, energy, dim, dt));
+            }
+            for (int dim = 0; dim <= (NDIM - 1); dim++)
+                for (int fi = 0; fi <= (NDIM - 1); fi++) {
+                    p->cmax = cmax[dim];
+                    CALL(cuhyperdifvisc1ir(&p, &d_p, &d_wmod, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, mom1 + fi, dim));
+                    CALL(cuhyperdifvisc1r(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, mom1 + fi, dim));
+                    CALL(cuhyperdifvisc1l(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, mom1 + fi, dim));
+                    for (int ii1 = 0; ii1 <= 1; ii1++) {
+                        int ii = ii1 == 0 ? dim : fi;
+                        int ii0 = ii1 == 0 ? fi : dim;
+                        if (ii == dim)
+                            CALL(cuhyperdifmomsource1(&p, &d_p, &d_wmod, &d_dwn1, &d_wd, order, ordero, &d_wtemp, fi, dim, ii, ii0, p->dt));
+                        else
+                            CALL(cuhyperdifmomsourcene1(&p, &d_p, &d_wmod, &d_dwn1, &d_wd, order, ordero, &d_wtemp, fi, dim, ii, ii0, p->dt));
+                    }
+                }
+            for (int dim = 0; dim <= (NDIM - 1); dim++)
+                for (int fi = 0; fi <= (NDIM - 1); fi++) {
+                    if (fi == dim) continue;
+                    p->cmax = cmax[dim];
+                    CALL(cuhyperdifvisc1ir(&p, &d_p, &d_wmod, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, b1 + fi, dim));
+                    CALL(cuhyperdifvisc1r(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, b1 + fi, dim));
+                    CALL(cuhyperdifvisc1l(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, b1 + fi, dim));
+                    for (int ii1 = 0; ii1 <= 1; ii1++) {
+                        int jj, mm, ii0;
+                        real sb;
+                        if (ii1 == 0) {
+                            jj = dim;
+                            mm = fi;
+                            sb = -1.0;
+                            ii0 = dim;
+                        } else {
+                            ii0 = fi;
+                            mm = dim;
+                            sb = 1.0;
+                            jj = fi;
+                        }
+                        if (mm == dim)
+                            CALL(cuhyperdifbsource1(&p, &d_p, &d_wmod, &d_dwn1, &d_wd, order, ordero, &d_wtemp, fi, dim, jj, ii0, mm, sb, p->dt));
+                        else
+                            CALL(cuhyperdifbsourcene1(&p, &d_p, &d_wmod, &d_dwn1, &d_wd, order, ordero, &d_wtemp, fi, dim, jj, ii0, mm, sb, p->dt));
+                    }
+                }
+        }
+        CALL(cusource(&p, &d_p, &d_state, &d_wmod, &d_wmod, &d_dwn1, &d_wd, order, ordero, p->dt));
+        for (int ii = 0; ii <= (b1 + (NDIM - 1)); ii++)
+            for (int idir = 0; idir < NDIM; idir++)
+                CALL(cuboundary(&p, &bp, &d_p, &d_bp, &d_state, &d_wmod, ordero, idir, ii));
+    }
+    /* the newest state is in slot 1 after an onestep loop: bring it to slot 0 */
+    if (p->rkon == 0) CALL(cucomputedervfields(&p, &d_p, &d_wmod, &d_wd, 0));
+    CALL(cuupdate(&p, &wmod, &wmod, &d_wtemp2, &st, &d_p, &d_wmod, &d_wmod, &d_wtemp2, &d_state, nt));
+    st->it = p->it;
+    st->t = p->qt;
+    printf("smaug_main: %d steps, t = %.10g, dt = %.10g\n", nt, p->qt, p->dt);
+    if (out) CALL(writeasciivacconfig((char *)out, *p, meta, wmod, wd, hlines, *st, 0));
+    CALL(cufinish(&p, &wmod, &wnew, &st, &d_p, &d_bp, &d_wmod, &d_wnew, &d_wmod, &d_dwn1, &d_wd,
+                  &d_state, &d_wtemp, &d_wtemp1, &d_wtemp2));
+    return 0;
+}

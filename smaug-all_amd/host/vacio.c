@@ -132,14 +132,14 @@ int writeasciivacconfig(char *cfgfile, params p, meta md, real *w, real *wd, cha
     int ne;
     eqpar(&p, e, &ne);
     fprintf(f, "%s\n", md.name ? md.name : "smaug");
-    fprintf(f, "%7d %12.5E %2d %2d %2d\n", st.it, st.t, NDIM, ne, NVAR);
+    fprintf(f, "%7d %23.16E %2d %2d %2d\n", st.it, st.t, NDIM, ne, NVAR);
     for (int d = 0; d < NDIM; d++) fprintf(f, "%4d%s", p.n[d], d + 1 < NDIM ? " " : "\n");
-    for (int k = 0; k < ne; k++) fprintf(f, " %12.5E", e[k]);
+    for (int k = 0; k < ne; k++) fprintf(f, " %23.16E", e[k]);
     fprintf(f, "\n%s\n", VARNAMES);
     long nc = ncells(&p);
     for (long c = 0; c < nc; c++) {
-        for (int d = 0; d < NDIM; d++) fprintf(f, " %22.15E", wd[(long)(pos1 + d) * nc + c]);
-        for (int v = 0; v < NVAR; v++) fprintf(f, " %22.15E", w[(long)v * nc + c]);
+        for (int d = 0; d < NDIM; d++) fprintf(f, " %24.16E", wd[(long)(pos1 + d) * nc + c]);
+        for (int v = 0; v < NVAR; v++) fprintf(f, " %24.16E", w[(long)v * nc + c]);
         fprintf(f, "\n");
     }
     return fclose(f) ? 2 : 0;

@@ -198,7 +198,10 @@ def stratified(nz=256, nx=256, ny=256, lz=6.0e6, lx=4.0e6, ly=4.0e6, bz_tesla=0.
     b_b = B/sqrt(mu0) (b^2/2 is the magnetic pressure in code units);
     perturbations start at 0; driver v_z = A sin(2 pi t/P)
     exp(-r^2/sigma_h^2) exp(-(z-z_drv)^2/sigma_z^2) centred horizontally.
-    dt is set to courant / max(cfast_d/dx_d) of the initial state."""
+    dt is the smaller of courant / max(cfast_d/dx_d) of the initial state and
+    the hyperviscous limit 0.25 dx^2 / nu_max (SAC getdt_visc,
+    vacusr.viscosity.t:512-546) with nu_max = dx cmax max(chyp) R_max and the
+    bound R_max = 6 of the hyperviscosity ratio max|D3|/max|D1|."""
     D = 3
     pz, dz = _axis(nz, 0.0, lz)
     px, dxx = _axis(nx, 0.0, lx)
@@ -218,8 +221,52 @@ def stratified(nz=256, nx=256, ny=256, lz=6.0e6, lx=4.0e6, ly=4.0e6, bz_tesla=0.
     s = Setup("stratified", D, n, (4, 4, 4), gamma, (-274.0, 0.0, 0.0), _chyp(D), 1, 1, drv,
               [dz, dxx, dxy], [pz, px, py], w, bg, nsteps=1000,
               meta={"courant": courant, "bz_code": bb})
-    s.dt = courant / initial_max_courant(s)
+    s.dt = min(courant / initial_max_courant(s), hyperviscous_dt(s))
     return s
+
+
+def hyperviscous_dt(s: Setup, rmax: float = 6.0, dtdiffpar: float = 0.25) -> float:
+    """0.25 dx^2/nu_max with nu_max = dx cmax_d max(chyp) rmax, over dims,
+    for the set-up's initial state (cmax_d from the same fast speed as
+    initial_max_courant)"""
+    best = float("inf")
+    cm = initial_cmax(s)
+    ch = float(np.max(s.chyp))
+    for d in range(s.ndim):
+        dxm = float(np.min(s.dx[d]))
+        nu = dxm * cm[d] * ch * rmax
+        if nu > 0:
+            best = min(best, dtdiffpar * dxm * dxm / nu)
+    return best
+
+
+def initial_cmax(s: Setup):
+    """cmax_d over all cells of the set-up's state"""
+    return [float(np.max(c)) for c in _cfast(s)]
+
+
+def _cfast(s: Setup):
+    D = s.ndim
+    w, bg = s.w, s.bg
+    gam = s.gamma
+    rho = w[0] + bg[1]
+    m = [w[1 + d] for d in range(D)]
+    b = [w[2 + D + d] for d in range(D)]
+    B = [bg[2 + d] for d in range(D)]
+    e = w[1 + D]
+    mm = sum(x * x for x in m)
+    bb = sum(x * x for x in b)
+    bgb = sum(B[d] * b[d] for d in range(D))
+    bbb = sum(x * x for x in B)
+    pk = (gam - 1) * (e - 0.5 * (mm / rho) - 0.5 * bb - bgb)
+    cs2 = (gam * (pk + (gam - 1) * (bg[0] - 0.5 * bbb))) / rho
+    cf2 = ((bb + bbb + 2.0 * bgb) / rho) + cs2
+    out = []
+    for d in range(D):
+        btd = B[d] + b[d]
+        cfi = 0.5 * (cf2 + np.sqrt(cf2 * cf2 - 4.0 * cs2 * (btd * btd) / rho))
+        out.append(np.sqrt(cfi) + np.abs(m[d] / rho))
+    return out
 
 
 def initial_max_courant(s: Setup) -> float:
