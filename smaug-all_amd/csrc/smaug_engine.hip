@@ -46,6 +46,7 @@ struct smaug_engine {
     void *U[3];   // U[0] = current state, U[1], U[2] stage buffers
     void *ACC;
     void *R;      // hyperviscosity ratios
+    void *AUX;    // per-cell 1/rho, pressure, internal energy, velocity
     double *bg;
     double *d_dx[3], *d_idx[3], *d_prof[3];
     unsigned long long *d_cmax;
@@ -65,8 +66,8 @@ static int launch_cmax(smaug_engine *e, const void *u)
     hipLaunchKernelGGL(k_reset_cmax, dim3(1), dim3(64), 0, e->stream, e->d_cmax);
     idx_t nb = (e->nc + 255) / 256;
     if (nb > 4096) nb = 4096;
-    hipLaunchKernelGGL((k_cmax<D, T>), dim3((unsigned)nb), dim3(256), 0, e->stream, e->geo,
-                       e->phys, (const T *)u, (const double *)e->bg, e->d_cmax);
+    hipLaunchKernelGGL((k_prep<D, T>), dim3((unsigned)nb), dim3(256), 0, e->stream, e->geo,
+                       e->phys, (const T *)u, (const double *)e->bg, (T *)e->AUX, e->d_cmax);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -95,7 +96,7 @@ static int launch_ratio(smaug_engine *e, const void *u)
     unsigned nb;
     TileMap m = make_map(e->n[0] - 3, e->n[1] - 3, D == 3 ? e->n[2] - 3 : 1, &nb); // box [1, n-3]
     hipLaunchKernelGGL((k_ratio<D, T>), dim3(nb), dim3(256), 0, e->stream, e->geo, m, (const T *)u,
-                       (const double *)e->bg, (T *)e->R);
+                       (const T *)e->AUX, (T *)e->R);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -108,7 +109,7 @@ static int launch_update(smaug_engine *e, StageArgs sa, const void *uin, const v
     TileMap m = make_map(e->n[0] - 4, e->n[1] - 4, D == 3 ? e->n[2] - 4 : 1, &nb);
     hipLaunchKernelGGL((k_update<D, T>), dim3(nb), dim3(256), 0, e->stream, e->geo, m, e->phys, sa,
                        (const T *)uin, (const T *)un, (T *)uout, (T *)e->ACC,
-                       (const T *)e->R, (const double *)e->bg,
+                       (const T *)e->R, (const T *)e->AUX, (const double *)e->bg,
                        (const unsigned long long *)e->d_cmax);
     HIPCHK(hipGetLastError());
     return 0;
@@ -136,10 +137,9 @@ static int stage(smaug_engine *e, StageArgs sa, const void *uin, const void *un,
                  hipEvent_t ev0, hipEvent_t ev1)
 {
     int rc;
-    if (e->phys.hyperdif) {
-        if ((rc = launch_cmax<D, T>(e, uin))) return rc;
+    if ((rc = launch_cmax<D, T>(e, uin))) return rc;
+    if (e->phys.hyperdif)
         if ((rc = launch_ratio<D, T>(e, uin))) return rc;
-    }
     if (ev0) HIPCHK(hipEventRecord(ev0, e->stream));
     if ((rc = launch_update<D, T>(e, sa, uin, un, uout))) return rc;
     if (ev1) HIPCHK(hipEventRecord(ev1, e->stream));
@@ -216,6 +216,7 @@ static void free_engine(smaug_engine *e)
     }
     if (e->ACC) hipFree(e->ACC);
     if (e->R) hipFree(e->R);
+    if (e->AUX) hipFree(e->AUX);
     if (e->bg) hipFree(e->bg);
     if (e->d_cmax) hipFree(e->d_cmax);
     if (e->d_tm) hipFree(e->d_tm);
@@ -274,6 +275,7 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
     for (int k = 0; k < 3; k++) TRYALLOC(e->U[k], e->tsz * nve * e->nc);
     TRYALLOC(e->ACC, e->tsz * nve * e->nc);
     if (cfg->hyperdif) TRYALLOC(e->R, e->tsz * npair * e->nc);
+    TRYALLOC(e->AUX, e->tsz * (3 + D) * e->nc);
     TRYALLOC(e->bg, sizeof(double) * nbg * e->nc);
     TRYALLOC(e->d_cmax, sizeof(unsigned long long) * 4);
     TRYALLOC(e->d_tm, sizeof(double) * 2);

@@ -1,9 +1,11 @@
 // smaug_kernels.hpp -- fused per-stage kernels of the MI355X path.
 //
 // One RK stage of the semi-discrete operator L(w) is four launches:
-//   k_cmax    : cmax_d = max over all cells of cfast_d (wavefront-shuffle
-//               max, one 64-bit atomic max per block); replaces
-//               cucomputec + cucomputemaxc (computedervfields.cu:1922-2116)
+//   k_prep    : per-cell 1/rho, pressure perturbation, internal energy and
+//               velocity (double arithmetic) into an aux buffer, and cmax_d =
+//               max over all cells of cfast_d (wavefront-shuffle max, one
+//               64-bit atomic max per block); replaces cucomputec +
+//               cucomputemaxc (computedervfields.cu:1922-2116)
 //   k_ratio   : per cell the hyperviscosity ratio max|D3|/max|D1| of every
 //               (q, dim) pair; replaces the hyperdifvisc1r/1l kernel chains
 //   k_update  : per interior cell the whole of L(w) -- 4th-order flux
@@ -34,12 +36,25 @@ __global__ void k_reset_cmax(unsigned long long *cm)
     if (threadIdx.x < 3) cm[threadIdx.x] = 0ull;
 }
 
+// Per-cell auxiliary quantities of a state, formed once per stage and read
+// by the ratio and update kernels instead of being recomputed at every
+// neighbour visit: aux[0] = 1/rho, aux[1] = total-pressure perturbation
+// (double arithmetic, computept3), aux[2] = e' - (b'^2 + m^2/rho)/2,
+// aux[3+d] = m_d/rho.  The same pass forms cmax_d = max over every cell of
+// cfast_d (cucomputec + cucomputemaxc, computedervfields.cu:1922-2116) with a
+// wavefront-shuffle max and one 64-bit atomic max per block.
+template <int D>
+struct Aux {
+    static constexpr int IRHO = 0, PT = 1, EINT = 2, V0 = 3, N = 3 + D;
+};
+
 // blockDim.x == 256; grid-stride over every cell (ghosts included)
 template <int D, typename T>
-__global__ __launch_bounds__(256) void k_cmax(Geo g, Phys ph, const T *__restrict__ u,
+__global__ __launch_bounds__(256) void k_prep(Geo g, Phys ph, const T *__restrict__ u,
                                                const double *__restrict__ bg,
-                                               unsigned long long *cm)
+                                               T *__restrict__ aux, unsigned long long *cm)
 {
+    typedef Aux<D> AX;
     __shared__ double red[D][4];
     double mx[D];
 #pragma unroll
@@ -48,6 +63,13 @@ __global__ __launch_bounds__(256) void k_cmax(Geo g, Phys ph, const T *__restric
          c += (idx_t)gridDim.x * blockDim.x) {
         St<D, T> s;
         load_st<D, T>(s, u, bg, g.nc, c);
+        const double rho = rho_full(s);
+        const double ir = 1.0 / rho;
+        aux[(idx_t)AX::IRHO * g.nc + c] = (T)ir;
+        aux[(idx_t)AX::PT * g.nc + c] = (T)ptot_pert<D, T>(s, rho, ph.gamma);
+        aux[(idx_t)AX::EINT * g.nc + c] = (T)e_int<D, T>(s, rho);
+#pragma unroll
+        for (int d = 0; d < D; d++) aux[(idx_t)(AX::V0 + d) * g.nc + c] = (T)((double)s.m[d] * ir);
 #pragma unroll
         for (int d = 0; d < D; d++) {
             double v = cfast<D, T>(s, d, ph.gamma);
@@ -98,9 +120,10 @@ __device__ __forceinline__ T ratio6(const T q[6])
 
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_ratio(Geo g, TileMap tm, const T *__restrict__ u,
-                                                const double *__restrict__ bg, T *__restrict__ R)
+                                                const T *__restrict__ aux, T *__restrict__ R)
 {
     typedef Lay<D> LY;
+    typedef Aux<D> AX;
     int tx, ty, tk;
     if (!map_block(tm, tx, ty, tk)) return;
     int x0 = 1 + tx * 64 + (threadIdx.x & 63);
@@ -108,40 +131,37 @@ __global__ __launch_bounds__(256) void k_ratio(Geo g, TileMap tm, const T *__res
     int x2 = (D == 3) ? 1 + tk : 0;
     if (x0 > g.n[0] - 3 || x1 > g.n[1] - 3) return;
     int xs[3] = {x0, x1, x2};
+    const idx_t nc = g.nc;
     idx_t c = (idx_t)x0 + g.s1 * x1 + g.s2 * x2;
 #pragma unroll
     for (int d = 0; d < D; d++) {
         int x = xs[d], n = g.n[d];
         idx_t sd = stride<D>(g, d);
         idx_t c0 = c - (idx_t)x * sd;
-        // the differences and the ratio are formed in double in both paths:
-        // third differences of nearly smooth FP32 data sit near the FP32
-        // rounding level, and the ratio is the quantity that amplifies it
+        // differences and ratios in double in both paths: third differences
+        // of nearly smooth FP32 data sit near the FP32 rounding level
         double qr[6], qe[6], qv[D][6], qb[D][6];
 #pragma unroll
         for (int p = 0; p < 6; p++) {
             int xx = x - 2 + p;
             if (xx < 0) xx = 5;
             if (xx >= n) xx = n - 6;
-            St<D, T> s;
-            load_light<D, T>(s, u, bg, g.nc, c0 + (idx_t)xx * sd);
-            double rho = rho_full(s);
-            qr[p] = s.r;
-            qe[p] = e_int<D, T>(s, rho);
-            double ir = 1.0 / rho;
+            const idx_t cp = c0 + (idx_t)xx * sd;
+            qr[p] = u[cp];
+            qe[p] = aux[(idx_t)AX::EINT * nc + cp];
 #pragma unroll
             for (int f = 0; f < D; f++) {
-                qv[f][p] = (double)s.m[f] * ir;
-                qb[f][p] = s.b[f];
+                qv[f][p] = aux[(idx_t)(AX::V0 + f) * nc + cp];
+                qb[f][p] = u[(idx_t)(LY::B0 + f) * nc + cp];
             }
         }
-        R[(idx_t)LY::prho(d) * g.nc + c] = (T)ratio6<double>(qr);
-        R[(idx_t)LY::pe(d) * g.nc + c] = (T)ratio6<double>(qe);
+        R[(idx_t)LY::prho(d) * nc + c] = (T)ratio6<double>(qr);
+        R[(idx_t)LY::pe(d) * nc + c] = (T)ratio6<double>(qe);
 #pragma unroll
-        for (int f = 0; f < D; f++) R[(idx_t)LY::pv(f, d) * g.nc + c] = (T)ratio6<double>(qv[f]);
+        for (int f = 0; f < D; f++) R[(idx_t)LY::pv(f, d) * nc + c] = (T)ratio6<double>(qv[f]);
 #pragma unroll
         for (int f = 0; f < D; f++)
-            if (f != d) R[(idx_t)LY::pb(f, d) * g.nc + c] = (T)ratio6<double>(qb[f]);
+            if (f != d) R[(idx_t)LY::pb(f, d) * nc + c] = (T)ratio6<double>(qb[f]);
     }
 }
 
@@ -210,23 +230,18 @@ struct StageArgs {
     const double *tm; // device [t, dt]
 };
 
-// 4th-order divergence, grad3dn form ((8(q1-qm1) + qm2 - q2)/6)/(2 dx)
-template <typename T>
-__device__ __forceinline__ T div4(T qm2, T qm1, T q1, T q2, T i12dx)
-{
-    return ((T)8 * (q1 - qm1) + (qm2 - q2)) * i12dx;
-}
-
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, StageArgs sa,
                                                  const T *__restrict__ uin,
                                                  const T *__restrict__ un,
                                                  T *__restrict__ uout, T *__restrict__ acc,
                                                  const T *__restrict__ R,
+                                                 const T *__restrict__ aux,
                                                  const double *__restrict__ bg,
                                                  const unsigned long long *__restrict__ cmb)
 {
     typedef Lay<D> LY;
+    typedef Aux<D> AX;
     int tx, ty, tk;
     if (!map_block(tm, tx, ty, tk)) return;
     const int x0 = 2 + tx * 64 + (threadIdx.x & 63);
@@ -237,26 +252,33 @@ __global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, Stag
     const idx_t nc = g.nc;
     const idx_t c = (idx_t)x0 + g.s1 * x1 + g.s2 * x2;
     const double gam = ph.gamma;
+    // field and aux accessors
+    auto U = [&](int f, idx_t cc) -> T { return uin[(idx_t)f * nc + cc]; };
+    auto A = [&](int a, idx_t cc) -> T { return aux[(idx_t)a * nc + cc]; };
 
     T dU[LY::NVE];
 #pragma unroll
     for (int f = 0; f < LY::NVE; f++) dU[f] = 0;
 
-    St<D, T> sc;
-    load_st<D, T>(sc, uin, bg, nc, c);
-    const double rhoc = rho_full(sc);
-    const T ptb = (T)ptot_bg<D, T>(sc, gam);
+    const double rbc = bg[nc + c];
+    T bbc[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) bbc[d] = (T)bg[(idx_t)(2 + d) * nc + c];
+    double bbb = 0;
+#pragma unroll
+    for (int d = 0; d < D; d++) bbb += bg[(idx_t)(2 + d) * nc + c] * bg[(idx_t)(2 + d) * nc + c];
+    // background total pressure (computepbg3)
+    const T ptb = (T)((gam - 1) * bg[c] - 0.5 * (gam - 2) * bbb);
 
     // ---- advection: 4th-order divergence of the fluxes along each axis a,
-    // plus the background work terms of the energy equation
-    // (centdiff1.cu:19-256, centdiff2.cu:20-356)
+    // accumulated neighbour by neighbour, plus the background work terms of
+    // the energy equation (centdiff1.cu:19-256, centdiff2.cu:20-356)
 #pragma unroll
     for (int a = 0; a < D; a++) {
         const idx_t sa_ = stride<D>(g, a);
         const int xa = xs[a], na = g.n[a];
         if (zero4(g.per[a], xa, na)) continue;
-        // d/dx_a q = (q[-2] - 8 q[-1] + 8 q[+1] - q[+2]) / (12 dx), accumulated
-        // neighbour by neighbour
+        // d/dx_a q = (q[-2] - 8 q[-1] + 8 q[+1] - q[+2]) / (12 dx)
         const T i12 = (T)(1.0 / (12.0 * g.dx[a][xa]));
         T gv[D];
 #pragma unroll
@@ -265,52 +287,58 @@ __global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, Stag
         for (int p = 0; p < 4; p++) {
             const int off = (p < 2) ? p - 2 : p - 1; // -2,-1,+1,+2
             const T cw = (p == 0 ? (T)1 : (p == 1 ? (T)-8 : (p == 2 ? (T)8 : (T)-1))) * i12;
-            St<D, T> s;
-            load_st<D, T>(s, uin, bg, nc, c + (idx_t)off * sa_);
-            const double rho = rho_full(s);
-            const T pt = (T)ptot_pert<D, T>(s, rho, gam);
-            const T ir = (T)(1.0 / rho);
-            const T ma = s.m[a];
+            const idx_t cn = c + (idx_t)off * sa_;
+            T m[D], b[D], bb[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                m[d] = U(LY::M0 + d, cn);
+                b[d] = U(LY::B0 + d, cn);
+                bb[d] = (T)bg[(idx_t)(2 + d) * nc + cn];
+            }
+            const T e = U(LY::EN, cn);
+            const T eb = (T)bg[cn];
+            const T ir = A(AX::IRHO, cn);
+            const T pt = A(AX::PT, cn);
+            const T ma = m[a];
             T bbm = 0, bm = 0;
 #pragma unroll
             for (int t = 0; t < D; t++) {
-                bbm += (T)s.bb[t] * s.m[t];
-                bm += s.b[t] * s.m[t];
+                bbm += bb[t] * m[t];
+                bm += b[t] * m[t];
             }
             // F_rho = m_a rho'/rho + rho_b m_a/rho = m_a
             dU[LY::RHO] -= cw * ma;
 #pragma unroll
             for (int i = 0; i < D; i++) {
-                T mag = -(s.b[i] * (T)s.bb[a] + (T)s.bb[i] * s.b[a]) - s.b[i] * s.b[a];
-                dU[LY::M0 + i] -= cw * (ma * s.m[i] * ir + ((i == a) ? pt + mag : mag));
+                const T mag = -(b[i] * bb[a] + bb[i] * b[a]) - b[i] * b[a];
+                dU[LY::M0 + i] -= cw * (ma * m[i] * ir + ((i == a) ? pt + mag : mag));
             }
-            dU[LY::EN] -= cw * ((ma * (s.e + (T)s.eb + pt) - s.b[a] * bbm - (T)s.bb[a] * bm -
-                                 s.b[a] * bm) * ir);
+            dU[LY::EN] -= cw * ((ma * (e + eb + pt) - b[a] * bbm - bb[a] * bm - b[a] * bm) * ir);
 #pragma unroll
             for (int i = 0; i < D; i++) {
                 if (i == a) continue;
-                dU[LY::B0 + i] -= cw * ((ma * s.b[i] - (s.b[a] + (T)s.bb[a]) * s.m[i] +
-                                         (T)s.bb[i] * ma) * ir);
+                dU[LY::B0 + i] -= cw * ((ma * b[i] - (b[a] + bb[a]) * m[i] + bb[i] * ma) * ir);
             }
 #pragma unroll
-            for (int d = 0; d < D; d++) gv[d] += cw * (s.m[d] * ir);
+            for (int d = 0; d < D; d++) gv[d] += cw * A(AX::V0 + d, cn);
         }
         // -p_Tb dv_a/dx_a + sum_d b_ba b_bd dv_d/dx_a  (addenergyterms_cd2)
         T w = -ptb * gv[a];
 #pragma unroll
-        for (int d = 0; d < D; d++) w += (T)sc.bb[a] * (T)sc.bb[d] * gv[d];
+        for (int d = 0; d < D; d++) w += bbc[a] * bbc[d] * gv[d];
         dU[LY::EN] += w;
     }
 
+    const T rc = U(LY::RHO, c);
     // ---- gravity (addgrav_cd2, centdiff2.cu:118-154)
     {
         T ge = 0;
 #pragma unroll
         for (int d = 0; d < D; d++) {
-            dU[LY::M0 + d] += (T)ph.g[d] * sc.r;
-            ge += (T)ph.g[d] * sc.m[d];
+            dU[LY::M0 + d] += (T)ph.g[d] * rc;
+            ge += (T)ph.g[d] * U(LY::M0 + d, c);
         }
-        dU[LY::EN] += sc.r * ge * (T)(1.0 / rhoc);
+        dU[LY::EN] += rc * ge * A(AX::IRHO, c);
     }
 
     // ---- hyperdiffusion (iosmaug.cpp:969-1146)
@@ -318,11 +346,8 @@ __global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, Stag
         double cmax[D];
 #pragma unroll
         for (int d = 0; d < D; d++) cmax[d] = __longlong_as_double((long long)cmb[d]);
-        const T eic = (T)e_int<D, T>(sc, rhoc);
-        const T irc = (T)(1.0 / rhoc);
-        T vc[D];
-#pragma unroll
-        for (int f = 0; f < D; f++) vc[f] = sc.m[f] * irc;
+        const T eic = A(AX::EINT, c);
+        const double rhoc = (double)rc + rbc;
 
 #pragma unroll
         for (int d = 0; d < D; d++) {
@@ -331,42 +356,36 @@ __global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, Stag
             const double dxd = g.dx[d][xd];
             const T idxd = (T)g.idx[d][xd];
             const bool z1 = zero1(g.per[d], xd, nd);
-            St<D, T> sl, sr;
-            load_light<D, T>(sl, uin, bg, nc, c - sd);
-            load_light<D, T>(sr, uin, bg, nc, c + sd);
-            const double rhol = rho_full(sl), rhor = rho_full(sr);
-            const T irl = (T)(1.0 / rhol), irr = (T)(1.0 / rhor);
+            const idx_t cl = c - sd, cr = c + sd;
 
-            // diagonal terms: q += (nu_R dq_R - nu_L dq_L)/dx
             auto nuRL = [&](int pair, int field, T &nR, T &nL) {
                 const T K = (T)(dxd * cmax[d] * ph.chyp[field]);
                 nR = K * R[(idx_t)pair * nc + c];
-                nL = K * R[(idx_t)pair * nc + c - sd];
+                nL = K * R[(idx_t)pair * nc + cl];
             };
             T nR, nL, gR, gL;
             // density (hyperdifrhosource1.cu:15-120)
             nuRL(LY::prho(d), LY::RHO, nR, nL);
-            gR = z1 ? (T)0 : (sr.r - sc.r) * idxd;
-            gL = z1 ? (T)0 : (sc.r - sl.r) * idxd;
+            gR = z1 ? (T)0 : (U(LY::RHO, cr) - rc) * idxd;
+            gL = z1 ? (T)0 : (rc - U(LY::RHO, cl)) * idxd;
             dU[LY::RHO] += (nR * gR - nL * gL) * idxd;
             // energy (hyperdifesource1.cu:18-330)
             nuRL(LY::pe(d), LY::EN, nR, nL);
-            {
-                const T el = (T)e_int<D, T>(sl, rhol), er = (T)e_int<D, T>(sr, rhor);
-                gR = z1 ? (T)0 : (er - eic) * idxd;
-                gL = z1 ? (T)0 : (eic - el) * idxd;
-                dU[LY::EN] += (nR * gR - nL * gL) * idxd;
-            }
+            gR = z1 ? (T)0 : (A(AX::EINT, cr) - eic) * idxd;
+            gL = z1 ? (T)0 : (eic - A(AX::EINT, cl)) * idxd;
+            dU[LY::EN] += (nR * gR - nL * gL) * idxd;
             // momentum, diagonal part for every f; twice when f == d
             // (hyperdifmomsource1.cu:17-533 called for ii1 = 0 and, if f == d, ii1 = 1)
+            const T rR = (T)((rhoc + (double)U(LY::RHO, cr) + bg[nc + cr]) / 2);
+            const T rL = (T)((rhoc + (double)U(LY::RHO, cl) + bg[nc + cl]) / 2);
 #pragma unroll
             for (int f = 0; f < D; f++) {
                 nuRL(LY::pv(f, d), LY::M0 + f, nR, nL);
-                const T vl = sl.m[f] * irl, vr = sr.m[f] * irr;
-                gR = z1 ? (T)0 : (vr - vc[f]) * idxd;
-                gL = z1 ? (T)0 : (vc[f] - vl) * idxd;
-                const T rR = (T)((rhoc + rhor) / 2), rL = (T)((rhoc + rhol) / 2);
-                const T mR = (sc.m[f] + sr.m[f]) / (T)2, mL = (sc.m[f] + sl.m[f]) / (T)2;
+                const T vc = A(AX::V0 + f, c);
+                gR = z1 ? (T)0 : (A(AX::V0 + f, cr) - vc) * idxd;
+                gL = z1 ? (T)0 : (vc - A(AX::V0 + f, cl)) * idxd;
+                const T mc = U(LY::M0 + f, c);
+                const T mR = (mc + U(LY::M0 + f, cr)) / (T)2, mL = (mc + U(LY::M0 + f, cl)) / (T)2;
                 const T k2 = (f == d) ? (T)1 : (T)0.5;
                 dU[LY::M0 + f] += k2 * (rR * nR * gR - rL * nL * gL) * idxd;
                 dU[LY::EN] += k2 * (mR * nR * gR - mL * nL * gL) * idxd;
@@ -376,20 +395,22 @@ __global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, Stag
             for (int f = 0; f < D; f++) {
                 if (f == d) continue;
                 nuRL(LY::pb(f, d), LY::B0 + f, nR, nL);
-                gR = z1 ? (T)0 : (sr.b[f] - sc.b[f]) * idxd;
-                gL = z1 ? (T)0 : (sc.b[f] - sl.b[f]) * idxd;
-                const T bR = (sc.b[f] + sr.b[f]) / (T)2, bL = (sc.b[f] + sl.b[f]) / (T)2;
+                const T bc_ = U(LY::B0 + f, c), br = U(LY::B0 + f, cr), bl = U(LY::B0 + f, cl);
+                gR = z1 ? (T)0 : (br - bc_) * idxd;
+                gL = z1 ? (T)0 : (bc_ - bl) * idxd;
+                const T bR = (bc_ + br) / (T)2, bL = (bc_ + bl) / (T)2;
                 dU[LY::B0 + f] += (gR * nR - gL * nL) * idxd;
                 dU[LY::EN] += (bR * gR * nR - bL * gL * nL) * idxd;
             }
 
-            // off-diagonal terms for f != d: d/dx_f of a product evaluated at
+            // off-diagonal terms for f != d: d/dx_f of products evaluated at
             // c -/+ e_f (hyperdifmomsourcene1.cu:16-740, hyperdifbsourcene1.cu:18-820)
 #pragma unroll
             for (int f = 0; f < D; f++) {
                 if (f == d) continue;
                 const idx_t sf = stride<D>(g, f);
                 const int xf = xs[f], nf = g.n[f];
+                if (zero1(g.per[f], xf, nf)) continue;
                 T Xr[2], Xm[2], Hb[2], He[2];
 #pragma unroll
                 for (int side = 0; side < 2; side++) {
@@ -398,26 +419,17 @@ __global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, Stag
                     Xr[side] = Xm[side] = Hb[side] = He[side] = 0;
                     const int xfn = xf + sg;
                     if (xfn < 2 || xfn > nf - 3) continue; // nu vanishes off the interior
-                    St<D, T> sn;
-                    load_light<D, T>(sn, uin, bg, nc, cn);
-                    const double rhon = rho_full(sn);
-                    // v_f and b_f at cn -/+ e_d
                     const idx_t ca = cn - sd, cb = cn + sd;
-                    const double rha = (double)uin[ca] + bg[nc + ca];
-                    const double rhb = (double)uin[cb] + bg[nc + cb];
-                    const T vfa = uin[(idx_t)(LY::M0 + f) * nc + ca] * (T)(1.0 / rha);
-                    const T vfb = uin[(idx_t)(LY::M0 + f) * nc + cb] * (T)(1.0 / rhb);
-                    const T bfa = uin[(idx_t)(LY::B0 + f) * nc + ca];
-                    const T bfb = uin[(idx_t)(LY::B0 + f) * nc + cb];
-                    const T dv = z1 ? (T)0 : (vfb - vfa) * idxd / (T)2;
-                    const T db = z1 ? (T)0 : (bfb - bfa) * idxd / (T)2;
+                    const T dv = z1 ? (T)0 : (A(AX::V0 + f, cb) - A(AX::V0 + f, ca)) * idxd / (T)2;
+                    const T db = z1 ? (T)0 : (U(LY::B0 + f, cb) - U(LY::B0 + f, ca)) * idxd / (T)2;
                     {
                         const T K = (T)(dxd * cmax[d] * ph.chyp[LY::M0 + f]);
                         const T s2 = K * (R[(idx_t)LY::pv(f, d) * nc + cn] +
                                           R[(idx_t)LY::pv(f, d) * nc + cn - sd]);
                         const T G = dv * s2 / (T)4;
-                        Xr[side] = (T)rhon * G;
-                        Xm[side] = G * sn.m[d];
+                        const T rhon = (T)((double)U(LY::RHO, cn) + bg[nc + cn]);
+                        Xr[side] = rhon * G;
+                        Xm[side] = G * U(LY::M0 + d, cn);
                     }
                     {
                         const T K = (T)(dxd * cmax[d] * ph.chyp[LY::B0 + f]);
@@ -425,16 +437,14 @@ __global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, Stag
                                           R[(idx_t)LY::pb(f, d) * nc + cn - sd]);
                         const T H = db * s2 / (T)2;
                         Hb[side] = H;
-                        He[side] = H * sn.b[d];
+                        He[side] = H * U(LY::B0 + d, cn);
                     }
                 }
-                if (!zero1(g.per[f], xf, nf)) {
-                    const T h = (T)g.idx[f][xf] / (T)2;
-                    dU[LY::M0 + d] += (Xr[1] - Xr[0]) * h;
-                    dU[LY::EN] += (Xm[1] - Xm[0]) * h;
-                    dU[LY::B0 + d] -= (Hb[1] - Hb[0]) * h;
-                    dU[LY::EN] -= (He[1] - He[0]) * h;
-                }
+                const T h = (T)g.idx[f][xf] / (T)2;
+                dU[LY::M0 + d] += (Xr[1] - Xr[0]) * h;
+                dU[LY::EN] += (Xm[1] - Xm[0]) * h;
+                dU[LY::B0 + d] -= (Hb[1] - Hb[0]) * h;
+                dU[LY::EN] -= (He[1] - He[0]) * h;
             }
         }
     }
@@ -447,8 +457,9 @@ __global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, Stag
         double pr = ph.drv[0] * tdep * g.prof[0][x0] * g.prof[1][x1];
         if (D == 3) pr *= g.prof[2][x2];
         const T v = (T)pr;
-        dU[LY::M0] += v * (T)rhoc;
-        dU[LY::EN] += v * v * (T)rhoc / (T)2;
+        const T rfull = (T)((double)rc + rbc);
+        dU[LY::M0] += v * rfull;
+        dU[LY::EN] += v * v * rfull / (T)2;
     }
 
     // ---- write
