@@ -14,6 +14,10 @@ Printed JSON line (rank 0): the contract fields plus
                 field's amplitude)
   roofline    : the stage-update kernel against HBM peak
   cpu_baseline: the CPU oracle (OpenMP) on a bounded sample of the workload
+  ensemble    : the batched 2-D ensemble (64 Orszag-Tang-type periodic runs,
+                sides drawn from {96,...,512}, 200 RK4 steps, mixed precision)
+                advanced by the persistent queue-driven kernels, and the same
+                runs stepped one grid at a time for comparison
 """
 import argparse
 import json
@@ -31,6 +35,9 @@ HBM_PEAK_GBS = 8000.0
 #   FP64  (128 + 16) * 8        = 1152
 #   mixed 128 * 4 + 16 * 8      = 640
 BYTES_PER_CELL_STEP = {"mixed": 640.0, "fp64": 1152.0}
+# 2-D: 6 evolved fields (16 accesses each per step) and N_B = 3 background
+# fields read in each of the 4 stages: mixed 96 * 4 + 12 * 8 = 480
+BYTES_PER_CELL_STEP_2D_MIXED = 480.0
 STAGES_PER_STEP = 4
 
 
@@ -42,6 +49,8 @@ def parse():
     ap.add_argument("--n", type=int, default=256, help="interior cells per axis")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ens-steps", type=int, default=200)
     return ap.parse_args()
 
 
@@ -82,6 +91,52 @@ def cpu_baseline(n):
         "sample": f"{steps} RK4 steps of the stratified workload on {n}x{nh}x{nh} interior "
                   f"cells (full vertical column, {nh}x{nh} horizontal), FP64, "
                   f"{el:.1f} s wall",
+    }
+
+
+def ensemble_leg(steps, warmup, seq_steps=20):
+    import numpy as np
+
+    from smaugamd import configs
+    from smaugamd.engine import MIXED, Engine, Ensemble
+
+    sides, scales = configs.ensemble_sizes()
+    setups = [configs.alfven_member(n, b) for n, b in zip(sides, scales)]
+    dts = [s.dt for s in setups]
+    ens = Ensemble(setups, MIXED)
+    for r in range(len(setups)):
+        ens.set_state(r)
+    ens.rk4(0.0, dts, warmup)
+    ms = ens.time_rk4([warmup * d for d in dts], dts, steps)
+    finite = all(np.isfinite(ens.get_state(r)).all() for r in range(len(setups)))
+    cells = ens.ninterior
+    ens.close()
+    # the same runs one grid at a time (single-grid engine, graph per grid)
+    seq_ms = 0.0
+    for s in setups:
+        e = Engine(s, MIXED)
+        e.set_state()
+        e.rk4(0.0, s.dt, 2)
+        tot, _ = e.time_rk4(2 * s.dt, s.dt, seq_steps)
+        seq_ms += tot / seq_steps
+        e.close()
+    ms_step = ms / steps
+    gbs = BYTES_PER_CELL_STEP_2D_MIXED * cells / (ms_step / 1e3) / 1e9
+    return {
+        "workload": f"{len(setups)} periodic 2-D Orszag-Tang-type runs, sides "
+                    f"{sorted(set(sides))} (numpy default_rng(0)), b0 in U[0.5,1.5]/sqrt(4 pi), "
+                    f"per-run dt = 0.3 CFL, mixed precision",
+        "runs": len(setups),
+        "cells": cells,
+        "steps": steps,
+        "ms_per_step": ms_step,
+        "value": cells * steps / (ms / 1e3) / 1e6,
+        "unit": "Mcell-updates/s",
+        "hbm_gbs_algorithmic": gbs,
+        "hbm_frac": gbs / HBM_PEAK_GBS,
+        "finite": bool(finite),
+        "one_grid_at_a_time_ms_per_step": seq_ms,
+        "speedup_over_one_grid_at_a_time": seq_ms / ms_step,
     }
 
 
@@ -158,6 +213,10 @@ def main():
 
     roof["measured_copy_gbs"] = copy_probe()
     roof["frac_of_measured_copy"] = achieved / roof["measured_copy_gbs"]
+    ens = None
+    if not a.no_ensemble:
+        ens = ensemble_leg(a.ens_steps, a.warmup)
+        print(f"[bench] ensemble: {ens['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     cpu = None if a.no_cpu else cpu_baseline(n)
     rec = {
         "metric": "Mcell-updates/s per RK4 step, mixed vs FP64 at stated L2 error; % HBM peak",
@@ -180,6 +239,7 @@ def main():
         "accuracy": acc,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "ensemble": ens,
     }
     print(json.dumps(rec))
 

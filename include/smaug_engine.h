@@ -87,6 +87,32 @@ int smaug_copy_probe(long long nbytes, int reps, double *gbs);
 long long smaug_ncell(const smaug_engine *e);
 long long smaug_ninterior(const smaug_engine *e);
 
+/* ---------------------------------------------------------------------------
+ * Ensemble: nrun independent grids of uneven sizes advanced together.  The
+ * reference runs one grid per process (iosmaug.cpp:786-1168) and has no
+ * counterpart.  Every run keeps its own n, bc, dx, pos, time and dt; ndim,
+ * precision, hyperdif, driver, gamma, g, chyp and drv must agree across runs.
+ * Each pass of an RK stage is one persistent launch pulling (run, tile)
+ * items from a device queue; one RK4 step of all runs replays a hipGraph.
+ * A member evolves bit-for-bit like the same grid in smaug_engine.
+ * ------------------------------------------------------------------------- */
+typedef struct smaug_ensemble smaug_ensemble;
+
+int smaug_ens_create(const smaug_cfg *cfgs, int nrun, smaug_ensemble **out);
+int smaug_ens_destroy(smaug_ensemble *e);
+int smaug_ens_nrun(const smaug_ensemble *e);
+/* interior cells summed over runs */
+long long smaug_ens_ninterior(const smaug_ensemble *e);
+/* state of one run, arrays as for smaug_set_state / smaug_get_state */
+int smaug_ens_set_state(smaug_ensemble *e, int run, const double *w, const double *bg);
+int smaug_ens_get_state(smaug_ensemble *e, int run, double *w);
+/* nsteps of classical RK4; t0 (may be NULL for all zero) and dt hold one
+ * value per run */
+int smaug_ens_rk4(smaug_ensemble *e, const double *t0, const double *dt, int nsteps);
+/* as smaug_ens_rk4, and the milliseconds of the step loop (HIP events) */
+int smaug_ens_time_rk4(smaug_ensemble *e, const double *t0, const double *dt, int nsteps,
+                       double *total_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2,21 +2,18 @@
 // arena, launch geometry, the captured RK4 step graph, and the C-ABI of
 // smaug_engine.h.
 #include "smaug_kernels.hpp"
-#include "../../include/smaug_engine.h"
+#include "smaug_host.hpp"
 
-#include <hip/hip_runtime.h>
-#include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
-#include <vector>
 
 using namespace smaug;
 
 static thread_local char g_err[512] = "";
 
-static int fail(int code, const char *fmt, ...)
+int smaug::fail(int code, const char *fmt, ...)
 {
     va_list ap;
     va_start(ap, fmt);
@@ -25,13 +22,43 @@ static int fail(int code, const char *fmt, ...)
     return code;
 }
 
-#define HIPCHK(x)                                                                          \
-    do {                                                                                   \
-        hipError_t e_ = (x);                                                               \
-        if (e_ != hipSuccess)                                                              \
-            return fail(SMAUG_EHIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_),    \
-                        __FILE__, __LINE__);                                               \
-    } while (0)
+int smaug::check_cfg(const smaug_cfg *cfg)
+{
+    int D = cfg->ndim;
+    if (D != 2 && D != 3) return fail(SMAUG_EINVAL, "ndim must be 2 or 3 (got %d)", D);
+    for (int d = 0; d < D; d++) {
+        if (cfg->n[d] < 8) return fail(SMAUG_EINVAL, "n[%d]=%d: need >= 8 cells incl. ghosts", d, cfg->n[d]);
+        if (cfg->bc[d] != 0 && cfg->bc[d] != 4)
+            return fail(SMAUG_EINVAL, "boundtype %d along dim %d unsupported (0 periodic, 4 contcd4)",
+                        cfg->bc[d], d);
+        if (!cfg->dx[d]) return fail(SMAUG_EINVAL, "dx[%d] missing", d);
+        if (cfg->driver && !cfg->pos[d]) return fail(SMAUG_EINVAL, "pos[%d] missing", d);
+    }
+    if (cfg->precision != SMAUG_FP64 && cfg->precision != SMAUG_MIXED)
+        return fail(SMAUG_EINVAL, "precision must be SMAUG_FP64 or SMAUG_MIXED");
+    return 0;
+}
+
+void smaug::axis_arrays(const smaug_cfg *cfg, int d, int nd, std::vector<double> &dx,
+                        std::vector<double> &idx, std::vector<double> &prof)
+{
+    dx.assign(nd, 1.0);
+    idx.assign(nd, 1.0);
+    prof.assign(nd, 1.0);
+    if (d >= cfg->ndim) return;
+    for (int i = 0; i < nd; i++) {
+        dx[i] = cfg->dx[d][i];
+        idx[i] = 1.0 / dx[i];
+    }
+    if (cfg->driver == 1) {
+        for (int i = 0; i < nd; i++) {
+            double p = cfg->pos[d][i];
+            double c0 = d == 0 ? cfg->drv[3] : (d == 1 ? cfg->drv[5] : cfg->drv[6]);
+            double s = d == 0 ? cfg->drv[4] : cfg->drv[2];
+            prof[i] = exp(-(p - c0) * (p - c0) / (s * s));
+        }
+    }
+}
 
 extern "C" const char *smaug_last_error(void) { return g_err; }
 
@@ -238,18 +265,9 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
 {
     if (!cfg || !out) return fail(SMAUG_EINVAL, "null argument");
     *out = nullptr;
+    int rc0 = check_cfg(cfg);
+    if (rc0) return rc0;
     int D = cfg->ndim;
-    if (D != 2 && D != 3) return fail(SMAUG_EINVAL, "ndim must be 2 or 3 (got %d)", D);
-    for (int d = 0; d < D; d++) {
-        if (cfg->n[d] < 8) return fail(SMAUG_EINVAL, "n[%d]=%d: need >= 8 cells incl. ghosts", d, cfg->n[d]);
-        if (cfg->bc[d] != 0 && cfg->bc[d] != 4)
-            return fail(SMAUG_EINVAL, "boundtype %d along dim %d unsupported (0 periodic, 4 contcd4)",
-                        cfg->bc[d], d);
-        if (!cfg->dx[d]) return fail(SMAUG_EINVAL, "dx[%d] missing", d);
-        if (cfg->driver && !cfg->pos[d]) return fail(SMAUG_EINVAL, "pos[%d] missing", d);
-    }
-    if (cfg->precision != SMAUG_FP64 && cfg->precision != SMAUG_MIXED)
-        return fail(SMAUG_EINVAL, "precision must be SMAUG_FP64 or SMAUG_MIXED");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(SMAUG_ENODEV, "no HIP device");
@@ -281,23 +299,8 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
     TRYALLOC(e->d_tm, sizeof(double) * 2);
     for (int d = 0; d < 3; d++) {
         int nd = e->n[d];
-        std::vector<double> dx(nd, 1.0), idx(nd, 1.0), prof(nd, 1.0);
-        if (d < D) {
-            for (int i = 0; i < nd; i++) {
-                dx[i] = cfg->dx[d][i];
-                idx[i] = 1.0 / dx[i];
-            }
-            if (cfg->driver == 1) {
-                // separable driver profile: exp(-(z-z0)^2/sz^2) along dim 0,
-                // exp(-(x-xc)^2/sh^2) along the horizontal dims
-                for (int i = 0; i < nd; i++) {
-                    double p = cfg->pos[d][i];
-                    double c0 = d == 0 ? cfg->drv[3] : (d == 1 ? cfg->drv[5] : cfg->drv[6]);
-                    double s = d == 0 ? cfg->drv[4] : cfg->drv[2];
-                    prof[i] = exp(-(p - c0) * (p - c0) / (s * s));
-                }
-            }
-        }
+        std::vector<double> dx, idx, prof;
+        axis_arrays(cfg, d, nd, dx, idx, prof);
         TRYALLOC(e->d_dx[d], sizeof(double) * nd);
         TRYALLOC(e->d_idx[d], sizeof(double) * nd);
         TRYALLOC(e->d_prof[d], sizeof(double) * nd);

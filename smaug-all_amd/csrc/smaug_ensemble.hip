@@ -1,0 +1,603 @@
+// smaug_ensemble.hip -- many independent grids of uneven sizes advanced
+// together by persistent, queue-driven kernels.
+//
+// Every pass of an RK stage (prep, ratio, update, ghost fill) is one
+// persistent launch over the whole ensemble: its work items are the
+// (run, tile) pairs of all runs, numbered run by run, and the blocks pull
+// them from a device queue.  The item range is cut into 8 contiguous
+// slices, one per XCD; a block first drains the slice of its own XCD
+// (blocks are dispatched to XCDs round-robin, so block b runs on XCD b % 8)
+// and then steals from the others.  Neighbouring tiles of one run are thus
+// mostly processed on one XCD and share its L2.  Each run keeps its own
+// geometry, cmax, time and dt; all runs share the physics parameters.
+// One RK4 step (a counter reset, 4 stages x 4 passes and the time advance)
+// is captured once into a hipGraph and replayed.
+//
+// The per-cell arithmetic is the single-grid engine's (prep_cell,
+// ratio_cell, update_cell, bc_elem), so an ensemble member evolves
+// bit-for-bit like the same grid stepped alone.
+#include "smaug_kernels.hpp"
+#include "smaug_host.hpp"
+
+#include <stdio.h>
+#include <string.h>
+#include <vector>
+
+using namespace smaug;
+
+namespace {
+
+constexpr int PREP_CELLS = 1024; // cells per prep item (4 per thread)
+constexpr int NPASS = 4;         // prep, ratio, update, ghost fill
+constexpr int NQ = 8;            // queue slices, one per XCD
+
+template <typename T>
+struct RunDesc {
+    Geo geo;
+    T *U[3];
+    T *ACC, *R, *AUX;
+    const double *bg;
+    unsigned long long *cm; // [4 stages][4] cmax bit patterns
+    double *tm;             // [t, dt]
+    int ntx[2], nty[2];     // 64 x 4 tiles of the ratio box (0) and the interior (1)
+};
+
+struct Queue {
+    const int *base;   // nrun + 1 prefix sums of the items of each run
+    int nrun, nitems;
+    unsigned *ctr;     // NQ slice counters
+};
+
+// next item for the calling block (thread 0 only), -1 when every slice is
+// drained; v is the number of slices this block has found empty so far
+__device__ __forceinline__ int pull(const Queue &q, int xcd, int &v)
+{
+    while (v < NQ) {
+        const int s = (xcd + v) & (NQ - 1);
+        const long long lo = ((long long)q.nitems * s) / NQ;
+        const long long hi = ((long long)q.nitems * (s + 1)) / NQ;
+        const unsigned k = atomicAdd(&q.ctr[s], 1u);
+        if (lo + (long long)k < hi) return (int)(lo + k);
+        v++;
+    }
+    return -1;
+}
+
+// run that owns item it: the largest r with base[r] <= it
+__device__ __forceinline__ int run_of(const int *__restrict__ base, int nrun, int it)
+{
+    int lo = 0, hi = nrun;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (base[mid] <= it)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// the block-uniform next item; two barriers so that s_it is not rewritten
+// before every thread has read it
+__device__ __forceinline__ int block_pull(const Queue &q, int &v, int *s_it)
+{
+    if (threadIdx.x == 0) *s_it = pull(q, blockIdx.x & (NQ - 1), v);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(*s_it);
+    __syncthreads();
+    return it;
+}
+
+__global__ void ke_reset(unsigned *ctr, int nctr, unsigned long long *cm, int ncm)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nctr + ncm;
+         i += gridDim.x * blockDim.x) {
+        if (i < nctr)
+            ctr[i] = 0u;
+        else
+            cm[i - nctr] = 0ull;
+    }
+}
+
+__global__ void ke_advance_time(double *tm, int nrun)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrun) tm[2 * r] += tm[2 * r + 1];
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(256) void ke_prep(const RunDesc<T> *__restrict__ runs, Queue q,
+                                                Phys ph, int stage, int src)
+{
+    __shared__ int s_it;
+    __shared__ double red[3][4];
+    int v = 0;
+    for (;;) {
+        const int it = block_pull(q, v, &s_it);
+        if (it < 0) break;
+        const int r = run_of(q.base, q.nrun, it);
+        const RunDesc<T> &rd = runs[r];
+        const idx_t c0 = (idx_t)(it - q.base[r]) * PREP_CELLS;
+        double mx[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < PREP_CELLS / 256; k++) {
+            const idx_t c = c0 + k * 256 + threadIdx.x;
+            if (c < rd.geo.nc) prep_cell<D, T>(rd.geo, ph, rd.U[src], rd.bg, rd.AUX, c, mx);
+        }
+        block_max_to<D>(mx, red, rd.cm + 4 * stage);
+    }
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(256) void ke_ratio(const RunDesc<T> *__restrict__ runs, Queue q,
+                                                 int src)
+{
+    __shared__ int s_it;
+    int v = 0;
+    for (;;) {
+        const int it = block_pull(q, v, &s_it);
+        if (it < 0) break;
+        const int r = run_of(q.base, q.nrun, it);
+        const RunDesc<T> &rd = runs[r];
+        const int l = it - q.base[r];
+        const int tx = l % rd.ntx[0], rest = l / rd.ntx[0];
+        const int ty = rest % rd.nty[0], tk = rest / rd.nty[0];
+        const int x0 = 1 + tx * 64 + (threadIdx.x & 63);
+        const int x1 = 1 + ty * 4 + (threadIdx.x >> 6);
+        const int x2 = (D == 3) ? 1 + tk : 0;
+        if (x0 <= rd.geo.n[0] - 3 && x1 <= rd.geo.n[1] - 3)
+            ratio_cell<D, T>(rd.geo, rd.U[src], rd.AUX, rd.R, x0, x1, x2);
+    }
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(256) void ke_update(const RunDesc<T> *__restrict__ runs, Queue q,
+                                                  Phys ph, int stage, double ct, int src, int dst)
+{
+    __shared__ int s_it;
+    int v = 0;
+    for (;;) {
+        const int it = block_pull(q, v, &s_it);
+        if (it < 0) break;
+        const int r = run_of(q.base, q.nrun, it);
+        const RunDesc<T> &rd = runs[r];
+        const int l = it - q.base[r];
+        const int tx = l % rd.ntx[1], rest = l / rd.ntx[1];
+        const int ty = rest % rd.nty[1], tk = rest / rd.nty[1];
+        const int x0 = 2 + tx * 64 + (threadIdx.x & 63);
+        const int x1 = 2 + ty * 4 + (threadIdx.x >> 6);
+        const int x2 = (D == 3) ? 2 + tk : 0;
+        if (x0 <= rd.geo.n[0] - 3 && x1 <= rd.geo.n[1] - 3) {
+            StageArgs sa;
+            sa.mode = 0;
+            sa.stage = stage;
+            sa.ct = ct;
+            sa.tm = rd.tm;
+            update_cell<D, T>(rd.geo, ph, sa, rd.U[src], rd.U[0], rd.U[dst], rd.ACC, rd.R, rd.AUX,
+                              rd.bg, rd.cm + 4 * stage, x0, x1, x2);
+        }
+    }
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(256) void ke_bc(const RunDesc<T> *__restrict__ runs, Queue q, int dst)
+{
+    __shared__ int s_it;
+    int v = 0;
+    for (;;) {
+        const int it = block_pull(q, v, &s_it);
+        if (it < 0) break;
+        const int r = run_of(q.base, q.nrun, it);
+        const RunDesc<T> &rd = runs[r];
+        const idx_t t = (idx_t)(it - q.base[r]) * 256 + threadIdx.x;
+        if (t < bc_count<D>(rd.geo.n[0], rd.geo.n[1], rd.geo.n[2])) bc_elem<D, T>(rd.geo, rd.U[dst], t);
+    }
+}
+
+} // namespace
+
+struct smaug_ensemble {
+    int nrun, D, prec;
+    size_t tsz;
+    hipStream_t stream;
+    std::vector<smaug_cfg> cfg;
+    std::vector<Geo> geo;          // host copies (device pointers inside)
+    std::vector<idx_t> nc, nint;
+    std::vector<size_t> off_u, off_r, off_aux, off_bg; // byte offsets into the arenas
+    char *state;                   // U0, U1, U2, ACC of every run
+    char *ratio, *aux;
+    double *bg, *axes, *tm;
+    unsigned long long *cm;        // nrun x 16
+    unsigned *ctr;                 // 16 launches x NQ
+    void *descs;                   // RunDesc<T>[nrun]
+    int *base[NPASS];              // per-pass item prefix sums (device)
+    int nitems[NPASS];
+    unsigned grid[NPASS];          // persistent grid per pass
+    Phys phys;
+    hipGraphExec_t gexec;
+    bool gvalid;
+};
+
+static void free_ens(smaug_ensemble *e)
+{
+    if (!e) return;
+    if (e->gvalid) hipGraphExecDestroy(e->gexec);
+    void *ptrs[] = {e->state, e->ratio, e->aux, e->bg, e->axes, e->tm, e->cm, e->ctr, e->descs};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    for (int k = 0; k < NPASS; k++)
+        if (e->base[k]) hipFree(e->base[k]);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static bool same_phys(const smaug_cfg &a, const smaug_cfg &b)
+{
+    if (a.ndim != b.ndim || a.precision != b.precision || a.hyperdif != b.hyperdif ||
+        a.driver != b.driver || a.gamma != b.gamma)
+        return false;
+    for (int k = 0; k < 3; k++)
+        if (a.g[k] != b.g[k]) return false;
+    for (int k = 0; k < 13; k++)
+        if (a.chyp[k] != b.chyp[k]) return false;
+    for (int k = 0; k < 8; k++)
+        if (a.drv[k] != b.drv[k]) return false;
+    return true;
+}
+
+// persistent grid of a pass: resident blocks per CU x CUs, a multiple of 8
+template <typename K>
+static int pass_grid(K kernel, unsigned *grid)
+{
+    int dev = 0, ncu = 0, per = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, 0));
+    if (per < 1) per = 1;
+    unsigned g = (unsigned)(per * ncu);
+    *grid = (g + 7) & ~7u;
+    return 0;
+}
+
+template <int D, typename T>
+static int setup_typed(smaug_ensemble *e)
+{
+    std::vector<RunDesc<T>> d(e->nrun);
+    std::vector<int> base[NPASS];
+    for (int k = 0; k < NPASS; k++) base[k].assign(e->nrun + 1, 0);
+    const int nve = 2 + 2 * D;
+    for (int r = 0; r < e->nrun; r++) {
+        RunDesc<T> &rd = d[r];
+        const Geo &g = e->geo[r];
+        rd.geo = g;
+        T *u = (T *)(e->state + e->off_u[r]);
+        for (int k = 0; k < 3; k++) rd.U[k] = u + (idx_t)k * nve * g.nc;
+        rd.ACC = u + (idx_t)3 * nve * g.nc;
+        rd.R = e->phys.hyperdif ? (T *)(e->ratio + e->off_r[r]) : nullptr;
+        rd.AUX = (T *)(e->aux + e->off_aux[r]);
+        rd.bg = (const double *)((char *)e->bg + e->off_bg[r]);
+        rd.cm = e->cm + 16 * r;
+        rd.tm = e->tm + 2 * r;
+        const int n2 = D == 3 ? g.n[2] : 1;
+        // ratio box [1, n-3], interior [2, n-3]
+        rd.ntx[0] = (g.n[0] - 3 + 63) / 64;
+        rd.nty[0] = (g.n[1] - 3 + 3) / 4;
+        rd.ntx[1] = (g.n[0] - 4 + 63) / 64;
+        rd.nty[1] = (g.n[1] - 4 + 3) / 4;
+        const int nk0 = D == 3 ? n2 - 3 : 1, nk1 = D == 3 ? n2 - 4 : 1;
+        long long items[NPASS] = {
+            (long long)((g.nc + PREP_CELLS - 1) / PREP_CELLS),
+            e->phys.hyperdif ? (long long)rd.ntx[0] * rd.nty[0] * nk0 : 0,
+            (long long)rd.ntx[1] * rd.nty[1] * nk1,
+            (long long)((bc_count<D>(g.n[0], g.n[1], n2) + 255) / 256)};
+        for (int k = 0; k < NPASS; k++) {
+            long long nb = (long long)base[k][r] + items[k];
+            if (nb > 0x7fffffffLL) return fail(SMAUG_EINVAL, "ensemble too large for the work queue");
+            base[k][r + 1] = (int)nb;
+        }
+    }
+    HIPCHK(hipMalloc(&e->descs, sizeof(RunDesc<T>) * e->nrun));
+    HIPCHK(hipMemcpy(e->descs, d.data(), sizeof(RunDesc<T>) * e->nrun, hipMemcpyHostToDevice));
+    for (int k = 0; k < NPASS; k++) {
+        e->nitems[k] = base[k][e->nrun];
+        HIPCHK(hipMalloc((void **)&e->base[k], sizeof(int) * (e->nrun + 1)));
+        HIPCHK(hipMemcpy(e->base[k], base[k].data(), sizeof(int) * (e->nrun + 1),
+                         hipMemcpyHostToDevice));
+    }
+    int rc;
+    if ((rc = pass_grid(ke_prep<D, T>, &e->grid[0]))) return rc;
+    if ((rc = pass_grid(ke_ratio<D, T>, &e->grid[1]))) return rc;
+    if ((rc = pass_grid(ke_update<D, T>, &e->grid[2]))) return rc;
+    if ((rc = pass_grid(ke_bc<D, T>, &e->grid[3]))) return rc;
+    return 0;
+}
+
+#define EDISPATCH(e, FN, ...)                                                              \
+    ((e)->D == 2 ? ((e)->prec == SMAUG_FP64 ? FN<2, double>(__VA_ARGS__)                   \
+                                            : FN<2, float>(__VA_ARGS__))                   \
+                 : ((e)->prec == SMAUG_FP64 ? FN<3, double>(__VA_ARGS__)                   \
+                                            : FN<3, float>(__VA_ARGS__)))
+
+extern "C" int smaug_ens_create(const smaug_cfg *cfgs, int nrun, smaug_ensemble **out)
+{
+    if (!cfgs || !out || nrun < 1) return fail(SMAUG_EINVAL, "bad argument");
+    *out = nullptr;
+    for (int r = 0; r < nrun; r++) {
+        int rc = check_cfg(&cfgs[r]);
+        if (rc) return rc;
+        if (!same_phys(cfgs[0], cfgs[r]))
+            return fail(SMAUG_EINVAL,
+                        "run %d: ndim, precision, hyperdif, driver, gamma, g, chyp and drv must "
+                        "be the same for every run of an ensemble", r);
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(SMAUG_ENODEV, "no HIP device");
+
+    smaug_ensemble *e = new smaug_ensemble();
+    e->nrun = nrun;
+    e->D = cfgs[0].ndim;
+    e->prec = cfgs[0].precision;
+    e->tsz = e->prec == SMAUG_FP64 ? sizeof(double) : sizeof(float);
+    e->cfg.assign(cfgs, cfgs + nrun);
+    e->state = e->ratio = e->aux = nullptr;
+    e->bg = e->axes = e->tm = nullptr;
+    e->cm = nullptr;
+    e->ctr = nullptr;
+    e->descs = nullptr;
+    for (int k = 0; k < NPASS; k++) e->base[k] = nullptr;
+    e->stream = nullptr;
+    e->gvalid = false;
+#define ECHK(x)                                                                            \
+    do {                                                                                   \
+        hipError_t he_ = (x);                                                              \
+        if (he_ != hipSuccess) {                                                           \
+            free_ens(e);                                                                   \
+            return fail(he_ == hipErrorOutOfMemory ? SMAUG_ENOMEM : SMAUG_EHIP,            \
+                        "%s failed: %s", #x, hipGetErrorString(he_));                       \
+        }                                                                                  \
+    } while (0)
+    ECHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+
+    const int D = e->D, nve = 2 + 2 * D, nbg = 2 + D;
+    const int npair = 2 * D + D * D + D * (D - 1);
+    size_t su = 0, sr = 0, sa = 0, sb = 0, sx = 0;
+    for (int r = 0; r < nrun; r++) {
+        const smaug_cfg &c = cfgs[r];
+        const idx_t n2 = D == 3 ? c.n[2] : 1;
+        const idx_t nc = (idx_t)c.n[0] * c.n[1] * n2;
+        e->nc.push_back(nc);
+        e->nint.push_back((idx_t)(c.n[0] - 4) * (c.n[1] - 4) * (D == 3 ? n2 - 4 : 1));
+        e->off_u.push_back(su);
+        e->off_r.push_back(sr);
+        e->off_aux.push_back(sa);
+        e->off_bg.push_back(sb);
+        su += align256(e->tsz * 4 * nve * nc);
+        if (c.hyperdif) sr += align256(e->tsz * npair * nc);
+        sa += align256(e->tsz * (3 + D) * nc);
+        sb += align256(sizeof(double) * nbg * nc);
+        sx += 3 * (size_t)(c.n[0] + c.n[1] + n2);
+    }
+    ECHK(hipMalloc((void **)&e->state, su));
+    if (sr) ECHK(hipMalloc((void **)&e->ratio, sr));
+    ECHK(hipMalloc((void **)&e->aux, sa));
+    ECHK(hipMalloc((void **)&e->bg, sb));
+    ECHK(hipMalloc((void **)&e->axes, sizeof(double) * sx));
+    ECHK(hipMalloc((void **)&e->tm, sizeof(double) * 2 * nrun));
+    ECHK(hipMalloc((void **)&e->cm, sizeof(unsigned long long) * 16 * nrun));
+    ECHK(hipMalloc((void **)&e->ctr, sizeof(unsigned) * 4 * NPASS * NQ));
+    ECHK(hipMemset(e->state, 0, su));
+    ECHK(hipMemset(e->tm, 0, sizeof(double) * 2 * nrun));
+
+    // per-run metric arrays, packed [dx | 1/dx | profile] per dim
+    std::vector<double> host(sx);
+    size_t pos = 0;
+    for (int r = 0; r < nrun; r++) {
+        const smaug_cfg &c = cfgs[r];
+        Geo g;
+        memset(&g, 0, sizeof(g));
+        for (int d = 0; d < 3; d++) {
+            const int nd = d < D ? c.n[d] : 1;
+            std::vector<double> dx, idx, prof;
+            axis_arrays(&c, d, nd, dx, idx, prof);
+            memcpy(&host[pos], dx.data(), sizeof(double) * nd);
+            memcpy(&host[pos + nd], idx.data(), sizeof(double) * nd);
+            memcpy(&host[pos + 2 * nd], prof.data(), sizeof(double) * nd);
+            g.n[d] = nd;
+            g.per[d] = d < D ? (c.bc[d] == 0) : 1;
+            g.dx[d] = e->axes + pos;
+            g.idx[d] = e->axes + pos + nd;
+            g.prof[d] = e->axes + pos + 2 * nd;
+            pos += 3 * (size_t)nd;
+        }
+        g.s1 = g.n[0];
+        g.s2 = (idx_t)g.n[0] * g.n[1];
+        g.nc = e->nc[r];
+        e->geo.push_back(g);
+    }
+    ECHK(hipMemcpy(e->axes, host.data(), sizeof(double) * pos, hipMemcpyHostToDevice));
+
+    Phys &p = e->phys;
+    const smaug_cfg &c0 = cfgs[0];
+    p.gamma = c0.gamma;
+    for (int d = 0; d < 3; d++) p.g[d] = c0.g[d];
+    for (int f = 0; f < 13; f++) p.chyp[f] = c0.chyp[f];
+    for (int k = 0; k < 8; k++) p.drv[k] = c0.drv[k];
+    p.hyperdif = c0.hyperdif;
+    p.driver = c0.driver;
+#undef ECHK
+    int rc = EDISPATCH(e, setup_typed, e);
+    if (rc) {
+        free_ens(e);
+        return rc;
+    }
+    *out = e;
+    return 0;
+}
+
+extern "C" int smaug_ens_destroy(smaug_ensemble *e)
+{
+    if (e) {
+        hipStreamSynchronize(e->stream);
+        free_ens(e);
+    }
+    return 0;
+}
+
+extern "C" int smaug_ens_nrun(const smaug_ensemble *e) { return e ? e->nrun : 0; }
+
+extern "C" long long smaug_ens_ninterior(const smaug_ensemble *e)
+{
+    long long s = 0;
+    if (e)
+        for (idx_t v : e->nint) s += v;
+    return s;
+}
+
+template <int D, typename T>
+static int ens_set_state(smaug_ensemble *e, int r, const double *w, const double *bg)
+{
+    const int nve = 2 + 2 * D, nbg = 2 + D;
+    const idx_t nc = e->nc[r];
+    std::vector<T> b((size_t)nve * nc);
+    for (size_t i = 0; i < b.size(); i++) b[i] = (T)w[i];
+    T *u0 = (T *)(e->state + e->off_u[r]);
+    HIPCHK(hipMemcpyAsync(u0, b.data(), sizeof(T) * b.size(), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync((char *)e->bg + e->off_bg[r], bg, sizeof(double) * nbg * nc,
+                          hipMemcpyHostToDevice, e->stream));
+    const Geo &g = e->geo[r];
+    const idx_t tot = bc_count<D>(g.n[0], g.n[1], g.n[2]);
+    hipLaunchKernelGGL((k_bc<D, T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, e->stream,
+                       g, u0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+extern "C" int smaug_ens_set_state(smaug_ensemble *e, int run, const double *w, const double *bg)
+{
+    if (!e || !w || !bg || run < 0 || run >= e->nrun) return fail(SMAUG_EINVAL, "bad argument");
+    return EDISPATCH(e, ens_set_state, e, run, w, bg);
+}
+
+extern "C" int smaug_ens_get_state(smaug_ensemble *e, int run, double *w)
+{
+    if (!e || !w || run < 0 || run >= e->nrun) return fail(SMAUG_EINVAL, "bad argument");
+    const idx_t n = (idx_t)(2 + 2 * e->D) * e->nc[run];
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const char *u0 = e->state + e->off_u[run];
+    if (e->prec == SMAUG_FP64) {
+        HIPCHK(hipMemcpy(w, u0, sizeof(double) * n, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> b(n);
+        HIPCHK(hipMemcpy(b.data(), u0, sizeof(float) * n, hipMemcpyDeviceToHost));
+        for (idx_t i = 0; i < n; i++) w[i] = b[i];
+    }
+    return 0;
+}
+
+static Queue queue(smaug_ensemble *e, int pass, int launch)
+{
+    Queue q;
+    q.base = e->base[pass];
+    q.nrun = e->nrun;
+    q.nitems = e->nitems[pass];
+    q.ctr = e->ctr + (size_t)launch * NQ;
+    return q;
+}
+
+// one RK4 step of every run (same slot algebra as the single-grid engine)
+template <int D, typename T>
+static int ens_step(smaug_ensemble *e)
+{
+    const double ct[4] = {0.0, 0.5, 0.5, 1.0};
+    const int in[4] = {0, 1, 2, 1}, out[4] = {1, 2, 1, 0};
+    const RunDesc<T> *rd = (const RunDesc<T> *)e->descs;
+    hipLaunchKernelGGL(ke_reset, dim3(8), dim3(256), 0, e->stream, e->ctr, 4 * NPASS * NQ, e->cm,
+                       16 * e->nrun);
+    for (int s = 0; s < 4; s++) {
+        hipLaunchKernelGGL((ke_prep<D, T>), dim3(e->grid[0]), dim3(256), 0, e->stream, rd,
+                           queue(e, 0, 4 * s + 0), e->phys, s, in[s]);
+        if (e->phys.hyperdif)
+            hipLaunchKernelGGL((ke_ratio<D, T>), dim3(e->grid[1]), dim3(256), 0, e->stream, rd,
+                               queue(e, 1, 4 * s + 1), in[s]);
+        hipLaunchKernelGGL((ke_update<D, T>), dim3(e->grid[2]), dim3(256), 0, e->stream, rd,
+                           queue(e, 2, 4 * s + 2), e->phys, s, ct[s], in[s], out[s]);
+        hipLaunchKernelGGL((ke_bc<D, T>), dim3(e->grid[3]), dim3(256), 0, e->stream, rd,
+                           queue(e, 3, 4 * s + 3), out[s]);
+    }
+    hipLaunchKernelGGL(ke_advance_time, dim3((e->nrun + 63) / 64), dim3(64), 0, e->stream, e->tm,
+                       e->nrun);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+template <int D, typename T>
+static int ens_capture(smaug_ensemble *e)
+{
+    if (e->gvalid) return 0;
+    hipGraph_t graph;
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = ens_step<D, T>(e);
+    hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+    if (rc) return rc;
+    if (ce != hipSuccess) return fail(SMAUG_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+    hipError_t ie = hipGraphInstantiate(&e->gexec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ie != hipSuccess) return fail(SMAUG_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+    e->gvalid = true;
+    return 0;
+}
+
+static int ens_set_time(smaug_ensemble *e, const double *t0, const double *dt)
+{
+    std::vector<double> tm(2 * (size_t)e->nrun);
+    for (int r = 0; r < e->nrun; r++) {
+        tm[2 * r] = t0 ? t0[r] : 0.0;
+        tm[2 * r + 1] = dt[r];
+    }
+    HIPCHK(hipMemcpyAsync(e->tm, tm.data(), sizeof(double) * tm.size(), hipMemcpyHostToDevice,
+                          e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+template <int D, typename T>
+static int ens_run(smaug_ensemble *e, const double *t0, const double *dt, int nsteps, double *ms)
+{
+    int rc = ens_capture<D, T>(e);
+    if (rc) return rc;
+    if ((rc = ens_set_time(e, t0, dt))) return rc;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (ms) {
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        HIPCHK(hipEventRecord(a, e->stream));
+    }
+    for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(e->gexec, e->stream));
+    if (ms) {
+        HIPCHK(hipEventRecord(b, e->stream));
+        HIPCHK(hipEventSynchronize(b));
+        float f = 0;
+        HIPCHK(hipEventElapsedTime(&f, a, b));
+        *ms = f;
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+extern "C" int smaug_ens_rk4(smaug_ensemble *e, const double *t0, const double *dt, int nsteps)
+{
+    if (!e || !dt || nsteps < 0) return fail(SMAUG_EINVAL, "bad argument");
+    return EDISPATCH(e, ens_run, e, t0, dt, nsteps, (double *)nullptr);
+}
+
+extern "C" int smaug_ens_time_rk4(smaug_ensemble *e, const double *t0, const double *dt,
+                                  int nsteps, double *total_ms)
+{
+    if (!e || !dt || nsteps < 1 || !total_ms) return fail(SMAUG_EINVAL, "bad argument");
+    return EDISPATCH(e, ens_run, e, t0, dt, nsteps, total_ms);
+}

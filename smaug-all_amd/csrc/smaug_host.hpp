@@ -1,0 +1,35 @@
+// smaug_host.hpp -- host-side helpers shared by the single-grid engine and
+// the ensemble: error text, HIP status checks, per-axis metric arrays.
+#pragma once
+#include "../../include/smaug_engine.h"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <vector>
+
+namespace smaug {
+
+// records the message returned by smaug_last_error() and returns code
+int fail(int code, const char *fmt, ...);
+
+// validates the fields of one grid's configuration that do not depend on
+// other grids (dimensionality, sizes, boundaries, metric pointers)
+int check_cfg(const smaug_cfg *cfg);
+
+// dx, 1/dx and the separable driver profile along dim d (length n[d]; unit
+// arrays along the unused third dim of a 2-D grid).  The driver profile is
+// exp(-(z-z0)^2/sz^2) along dim 0 and exp(-(x-xc)^2/sh^2) along the
+// horizontal dims, so that the product over dims is the driver's
+// Gaussian footprint (usersource.cu:4-92).
+void axis_arrays(const smaug_cfg *cfg, int d, int nd, std::vector<double> &dx,
+                 std::vector<double> &idx, std::vector<double> &prof);
+
+} // namespace smaug
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return smaug::fail(SMAUG_EHIP, "%s failed: %s (%s:%d)", #x,                   \
+                               hipGetErrorString(e_), __FILE__, __LINE__);                 \
+    } while (0)

@@ -31,7 +31,7 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
-__global__ void k_reset_cmax(unsigned long long *cm)
+static __global__ void k_reset_cmax(unsigned long long *cm)
 {
     if (threadIdx.x < 3) cm[threadIdx.x] = 0ull;
 }
@@ -48,6 +48,49 @@ struct Aux {
     static constexpr int IRHO = 0, PT = 1, EINT = 2, V0 = 3, N = 3 + D;
 };
 
+// aux of cell c and the running cfast maxima
+template <int D, typename T>
+__device__ __forceinline__ void prep_cell(const Geo &g, const Phys &ph, const T *__restrict__ u,
+                                          const double *__restrict__ bg, T *__restrict__ aux,
+                                          idx_t c, double *mx)
+{
+    typedef Aux<D> AX;
+    St<D, T> s;
+    load_st<D, T>(s, u, bg, g.nc, c);
+    const double rho = rho_full(s);
+    const double ir = 1.0 / rho;
+    aux[(idx_t)AX::IRHO * g.nc + c] = (T)ir;
+    aux[(idx_t)AX::PT * g.nc + c] = (T)ptot_pert<D, T>(s, rho, ph.gamma);
+    aux[(idx_t)AX::EINT * g.nc + c] = (T)e_int<D, T>(s, rho);
+#pragma unroll
+    for (int d = 0; d < D; d++) aux[(idx_t)(AX::V0 + d) * g.nc + c] = (T)((double)s.m[d] * ir);
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        double v = cfast<D, T>(s, d, ph.gamma);
+        mx[d] = v > mx[d] ? v : mx[d];
+    }
+}
+
+// block-wide max of mx[] (blockDim.x == 256) folded into cm[] with a 64-bit
+// atomic max; non-negative doubles order like their bit patterns
+template <int D>
+__device__ __forceinline__ void block_max_to(double *mx, double (*red)[4], unsigned long long *cm)
+{
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        double v = wave_max(mx[d]);
+        if (lane == 0) red[d][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < D) {
+        int d = threadIdx.x;
+        double v = red[d][0];
+        for (int k = 1; k < 4; k++) v = red[d][k] > v ? red[d][k] : v;
+        atomicMax(&cm[d], (unsigned long long)__double_as_longlong(v));
+    }
+}
+
 // blockDim.x == 256; grid-stride over every cell (ghosts included)
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_prep(Geo g, Phys ph, const T *__restrict__ u,
@@ -60,36 +103,9 @@ __global__ __launch_bounds__(256) void k_prep(Geo g, Phys ph, const T *__restric
 #pragma unroll
     for (int d = 0; d < D; d++) mx[d] = 0.0;
     for (idx_t c = (idx_t)blockIdx.x * blockDim.x + threadIdx.x; c < g.nc;
-         c += (idx_t)gridDim.x * blockDim.x) {
-        St<D, T> s;
-        load_st<D, T>(s, u, bg, g.nc, c);
-        const double rho = rho_full(s);
-        const double ir = 1.0 / rho;
-        aux[(idx_t)AX::IRHO * g.nc + c] = (T)ir;
-        aux[(idx_t)AX::PT * g.nc + c] = (T)ptot_pert<D, T>(s, rho, ph.gamma);
-        aux[(idx_t)AX::EINT * g.nc + c] = (T)e_int<D, T>(s, rho);
-#pragma unroll
-        for (int d = 0; d < D; d++) aux[(idx_t)(AX::V0 + d) * g.nc + c] = (T)((double)s.m[d] * ir);
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-            double v = cfast<D, T>(s, d, ph.gamma);
-            mx[d] = v > mx[d] ? v : mx[d];
-        }
-    }
-    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-        double v = wave_max(mx[d]);
-        if (lane == 0) red[d][wv] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < D) {
-        int d = threadIdx.x;
-        double v = red[d][0];
-        for (int k = 1; k < 4; k++) v = red[d][k] > v ? red[d][k] : v;
-        // non-negative doubles order like their bit patterns
-        atomicMax(&cm[d], (unsigned long long)__double_as_longlong(v));
-    }
+         c += (idx_t)gridDim.x * blockDim.x)
+        prep_cell<D, T>(g, ph, u, bg, aux, c, mx);
+    block_max_to<D>(mx, red, cm);
 }
 
 // ---------------------------------------------------------------------------
@@ -119,17 +135,12 @@ __device__ __forceinline__ T ratio6(const T q[6])
 }
 
 template <int D, typename T>
-__global__ __launch_bounds__(256) void k_ratio(Geo g, TileMap tm, const T *__restrict__ u,
-                                                const T *__restrict__ aux, T *__restrict__ R)
+__device__ __forceinline__ void ratio_cell(const Geo &g, const T *__restrict__ u,
+                                           const T *__restrict__ aux, T *__restrict__ R,
+                                           const int x0, const int x1, const int x2)
 {
     typedef Lay<D> LY;
     typedef Aux<D> AX;
-    int tx, ty, tk;
-    if (!map_block(tm, tx, ty, tk)) return;
-    int x0 = 1 + tx * 64 + (threadIdx.x & 63);
-    int x1 = 1 + ty * 4 + (threadIdx.x >> 6);
-    int x2 = (D == 3) ? 1 + tk : 0;
-    if (x0 > g.n[0] - 3 || x1 > g.n[1] - 3) return;
     int xs[3] = {x0, x1, x2};
     const idx_t nc = g.nc;
     idx_t c = (idx_t)x0 + g.s1 * x1 + g.s2 * x2;
@@ -165,6 +176,19 @@ __global__ __launch_bounds__(256) void k_ratio(Geo g, TileMap tm, const T *__res
     }
 }
 
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_ratio(Geo g, TileMap tm, const T *__restrict__ u,
+                                                const T *__restrict__ aux, T *__restrict__ R)
+{
+    int tx, ty, tk;
+    if (!map_block(tm, tx, ty, tk)) return;
+    int x0 = 1 + tx * 64 + (threadIdx.x & 63);
+    int x1 = 1 + ty * 4 + (threadIdx.x >> 6);
+    int x2 = (D == 3) ? 1 + tk : 0;
+    if (x0 > g.n[0] - 3 || x1 > g.n[1] - 3) return;
+    ratio_cell<D, T>(g, u, aux, R, x0, x1, x2);
+}
+
 // ---------------------------------------------------------------------------
 // ghost fill: every ghost cell takes the value at the image of each of its
 // coordinates (contcd4: 0<-4, 1<-3, n-1<-n-5, n-2<-n-4; periodic: 0,1 <-
@@ -180,11 +204,18 @@ __device__ __forceinline__ int bc_map(int x, int n, int per)
     return (x == n - 1) ? x - 4 : x - 2;
 }
 
+// number of ghost-slab elements of a grid (4 layers per slab, slabs may overlap)
+template <int D>
+__host__ __device__ __forceinline__ idx_t bc_count(int n0, int n1, int n2)
+{
+    return (idx_t)4 * n1 * n2 + (idx_t)4 * n0 * n2 + (D == 3 ? (idx_t)4 * n0 * n1 : 0);
+}
+
+// fill ghost-slab element t
 template <int D, typename T>
-__global__ __launch_bounds__(256) void k_bc(Geo g, T *__restrict__ u)
+__device__ __forceinline__ void bc_elem(const Geo &g, T *__restrict__ u, idx_t t)
 {
     typedef Lay<D> LY;
-    idx_t t = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
     int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     idx_t a0 = (idx_t)4 * n1 * n2, a1 = (idx_t)4 * n0 * n2, a2 = (D == 3) ? (idx_t)4 * n0 * n1 : 0;
     int x[3];
@@ -220,6 +251,12 @@ __global__ __launch_bounds__(256) void k_bc(Geo g, T *__restrict__ u)
     for (int f = 0; f < LY::NVE; f++) u[(idx_t)f * g.nc + dst] = u[(idx_t)f * g.nc + src];
 }
 
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_bc(Geo g, T *__restrict__ u)
+{
+    bc_elem<D, T>(g, u, (idx_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // stage update
 // ---------------------------------------------------------------------------
@@ -230,24 +267,20 @@ struct StageArgs {
     const double *tm; // device [t, dt]
 };
 
+// the whole stage update of interior cell (x0, x1, x2)
 template <int D, typename T>
-__global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, StageArgs sa,
-                                                 const T *__restrict__ uin,
-                                                 const T *__restrict__ un,
-                                                 T *__restrict__ uout, T *__restrict__ acc,
-                                                 const T *__restrict__ R,
-                                                 const T *__restrict__ aux,
-                                                 const double *__restrict__ bg,
-                                                 const unsigned long long *__restrict__ cmb)
+__device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const StageArgs &sa,
+                                            const T *__restrict__ uin,
+                                            const T *__restrict__ un,
+                                            T *__restrict__ uout, T *__restrict__ acc,
+                                            const T *__restrict__ R,
+                                            const T *__restrict__ aux,
+                                            const double *__restrict__ bg,
+                                            const unsigned long long *__restrict__ cmb,
+                                            const int x0, const int x1, const int x2)
 {
     typedef Lay<D> LY;
     typedef Aux<D> AX;
-    int tx, ty, tk;
-    if (!map_block(tm, tx, ty, tk)) return;
-    const int x0 = 2 + tx * 64 + (threadIdx.x & 63);
-    const int x1 = 2 + ty * 4 + (threadIdx.x >> 6);
-    const int x2 = (D == 3) ? 2 + tk : 0;
-    if (x0 > g.n[0] - 3 || x1 > g.n[1] - 3) return;
     const int xs[3] = {x0, x1, x2};
     const idx_t nc = g.nc;
     const idx_t c = (idx_t)x0 + g.s1 * x1 + g.s2 * x2;
@@ -494,6 +527,25 @@ __global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, Stag
             acc[o] = (s == 0 ? base : acc[o]) + b * dU[f];
         }
     }
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, StageArgs sa,
+                                                 const T *__restrict__ uin,
+                                                 const T *__restrict__ un,
+                                                 T *__restrict__ uout, T *__restrict__ acc,
+                                                 const T *__restrict__ R,
+                                                 const T *__restrict__ aux,
+                                                 const double *__restrict__ bg,
+                                                 const unsigned long long *__restrict__ cmb)
+{
+    int tx, ty, tk;
+    if (!map_block(tm, tx, ty, tk)) return;
+    const int x0 = 2 + tx * 64 + (threadIdx.x & 63);
+    const int x1 = 2 + ty * 4 + (threadIdx.x >> 6);
+    const int x2 = (D == 3) ? 2 + tk : 0;
+    if (x0 > g.n[0] - 3 || x1 > g.n[1] - 3) return;
+    update_cell<D, T>(g, ph, sa, uin, un, uout, acc, R, aux, bg, cmb, x0, x1, x2);
 }
 
 } // namespace smaug

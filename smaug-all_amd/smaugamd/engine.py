@@ -24,6 +24,8 @@ EXPORTS = [
     "smaug_last_error", "smaug_create", "smaug_destroy", "smaug_set_state", "smaug_get_state",
     "smaug_rhs", "smaug_cmax", "smaug_maxcourant", "smaug_rk4", "smaug_euler",
     "smaug_time_rk4", "smaug_ncell", "smaug_ninterior", "smaug_copy_probe",
+    "smaug_ens_create", "smaug_ens_destroy", "smaug_ens_nrun", "smaug_ens_ninterior",
+    "smaug_ens_set_state", "smaug_ens_get_state", "smaug_ens_rk4", "smaug_ens_time_rk4",
 ]
 
 
@@ -75,6 +77,17 @@ def load_library(path: str = LIB_PATH):
         lib.smaug_ninterior.argtypes = [ctypes.c_void_p]
         lib.smaug_ninterior.restype = ctypes.c_longlong
         lib.smaug_copy_probe.argtypes = [ctypes.c_longlong, ctypes.c_int, dp]
+        vp = ctypes.c_void_p
+        lib.smaug_ens_create.argtypes = [ctypes.POINTER(SmaugCfg), ctypes.c_int,
+                                         ctypes.POINTER(vp)]
+        lib.smaug_ens_destroy.argtypes = [vp]
+        lib.smaug_ens_nrun.argtypes = [vp]
+        lib.smaug_ens_ninterior.argtypes = [vp]
+        lib.smaug_ens_ninterior.restype = ctypes.c_longlong
+        lib.smaug_ens_set_state.argtypes = [vp, ctypes.c_int, dp, dp]
+        lib.smaug_ens_get_state.argtypes = [vp, ctypes.c_int, dp]
+        lib.smaug_ens_rk4.argtypes = [vp, dp, dp, ctypes.c_int]
+        lib.smaug_ens_time_rk4.argtypes = [vp, dp, dp, ctypes.c_int, dp]
         _lib = lib
     return _lib
 
@@ -172,6 +185,76 @@ class Engine:
         self._check(self.lib.smaug_time_rk4(self.h, t0, dt, nsteps, ctypes.byref(tot),
                                             ctypes.byref(st)))
         return tot.value, st.value
+
+
+class Ensemble:
+    """Uneven grids advanced together by the persistent queue-driven kernels.
+    Every set-up keeps its own grid, time and dt; the physics parameters
+    (ndim, hyperdif, driver, gamma, g, chyp, drv) must agree."""
+
+    def __init__(self, setups, precision: int = MIXED):
+        self.lib = load_library()
+        self.setups = list(setups)
+        self.precision = precision
+        arr = (SmaugCfg * len(self.setups))()
+        self._keep = []
+        for r, s in enumerate(self.setups):
+            cfg, keep = make_cfg(s, precision)
+            arr[r] = cfg
+            self._keep += keep
+        self._cfgs = arr
+        h = ctypes.c_void_p()
+        self._check(self.lib.smaug_ens_create(arr, len(self.setups), ctypes.byref(h)))
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise SmaugError(f"smaug error {rc}: {self.lib.smaug_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.smaug_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ninterior(self) -> int:
+        return int(self.lib.smaug_ens_ninterior(self.h))
+
+    def set_state(self, run, w=None, bg=None):
+        s = self.setups[run]
+        w = np.ascontiguousarray(s.w if w is None else w, dtype=np.float64)
+        bg = np.ascontiguousarray(s.bg if bg is None else bg, dtype=np.float64)
+        assert w.shape == (s.nve, s.ncell) and bg.shape == (s.nbg, s.ncell)
+        self._check(self.lib.smaug_ens_set_state(self.h, run, _dptr(w), _dptr(bg)))
+
+    def get_state(self, run):
+        s = self.setups[run]
+        w = np.empty((s.nve, s.ncell))
+        self._check(self.lib.smaug_ens_get_state(self.h, run, _dptr(w)))
+        return w
+
+    def _times(self, t0, dt):
+        n = len(self.setups)
+        t0 = np.ascontiguousarray(np.broadcast_to(np.asarray(t0, dtype=np.float64), (n,)))
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (n,)))
+        return t0, dt
+
+    def rk4(self, t0, dt, nsteps):
+        t0, dt = self._times(t0, dt)
+        self._check(self.lib.smaug_ens_rk4(self.h, _dptr(t0), _dptr(dt), nsteps))
+
+    def time_rk4(self, t0, dt, nsteps):
+        t0, dt = self._times(t0, dt)
+        ms = ctypes.c_double()
+        self._check(self.lib.smaug_ens_time_rk4(self.h, _dptr(t0), _dptr(dt), nsteps,
+                                                ctypes.byref(ms)))
+        return ms.value
 
 
 def copy_probe(nbytes=2 << 30, reps=10):

@@ -1,0 +1,54 @@
+"""The C-ABI libraries load on a host without a GPU and export every function
+that include/*.h declares.  No compute call is made here."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT as REPO
+
+INC = os.path.join(REPO, "include")
+LIB = os.path.join(REPO, "smaug-all_amd", "lib")
+
+DECL = re.compile(r"^(?:const\s+)?(?:int|long long|char|void|double)\s*\**\s*(\w+)\s*\(", re.M)
+
+
+def declared(header):
+    with open(os.path.join(INC, header)) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(DECL.findall(text)))
+
+
+def exported(lib):
+    out = subprocess.run(["nm", "-D", "--defined-only", lib], capture_output=True, text=True,
+                         check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if " T " in line}
+
+
+def test_engine_header_parsed():
+    names = declared("smaug_engine.h")
+    for n in ("smaug_create", "smaug_rk4", "smaug_ens_create", "smaug_ens_rk4"):
+        assert n in names
+
+
+@pytest.mark.parametrize("lib,header", [
+    ("libsmaug_engine.so", "smaug_engine.h"),
+    ("libsmaug_hip2d.so", "smaug_hip.h"),
+    ("libsmaug_hip3d.so", "smaug_hip.h"),
+])
+def test_library_exports_header(lib, header):
+    path = os.path.join(LIB, lib)
+    assert os.path.exists(path), f"{path} not built"
+    ctypes.CDLL(path)  # resolves its HIP runtime dependencies
+    names = declared(header)
+    assert len(names) > 10
+    missing = [n for n in names if n not in exported(path)]
+    assert not missing, f"{lib} lacks {missing}"
+
+
+def test_python_binding_lists_every_engine_symbol():
+    import smaugamd.engine as eng
+    assert sorted(eng.EXPORTS) == declared("smaug_engine.h")
