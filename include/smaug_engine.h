@@ -79,6 +79,18 @@ int smaug_euler(smaug_engine *e, double t0, double dt, int nsteps);
 int smaug_time_rk4(smaug_engine *e, double t0, double dt, int nsteps,
                    double *total_ms, double *stage_kernel_ms);
 
+/* Launch shape.  The ratio (pass 0) and update (pass 1) kernels cover the
+ * grid with 64 x 4-cell tiles grouped in XCD panels of tpx x tpy tiles, each
+ * panel swept plane by plane on one XCD.  smaug_set_panel fixes a panel
+ * (0, 0 = default), smaug_get_panel reports the one in use,
+ * smaug_time_pass times reps launches of a pass on the current state (the
+ * state is not changed), and smaug_autotune times every panel of
+ * tpx in {1,2,4,8} x tpy in {1,..,32} and keeps the fastest per pass. */
+int smaug_set_panel(smaug_engine *e, int pass, int tpx, int tpy);
+int smaug_get_panel(smaug_engine *e, int pass, int *tpx, int *tpy);
+int smaug_time_pass(smaug_engine *e, int pass, int reps, double *ms);
+int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double best_ms[2]);
+
 /* measured copy-bandwidth ceiling: GB/s (read + write bytes) of a plain
  * streaming copy of nbytes, mean of reps launches */
 int smaug_copy_probe(long long nbytes, int reps, double *gbs);

@@ -36,6 +36,9 @@ int smaug::check_cfg(const smaug_cfg *cfg)
     }
     if (cfg->precision != SMAUG_FP64 && cfg->precision != SMAUG_MIXED)
         return fail(SMAUG_EINVAL, "precision must be SMAUG_FP64 or SMAUG_MIXED");
+    long long nc = (long long)cfg->n[0] * cfg->n[1] * (D == 3 ? cfg->n[2] : 1);
+    if (nc > MAX_CELLS)
+        return fail(SMAUG_EINVAL, "%lld cells exceed the per-grid limit of %lld", nc, MAX_CELLS);
     return 0;
 }
 
@@ -82,6 +85,7 @@ struct smaug_engine {
     Phys phys;
     hipGraphExec_t gexec[2]; // 0: RK4 step, 1: Euler step
     bool gvalid[2];
+    int panel[2][2];         // XCD panel (tpx, tpy) of the ratio and update passes; 0 = default
 };
 
 // ---------------------------------------------------------------------------
@@ -100,15 +104,20 @@ static int launch_cmax(smaug_engine *e, const void *u)
 }
 
 // tiles of 64 x 4 cells over an extent of e0 x e1 x e2 cells, grouped in
-// panels so that each XCD sweeps its own sub-domain (see map_block)
-static TileMap make_map(int e0, int e1, int e2, unsigned *nblocks)
+// panels of tpx x tpy tiles so that each XCD sweeps its own sub-domain (see
+// map_block); tpx, tpy <= 0 pick the default panel
+static TileMap make_map(int e0, int e1, int e2, int tpx, int tpy, unsigned *nblocks)
 {
     TileMap m;
     m.ntx = (e0 + 63) / 64;
     m.nty = (e1 + 3) / 4;
     m.nk = e2;
-    m.tpx = m.ntx < 8 ? m.ntx : 8;
-    m.tpy = 16 / m.tpx > 0 ? 16 / m.tpx : 1;
+    if (tpx <= 0 || tpy <= 0) {
+        tpx = m.ntx < 8 ? m.ntx : 8;
+        tpy = 16 / tpx > 0 ? 16 / tpx : 1;
+    }
+    m.tpx = tpx < m.ntx ? tpx : m.ntx;
+    m.tpy = tpy < m.nty ? tpy : m.nty;
     m.npx = (m.ntx + m.tpx - 1) / m.tpx;
     int npy = (m.nty + m.tpy - 1) / m.tpy;
     m.npanels = m.npx * npy;
@@ -121,7 +130,8 @@ template <int D, typename T>
 static int launch_ratio(smaug_engine *e, const void *u)
 {
     unsigned nb;
-    TileMap m = make_map(e->n[0] - 3, e->n[1] - 3, D == 3 ? e->n[2] - 3 : 1, &nb); // box [1, n-3]
+    TileMap m = make_map(e->n[0] - 3, e->n[1] - 3, D == 3 ? e->n[2] - 3 : 1, e->panel[0][0],
+                         e->panel[0][1], &nb); // box [1, n-3]
     hipLaunchKernelGGL((k_ratio<D, T>), dim3(nb), dim3(256), 0, e->stream, e->geo, m, (const T *)u,
                        (const T *)e->AUX, (T *)e->R);
     HIPCHK(hipGetLastError());
@@ -133,7 +143,8 @@ static int launch_update(smaug_engine *e, StageArgs sa, const void *uin, const v
                          void *uout)
 {
     unsigned nb;
-    TileMap m = make_map(e->n[0] - 4, e->n[1] - 4, D == 3 ? e->n[2] - 4 : 1, &nb);
+    TileMap m = make_map(e->n[0] - 4, e->n[1] - 4, D == 3 ? e->n[2] - 4 : 1, e->panel[1][0],
+                         e->panel[1][1], &nb);
     hipLaunchKernelGGL((k_update<D, T>), dim3(nb), dim3(256), 0, e->stream, e->geo, m, e->phys, sa,
                        (const T *)uin, (const T *)un, (T *)uout, (T *)e->ACC,
                        (const T *)e->R, (const T *)e->AUX, (const double *)e->bg,
@@ -557,6 +568,122 @@ extern "C" int smaug_time_rk4(smaug_engine *e, double t0, double dt, int nsteps,
 {
     if (!e || nsteps < 1 || !total_ms || !stage_kernel_ms) return fail(SMAUG_EINVAL, "bad argument");
     return DISPATCH(e, time_rk4, e, t0, dt, nsteps, total_ms, stage_kernel_ms);
+}
+
+// ---------------------------------------------------------------------------
+// launch-shape tuning
+// ---------------------------------------------------------------------------
+static void drop_graphs(smaug_engine *e)
+{
+    for (int k = 0; k < 2; k++)
+        if (e->gvalid[k]) {
+            hipGraphExecDestroy(e->gexec[k]);
+            e->gvalid[k] = false;
+        }
+}
+
+extern "C" int smaug_set_panel(smaug_engine *e, int pass, int tpx, int tpy)
+{
+    if (!e || pass < 0 || pass > 1) return fail(SMAUG_EINVAL, "bad argument");
+    e->panel[pass][0] = tpx;
+    e->panel[pass][1] = tpy;
+    drop_graphs(e);
+    return 0;
+}
+
+extern "C" int smaug_get_panel(smaug_engine *e, int pass, int *tpx, int *tpy)
+{
+    if (!e || pass < 0 || pass > 1 || !tpx || !tpy) return fail(SMAUG_EINVAL, "bad argument");
+    unsigned nb;
+    const int D = e->D;
+    const int c = pass == 0 ? 3 : 4;
+    TileMap m = make_map(e->n[0] - c, e->n[1] - c, D == 3 ? e->n[2] - c : 1, e->panel[pass][0],
+                         e->panel[pass][1], &nb);
+    *tpx = m.tpx;
+    *tpy = m.tpy;
+    return 0;
+}
+
+// mean duration (ms) of reps launches of one pass with the current panel;
+// the inputs are the current state, the outputs go to the stage buffers,
+// which every RK4 step rewrites before reading
+template <int D, typename T>
+static int time_pass(smaug_engine *e, int pass, int reps, double *ms)
+{
+    int rc = launch_cmax<D, T>(e, e->U[0]);
+    if (!rc && e->phys.hyperdif) rc = launch_ratio<D, T>(e, e->U[0]);
+    if (rc) return rc;
+    StageArgs sa;
+    sa.mode = 0;
+    sa.stage = 1;
+    sa.ct = 0.5;
+    sa.tm = e->d_tm;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    for (int r = -1; r < reps; r++) {
+        if (r == 0) HIPCHK(hipEventRecord(a, e->stream));
+        rc = pass == 0 ? launch_ratio<D, T>(e, e->U[0])
+                       : launch_update<D, T>(e, sa, e->U[0], e->U[0], e->U[1]);
+        if (rc) return rc;
+    }
+    HIPCHK(hipEventRecord(b, e->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float f = 0;
+    HIPCHK(hipEventElapsedTime(&f, a, b));
+    *ms = f / reps;
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return 0;
+}
+
+extern "C" int smaug_time_pass(smaug_engine *e, int pass, int reps, double *ms)
+{
+    if (!e || pass < 0 || pass > 1 || reps < 1 || !ms) return fail(SMAUG_EINVAL, "bad argument");
+    if (pass == 0 && !e->phys.hyperdif) return fail(SMAUG_EINVAL, "no ratio pass without hyperdiffusion");
+    return DISPATCH(e, time_pass, e, pass, reps, ms);
+}
+
+// picks, for the ratio and update passes, the fastest XCD panel among
+// tpx in {1, 2, 4, 8} and tpy in {1, 2, 4, 8, 16, 32}
+extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double best_ms[2])
+{
+    if (!e || reps < 1) return fail(SMAUG_EINVAL, "bad argument");
+    static const int TX[] = {1, 2, 4, 8}, TY[] = {1, 2, 4, 8, 16, 32};
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 0 && !e->phys.hyperdif) {
+            if (best) best[0][0] = best[0][1] = 0;
+            if (best_ms) best_ms[0] = 0;
+            continue;
+        }
+        double bm = 1e30;
+        int bx = 0, by = 0;
+        for (int tx : TX)
+            for (int ty : TY) {
+                int gx, gy;
+                e->panel[pass][0] = tx;
+                e->panel[pass][1] = ty;
+                smaug_get_panel(e, pass, &gx, &gy);
+                if (gx != tx || gy != ty) continue; // clamped duplicate
+                double ms;
+                int rc = DISPATCH(e, time_pass, e, pass, reps, &ms);
+                if (rc) return rc;
+                if (ms < bm) {
+                    bm = ms;
+                    bx = tx;
+                    by = ty;
+                }
+            }
+        e->panel[pass][0] = bx;
+        e->panel[pass][1] = by;
+        if (best) {
+            best[pass][0] = bx;
+            best[pass][1] = by;
+        }
+        if (best_ms) best_ms[pass] = bm;
+    }
+    drop_graphs(e);
+    return 0;
 }
 
 // ---------------------------------------------------------------------------

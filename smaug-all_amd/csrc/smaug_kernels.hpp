@@ -52,18 +52,19 @@ struct Aux {
 template <int D, typename T>
 __device__ __forceinline__ void prep_cell(const Geo &g, const Phys &ph, const T *__restrict__ u,
                                           const double *__restrict__ bg, T *__restrict__ aux,
-                                          idx_t c, double *mx)
+                                          ci_t c, double *mx)
 {
     typedef Aux<D> AX;
+    const size_t nc = (size_t)g.nc;
     St<D, T> s;
-    load_st<D, T>(s, u, bg, g.nc, c);
+    load_st32<D, T>(s, u, bg, nc, c);
     const double rho = rho_full(s);
     const double ir = 1.0 / rho;
-    aux[(idx_t)AX::IRHO * g.nc + c] = (T)ir;
-    aux[(idx_t)AX::PT * g.nc + c] = (T)ptot_pert<D, T>(s, rho, ph.gamma);
-    aux[(idx_t)AX::EINT * g.nc + c] = (T)e_int<D, T>(s, rho);
+    st(aux + AX::IRHO * nc, c, (T)ir);
+    st(aux + AX::PT * nc, c, (T)ptot_pert<D, T>(s, rho, ph.gamma));
+    st(aux + AX::EINT * nc, c, (T)e_int<D, T>(s, rho));
 #pragma unroll
-    for (int d = 0; d < D; d++) aux[(idx_t)(AX::V0 + d) * g.nc + c] = (T)((double)s.m[d] * ir);
+    for (int d = 0; d < D; d++) st(aux + (AX::V0 + d) * nc, c, (T)((double)s.m[d] * ir));
 #pragma unroll
     for (int d = 0; d < D; d++) {
         double v = cfast<D, T>(s, d, ph.gamma);
@@ -102,8 +103,8 @@ __global__ __launch_bounds__(256) void k_prep(Geo g, Phys ph, const T *__restric
     double mx[D];
 #pragma unroll
     for (int d = 0; d < D; d++) mx[d] = 0.0;
-    for (idx_t c = (idx_t)blockIdx.x * blockDim.x + threadIdx.x; c < g.nc;
-         c += (idx_t)gridDim.x * blockDim.x)
+    const ci_t nc = (ci_t)g.nc;
+    for (ci_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
         prep_cell<D, T>(g, ph, u, bg, aux, c, mx);
     block_max_to<D>(mx, red, cm);
 }
@@ -142,13 +143,13 @@ __device__ __forceinline__ void ratio_cell(const Geo &g, const T *__restrict__ u
     typedef Lay<D> LY;
     typedef Aux<D> AX;
     int xs[3] = {x0, x1, x2};
-    const idx_t nc = g.nc;
-    idx_t c = (idx_t)x0 + g.s1 * x1 + g.s2 * x2;
+    const size_t nc = (size_t)g.nc;
+    const ci_t c = (ci_t)x0 + (ci_t)g.s1 * (ci_t)x1 + (ci_t)g.s2 * (ci_t)x2;
 #pragma unroll
     for (int d = 0; d < D; d++) {
         int x = xs[d], n = g.n[d];
-        idx_t sd = stride<D>(g, d);
-        idx_t c0 = c - (idx_t)x * sd;
+        const ci_t sd = stride32<D>(g, d);
+        const ci_t c0 = c - (ci_t)x * sd;
         // differences and ratios in double in both paths: third differences
         // of nearly smooth FP32 data sit near the FP32 rounding level
         double qr[6], qe[6], qv[D][6], qb[D][6];
@@ -157,22 +158,22 @@ __device__ __forceinline__ void ratio_cell(const Geo &g, const T *__restrict__ u
             int xx = x - 2 + p;
             if (xx < 0) xx = 5;
             if (xx >= n) xx = n - 6;
-            const idx_t cp = c0 + (idx_t)xx * sd;
-            qr[p] = u[cp];
-            qe[p] = aux[(idx_t)AX::EINT * nc + cp];
+            const ci_t cp = c0 + (ci_t)xx * sd;
+            qr[p] = ld(u, cp);
+            qe[p] = ld(aux + AX::EINT * nc, cp);
 #pragma unroll
             for (int f = 0; f < D; f++) {
-                qv[f][p] = aux[(idx_t)(AX::V0 + f) * nc + cp];
-                qb[f][p] = u[(idx_t)(LY::B0 + f) * nc + cp];
+                qv[f][p] = ld(aux + (AX::V0 + f) * nc, cp);
+                qb[f][p] = ld(u + (LY::B0 + f) * nc, cp);
             }
         }
-        R[(idx_t)LY::prho(d) * nc + c] = (T)ratio6<double>(qr);
-        R[(idx_t)LY::pe(d) * nc + c] = (T)ratio6<double>(qe);
+        st(R + LY::prho(d) * nc, c, (T)ratio6<double>(qr));
+        st(R + LY::pe(d) * nc, c, (T)ratio6<double>(qe));
 #pragma unroll
-        for (int f = 0; f < D; f++) R[(idx_t)LY::pv(f, d) * nc + c] = (T)ratio6<double>(qv[f]);
+        for (int f = 0; f < D; f++) st(R + LY::pv(f, d) * nc, c, (T)ratio6<double>(qv[f]));
 #pragma unroll
         for (int f = 0; f < D; f++)
-            if (f != d) R[(idx_t)LY::pb(f, d) * nc + c] = (T)ratio6<double>(qb[f]);
+            if (f != d) st(R + LY::pb(f, d) * nc, c, (T)ratio6<double>(qb[f]));
     }
 }
 
@@ -282,33 +283,34 @@ __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const 
     typedef Lay<D> LY;
     typedef Aux<D> AX;
     const int xs[3] = {x0, x1, x2};
-    const idx_t nc = g.nc;
-    const idx_t c = (idx_t)x0 + g.s1 * x1 + g.s2 * x2;
+    const size_t nc = (size_t)g.nc;
+    const ci_t c = (ci_t)x0 + (ci_t)g.s1 * (ci_t)x1 + (ci_t)g.s2 * (ci_t)x2;
     const double gam = ph.gamma;
     // field and aux accessors
-    auto U = [&](int f, idx_t cc) -> T { return uin[(idx_t)f * nc + cc]; };
-    auto A = [&](int a, idx_t cc) -> T { return aux[(idx_t)a * nc + cc]; };
+    auto U = [&](int f, ci_t cc) -> T { return ld(uin + f * nc, cc); };
+    auto A = [&](int a, ci_t cc) -> T { return ld(aux + a * nc, cc); };
+    auto BG = [&](int f, ci_t cc) -> double { return ld(bg + f * nc, cc); };
 
     T dU[LY::NVE];
 #pragma unroll
     for (int f = 0; f < LY::NVE; f++) dU[f] = 0;
 
-    const double rbc = bg[nc + c];
+    const double rbc = BG(1, c);
     T bbc[D];
 #pragma unroll
-    for (int d = 0; d < D; d++) bbc[d] = (T)bg[(idx_t)(2 + d) * nc + c];
+    for (int d = 0; d < D; d++) bbc[d] = (T)BG(2 + d, c);
     double bbb = 0;
 #pragma unroll
-    for (int d = 0; d < D; d++) bbb += bg[(idx_t)(2 + d) * nc + c] * bg[(idx_t)(2 + d) * nc + c];
+    for (int d = 0; d < D; d++) bbb += BG(2 + d, c) * BG(2 + d, c);
     // background total pressure (computepbg3)
-    const T ptb = (T)((gam - 1) * bg[c] - 0.5 * (gam - 2) * bbb);
+    const T ptb = (T)((gam - 1) * BG(0, c) - 0.5 * (gam - 2) * bbb);
 
     // ---- advection: 4th-order divergence of the fluxes along each axis a,
     // accumulated neighbour by neighbour, plus the background work terms of
     // the energy equation (centdiff1.cu:19-256, centdiff2.cu:20-356)
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        const idx_t sa_ = stride<D>(g, a);
+        const ci_t sa_ = stride32<D>(g, a);
         const int xa = xs[a], na = g.n[a];
         if (zero4(g.per[a], xa, na)) continue;
         // d/dx_a q = (q[-2] - 8 q[-1] + 8 q[+1] - q[+2]) / (12 dx)
@@ -320,16 +322,16 @@ __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const 
         for (int p = 0; p < 4; p++) {
             const int off = (p < 2) ? p - 2 : p - 1; // -2,-1,+1,+2
             const T cw = (p == 0 ? (T)1 : (p == 1 ? (T)-8 : (p == 2 ? (T)8 : (T)-1))) * i12;
-            const idx_t cn = c + (idx_t)off * sa_;
+            const ci_t cn = c + (ci_t)off * sa_;
             T m[D], b[D], bb[D];
 #pragma unroll
             for (int d = 0; d < D; d++) {
                 m[d] = U(LY::M0 + d, cn);
                 b[d] = U(LY::B0 + d, cn);
-                bb[d] = (T)bg[(idx_t)(2 + d) * nc + cn];
+                bb[d] = (T)BG(2 + d, cn);
             }
             const T e = U(LY::EN, cn);
-            const T eb = (T)bg[cn];
+            const T eb = (T)BG(0, cn);
             const T ir = A(AX::IRHO, cn);
             const T pt = A(AX::PT, cn);
             const T ma = m[a];
@@ -384,17 +386,17 @@ __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const 
 
 #pragma unroll
         for (int d = 0; d < D; d++) {
-            const idx_t sd = stride<D>(g, d);
+            const ci_t sd = stride32<D>(g, d);
             const int xd = xs[d], nd = g.n[d];
             const double dxd = g.dx[d][xd];
             const T idxd = (T)g.idx[d][xd];
             const bool z1 = zero1(g.per[d], xd, nd);
-            const idx_t cl = c - sd, cr = c + sd;
+            const ci_t cl = c - sd, cr = c + sd;
 
             auto nuRL = [&](int pair, int field, T &nR, T &nL) {
                 const T K = (T)(dxd * cmax[d] * ph.chyp[field]);
-                nR = K * R[(idx_t)pair * nc + c];
-                nL = K * R[(idx_t)pair * nc + cl];
+                nR = K * ld(R + pair * nc, c);
+                nL = K * ld(R + pair * nc, cl);
             };
             T nR, nL, gR, gL;
             // density (hyperdifrhosource1.cu:15-120)
@@ -409,8 +411,8 @@ __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const 
             dU[LY::EN] += (nR * gR - nL * gL) * idxd;
             // momentum, diagonal part for every f; twice when f == d
             // (hyperdifmomsource1.cu:17-533 called for ii1 = 0 and, if f == d, ii1 = 1)
-            const T rR = (T)((rhoc + (double)U(LY::RHO, cr) + bg[nc + cr]) / 2);
-            const T rL = (T)((rhoc + (double)U(LY::RHO, cl) + bg[nc + cl]) / 2);
+            const T rR = (T)((rhoc + (double)U(LY::RHO, cr) + BG(1, cr)) / 2);
+            const T rL = (T)((rhoc + (double)U(LY::RHO, cl) + BG(1, cl)) / 2);
 #pragma unroll
             for (int f = 0; f < D; f++) {
                 nuRL(LY::pv(f, d), LY::M0 + f, nR, nL);
@@ -441,33 +443,33 @@ __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const 
 #pragma unroll
             for (int f = 0; f < D; f++) {
                 if (f == d) continue;
-                const idx_t sf = stride<D>(g, f);
+                const ci_t sf = stride32<D>(g, f);
                 const int xf = xs[f], nf = g.n[f];
                 if (zero1(g.per[f], xf, nf)) continue;
                 T Xr[2], Xm[2], Hb[2], He[2];
 #pragma unroll
                 for (int side = 0; side < 2; side++) {
                     const int sg = side ? 1 : -1;
-                    const idx_t cn = c + sg * sf;
+                    const ci_t cn = side ? c + sf : c - sf;
                     Xr[side] = Xm[side] = Hb[side] = He[side] = 0;
                     const int xfn = xf + sg;
                     if (xfn < 2 || xfn > nf - 3) continue; // nu vanishes off the interior
-                    const idx_t ca = cn - sd, cb = cn + sd;
+                    const ci_t ca = cn - sd, cb = cn + sd;
                     const T dv = z1 ? (T)0 : (A(AX::V0 + f, cb) - A(AX::V0 + f, ca)) * idxd / (T)2;
                     const T db = z1 ? (T)0 : (U(LY::B0 + f, cb) - U(LY::B0 + f, ca)) * idxd / (T)2;
                     {
                         const T K = (T)(dxd * cmax[d] * ph.chyp[LY::M0 + f]);
-                        const T s2 = K * (R[(idx_t)LY::pv(f, d) * nc + cn] +
-                                          R[(idx_t)LY::pv(f, d) * nc + cn - sd]);
+                        const T s2 = K * (ld(R + LY::pv(f, d) * nc, cn) +
+                                          ld(R + LY::pv(f, d) * nc, cn - sd));
                         const T G = dv * s2 / (T)4;
-                        const T rhon = (T)((double)U(LY::RHO, cn) + bg[nc + cn]);
+                        const T rhon = (T)((double)U(LY::RHO, cn) + BG(1, cn));
                         Xr[side] = rhon * G;
                         Xm[side] = G * U(LY::M0 + d, cn);
                     }
                     {
                         const T K = (T)(dxd * cmax[d] * ph.chyp[LY::B0 + f]);
-                        const T s2 = K * (R[(idx_t)LY::pb(f, d) * nc + cn] +
-                                          R[(idx_t)LY::pb(f, d) * nc + cn - sd]);
+                        const T s2 = K * (ld(R + LY::pb(f, d) * nc, cn) +
+                                          ld(R + LY::pb(f, d) * nc, cn - sd));
                         const T H = db * s2 / (T)2;
                         Hb[side] = H;
                         He[side] = H * U(LY::B0 + d, cn);
@@ -498,14 +500,14 @@ __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const 
     // ---- write
     if (sa.mode == 2) {
 #pragma unroll
-        for (int f = 0; f < LY::NVE; f++) uout[(idx_t)f * nc + c] = dU[f];
+        for (int f = 0; f < LY::NVE; f++) st(uout + f * nc, c, dU[f]);
         return;
     }
     if (sa.mode == 1) {
         const T tdt = (T)dt;
 #pragma unroll
         for (int f = 0; f < LY::NVE; f++)
-            uout[(idx_t)f * nc + c] = un[(idx_t)f * nc + c] + tdt * dU[f];
+            st(uout + f * nc, c, ld(un + f * nc, c) + tdt * dU[f]);
         return;
     }
     // low-storage classical RK4:
@@ -518,13 +520,13 @@ __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const 
     const T b = (T)(s == 0 ? dt / 6 : dt / 3);
 #pragma unroll
     for (int f = 0; f < LY::NVE; f++) {
-        const idx_t o = (idx_t)f * nc + c;
+        const size_t o = f * nc;
         if (s == 3) {
-            uout[o] = acc[o] + a * dU[f];
+            st(uout + o, c, ld(acc + o, c) + a * dU[f]);
         } else {
-            const T base = un[o];
-            uout[o] = base + a * dU[f];
-            acc[o] = (s == 0 ? base : acc[o]) + b * dU[f];
+            const T base = ld(un + o, c);
+            st(uout + o, c, base + a * dU[f]);
+            st(acc + o, c, (s == 0 ? base : ld(acc + o, c)) + b * dU[f]);
         }
     }
 }

@@ -60,6 +60,31 @@ __device__ __forceinline__ idx_t stride(const Geo &g, int d)
     return d == 0 ? (idx_t)1 : (d == 1 ? g.s1 : g.s2);
 }
 
+// Cell index within one field for the fused kernels.  A grid is limited to
+// 2^29 cells so that the byte offset of a cell in a double field fits in 32
+// bits: each access is then one 32-bit VGPR offset against a field base
+// address held in SGPRs (the global_load saddr form), shared by every field.
+typedef unsigned int ci_t;
+constexpr long long MAX_CELLS = 1ll << 29;
+
+template <typename T>
+__device__ __forceinline__ T ld(const T *__restrict__ base, ci_t c)
+{
+    return *(const T *)((const char *)base + (ci_t)(c * (ci_t)sizeof(T)));
+}
+
+template <typename T>
+__device__ __forceinline__ void st(T *__restrict__ base, ci_t c, T v)
+{
+    *(T *)((char *)base + (ci_t)(c * (ci_t)sizeof(T))) = v;
+}
+
+template <int D>
+__device__ __forceinline__ ci_t stride32(const Geo &g, int d)
+{
+    return d == 0 ? 1u : (d == 1 ? (ci_t)g.s1 : (ci_t)g.s2);
+}
+
 // one cell's state
 template <int D, typename T>
 struct St {
@@ -81,6 +106,22 @@ __device__ __forceinline__ void load_st(St<D, T> &s, const T *__restrict__ u,
     s.rb = bg[nc + c];
 #pragma unroll
     for (int d = 0; d < D; d++) s.bb[d] = bg[(idx_t)(2 + d) * nc + c];
+}
+
+template <int D, typename T>
+__device__ __forceinline__ void load_st32(St<D, T> &s, const T *__restrict__ u,
+                                          const double *__restrict__ bg, size_t nc, ci_t c)
+{
+    s.r = ld(u, c);
+#pragma unroll
+    for (int d = 0; d < D; d++) s.m[d] = ld(u + (1 + d) * nc, c);
+    s.e = ld(u + (1 + D) * nc, c);
+#pragma unroll
+    for (int d = 0; d < D; d++) s.b[d] = ld(u + (2 + D + d) * nc, c);
+    s.eb = ld(bg, c);
+    s.rb = ld(bg + nc, c);
+#pragma unroll
+    for (int d = 0; d < D; d++) s.bb[d] = ld(bg + (2 + d) * nc, c);
 }
 
 // the perturbations and rho_b only (what the hyperdiffusion variables need)

@@ -24,6 +24,7 @@ EXPORTS = [
     "smaug_last_error", "smaug_create", "smaug_destroy", "smaug_set_state", "smaug_get_state",
     "smaug_rhs", "smaug_cmax", "smaug_maxcourant", "smaug_rk4", "smaug_euler",
     "smaug_time_rk4", "smaug_ncell", "smaug_ninterior", "smaug_copy_probe",
+    "smaug_set_panel", "smaug_get_panel", "smaug_time_pass", "smaug_autotune",
     "smaug_ens_create", "smaug_ens_destroy", "smaug_ens_nrun", "smaug_ens_ninterior",
     "smaug_ens_set_state", "smaug_ens_get_state", "smaug_ens_rk4", "smaug_ens_time_rk4",
 ]
@@ -78,6 +79,11 @@ def load_library(path: str = LIB_PATH):
         lib.smaug_ninterior.restype = ctypes.c_longlong
         lib.smaug_copy_probe.argtypes = [ctypes.c_longlong, ctypes.c_int, dp]
         vp = ctypes.c_void_p
+        ip = ctypes.POINTER(ctypes.c_int)
+        lib.smaug_set_panel.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.smaug_get_panel.argtypes = [vp, ctypes.c_int, ip, ip]
+        lib.smaug_time_pass.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        lib.smaug_autotune.argtypes = [vp, ctypes.c_int, ip, dp]
         lib.smaug_ens_create.argtypes = [ctypes.POINTER(SmaugCfg), ctypes.c_int,
                                          ctypes.POINTER(vp)]
         lib.smaug_ens_destroy.argtypes = [vp]
@@ -185,6 +191,28 @@ class Engine:
         self._check(self.lib.smaug_time_rk4(self.h, t0, dt, nsteps, ctypes.byref(tot),
                                             ctypes.byref(st)))
         return tot.value, st.value
+
+
+    # launch-shape tuning: pass 0 = hyperviscosity ratios, 1 = stage update
+    def set_panel(self, pass_, tpx, tpy):
+        self._check(self.lib.smaug_set_panel(self.h, pass_, tpx, tpy))
+
+    def get_panel(self, pass_):
+        x, y = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.smaug_get_panel(self.h, pass_, ctypes.byref(x), ctypes.byref(y)))
+        return x.value, y.value
+
+    def time_pass(self, pass_, reps=5):
+        ms = ctypes.c_double()
+        self._check(self.lib.smaug_time_pass(self.h, pass_, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def autotune(self, reps=3):
+        """fastest XCD panel per pass; returns ({pass: (tpx, tpy)}, {pass: ms})"""
+        best = (ctypes.c_int * 4)()
+        ms = (ctypes.c_double * 2)()
+        self._check(self.lib.smaug_autotune(self.h, reps, best, ms))
+        return ({0: (best[0], best[1]), 1: (best[2], best[3])}, {0: ms[0], 1: ms[1]})
 
 
 class Ensemble:
