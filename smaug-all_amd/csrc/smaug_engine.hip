@@ -2,6 +2,7 @@
 // arena, launch geometry, the captured RK4 step graph, and the C-ABI of
 // smaug_engine.h.
 #include "smaug_kernels.hpp"
+#include "smaug_stream.hpp"
 #include "smaug_host.hpp"
 
 #include <stdarg.h>
@@ -77,6 +78,7 @@ struct smaug_engine {
     void *ACC;
     void *R;      // hyperviscosity ratios
     void *AUX;    // per-cell 1/rho, pressure, internal energy, velocity
+    void *HYP;    // 3-D: hyperdiffusive part of L(w) for the z-streamed update
     double *bg;
     double *d_dx[3], *d_idx[3], *d_prof[3];
     unsigned long long *d_cmax;
@@ -86,6 +88,9 @@ struct smaug_engine {
     hipGraphExec_t gexec[2]; // 0: RK4 step, 1: Euler step
     bool gvalid[2];
     int panel[2][2];         // XCD panel (tpx, tpy) of the ratio and update passes; 0 = default
+    int zstream;             // 3-D: 1 = z-streamed stage update, 0 = per-cell stage update
+    int kz;                  // z-stream depth (output planes per block)
+    int hyper_lds;           // 3-D: hyperdiffusive sources with the plane staged in LDS
 };
 
 // ---------------------------------------------------------------------------
@@ -153,6 +158,69 @@ static int launch_update(smaug_engine *e, StageArgs sa, const void *uin, const v
     return 0;
 }
 
+static ZMap make_zmap(const smaug_engine *e, unsigned *nblocks)
+{
+    ZMap m;
+    m.ntx = (e->n[0] - 4 + ZS_TI - 1) / ZS_TI;
+    m.nty = (e->n[1] - 4 + ZS_TJ - 1) / ZS_TJ;
+    const int nk = e->n[2] - 4;
+    m.kz = e->kz > 0 ? (e->kz < nk ? e->kz : nk) : (nk < 32 ? nk : 32);
+    m.nkz = (nk + m.kz - 1) / m.kz;
+    m.per_xcd = (m.ntx * m.nty + 7) / 8;
+    *nblocks = (unsigned)(8 * m.per_xcd * m.nkz);
+    return m;
+}
+
+template <typename T>
+static int launch_zstream(smaug_engine *e, StageArgs sa, const void *uin, const void *un,
+                          void *uout)
+{
+    unsigned nb;
+    ZMap m = make_zmap(e, &nb);
+    if (e->phys.hyperdif) {
+        unsigned nh;
+        TileMap tm = make_map(e->n[0] - 4, e->n[1] - 4, e->n[2] - 4, e->panel[1][0],
+                              e->panel[1][1], &nh);
+        if (e->hyper_lds) {
+            if constexpr (sizeof(T) == 4)
+                hipLaunchKernelGGL(k_hyper_lds_f, dim3(nh), dim3(256), 0, e->stream, e->geo, tm,
+                                   e->phys, (const T *)uin, (const T *)e->R, (const T *)e->AUX,
+                                   (const double *)e->bg, (const unsigned long long *)e->d_cmax,
+                                   (T *)e->HYP);
+            else
+                hipLaunchKernelGGL(k_hyper_lds_d, dim3(nh), dim3(256), 0, e->stream, e->geo, tm,
+                                   e->phys, (const T *)uin, (const T *)e->R, (const T *)e->AUX,
+                                   (const double *)e->bg, (const unsigned long long *)e->d_cmax,
+                                   (T *)e->HYP);
+        } else {
+            hipLaunchKernelGGL((k_hyper<3, T>), dim3(nh), dim3(256), 0, e->stream, e->geo, tm,
+                               e->phys, (const T *)uin, (const T *)e->R, (const T *)e->AUX,
+                               (const double *)e->bg, (const unsigned long long *)e->d_cmax,
+                               (T *)e->HYP);
+        }
+        HIPCHK(hipGetLastError());
+    }
+    if constexpr (sizeof(T) == 4)
+        hipLaunchKernelGGL(k_zstream_f, dim3(nb), dim3(ZS_NT), 0, e->stream, e->geo, e->phys, sa, m,
+                           (const T *)uin, (const T *)un, (T *)uout, (T *)e->ACC,
+                           (const T *)e->HYP, (const double *)e->bg);
+    else
+        hipLaunchKernelGGL(k_zstream_d, dim3(nb), dim3(ZS_NT), 0, e->stream, e->geo, e->phys, sa, m,
+                           (const T *)uin, (const T *)un, (T *)uout, (T *)e->ACC,
+                           (const T *)e->HYP, (const double *)e->bg);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+template <int D, typename T>
+static int launch_stage_update(smaug_engine *e, StageArgs sa, const void *uin, const void *un,
+                               void *uout)
+{
+    if constexpr (D == 3)
+        if (e->zstream) return launch_zstream<T>(e, sa, uin, un, uout);
+    return launch_update<D, T>(e, sa, uin, un, uout);
+}
+
 template <int D, typename T>
 static int launch_bc(smaug_engine *e, void *u)
 {
@@ -179,7 +247,7 @@ static int stage(smaug_engine *e, StageArgs sa, const void *uin, const void *un,
     if (e->phys.hyperdif)
         if ((rc = launch_ratio<D, T>(e, uin))) return rc;
     if (ev0) HIPCHK(hipEventRecord(ev0, e->stream));
-    if ((rc = launch_update<D, T>(e, sa, uin, un, uout))) return rc;
+    if ((rc = launch_stage_update<D, T>(e, sa, uin, un, uout))) return rc;
     if (ev1) HIPCHK(hipEventRecord(ev1, e->stream));
     if (sa.mode != 2)
         if ((rc = launch_bc<D, T>(e, uout))) return rc;
@@ -255,6 +323,7 @@ static void free_engine(smaug_engine *e)
     if (e->ACC) hipFree(e->ACC);
     if (e->R) hipFree(e->R);
     if (e->AUX) hipFree(e->AUX);
+    if (e->HYP) hipFree(e->HYP);
     if (e->bg) hipFree(e->bg);
     if (e->d_cmax) hipFree(e->d_cmax);
     if (e->d_tm) hipFree(e->d_tm);
@@ -305,6 +374,7 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
     TRYALLOC(e->ACC, e->tsz * nve * e->nc);
     if (cfg->hyperdif) TRYALLOC(e->R, e->tsz * npair * e->nc);
     TRYALLOC(e->AUX, e->tsz * (3 + D) * e->nc);
+    if (D == 3 && cfg->hyperdif) TRYALLOC(e->HYP, e->tsz * nve * e->nc);
     TRYALLOC(e->bg, sizeof(double) * nbg * e->nc);
     TRYALLOC(e->d_cmax, sizeof(unsigned long long) * 4);
     TRYALLOC(e->d_tm, sizeof(double) * 2);
@@ -337,6 +407,9 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
     for (int k = 0; k < 8; k++) p.drv[k] = cfg->drv[k];
     p.hyperdif = cfg->hyperdif;
     p.driver = cfg->driver;
+    e->zstream = D == 3;
+    e->kz = 0;
+    e->hyper_lds = 1;
     *out = e;
     return 0;
 }
@@ -624,7 +697,7 @@ static int time_pass(smaug_engine *e, int pass, int reps, double *ms)
     for (int r = -1; r < reps; r++) {
         if (r == 0) HIPCHK(hipEventRecord(a, e->stream));
         rc = pass == 0 ? launch_ratio<D, T>(e, e->U[0])
-                       : launch_update<D, T>(e, sa, e->U[0], e->U[0], e->U[1]);
+                       : launch_stage_update<D, T>(e, sa, e->U[0], e->U[0], e->U[1]);
         if (rc) return rc;
     }
     HIPCHK(hipEventRecord(b, e->stream));
@@ -637,6 +710,17 @@ static int time_pass(smaug_engine *e, int pass, int reps, double *ms)
     return 0;
 }
 
+extern "C" int smaug_set_zstream(smaug_engine *e, int enable, int kz)
+{
+    if (!e || kz < 0) return fail(SMAUG_EINVAL, "bad argument");
+    if (enable && e->D != 3) return fail(SMAUG_EINVAL, "the z-streamed update is for 3-D grids");
+    e->zstream = enable ? 1 : 0;
+    e->hyper_lds = enable != 2;
+    e->kz = kz;
+    drop_graphs(e);
+    return 0;
+}
+
 extern "C" int smaug_time_pass(smaug_engine *e, int pass, int reps, double *ms)
 {
     if (!e || pass < 0 || pass > 1 || reps < 1 || !ms) return fail(SMAUG_EINVAL, "bad argument");
@@ -645,11 +729,15 @@ extern "C" int smaug_time_pass(smaug_engine *e, int pass, int reps, double *ms)
 }
 
 // picks, for the ratio and update passes, the fastest XCD panel among
-// tpx in {1, 2, 4, 8} and tpy in {1, 2, 4, 8, 16, 32}
+// tpx in {1, 2, 4, 8} and tpy in {1, 2, 4, 8, 16, 32}; for 3-D grids the
+// update pass also tries the z-streamed kernel at z-stream depths
+// 4, 8, 16, 32 and 64 and keeps whichever form is fastest
 extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double best_ms[2])
 {
     if (!e || reps < 1) return fail(SMAUG_EINVAL, "bad argument");
     static const int TX[] = {1, 2, 4, 8}, TY[] = {1, 2, 4, 8, 16, 32};
+    const int zs0 = e->zstream;
+    e->zstream = 0;
     for (int pass = 0; pass < 2; pass++) {
         if (pass == 0 && !e->phys.hyperdif) {
             if (best) best[0][0] = best[0][1] = 0;
@@ -676,13 +764,40 @@ extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double 
             }
         e->panel[pass][0] = bx;
         e->panel[pass][1] = by;
+        if (pass == 1 && e->D == 3) {
+            static const int KZ[] = {4, 8, 16, 32, 64};
+            int bkz = -1;
+            e->zstream = 1;
+            for (int kz : KZ) {
+                e->kz = kz;
+                double ms;
+                int rc = DISPATCH(e, time_pass, e, pass, reps, &ms);
+                if (rc) return rc;
+                if (ms < bm) {
+                    bm = ms;
+                    bkz = kz;
+                }
+            }
+            e->zstream = bkz > 0;
+            e->kz = bkz > 0 ? bkz : 0;
+        }
         if (best) {
             best[pass][0] = bx;
             best[pass][1] = by;
         }
         if (best_ms) best_ms[pass] = bm;
     }
+    if (e->D != 3) e->zstream = zs0;
     drop_graphs(e);
+    return 0;
+}
+
+extern "C" int smaug_get_zstream(smaug_engine *e, int *enable, int *kz)
+{
+    if (!e || !enable || !kz) return fail(SMAUG_EINVAL, "bad argument");
+    *enable = e->zstream ? (e->hyper_lds ? 1 : 2) : 0;
+    unsigned nb;
+    *kz = e->D == 3 ? make_zmap(e, &nb).kz : 0;
     return 0;
 }
 

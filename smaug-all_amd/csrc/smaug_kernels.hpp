@@ -98,7 +98,6 @@ __global__ __launch_bounds__(256) void k_prep(Geo g, Phys ph, const T *__restric
                                                const double *__restrict__ bg,
                                                T *__restrict__ aux, unsigned long long *cm)
 {
-    typedef Aux<D> AX;
     __shared__ double red[D][4];
     double mx[D];
 #pragma unroll
@@ -118,21 +117,33 @@ __global__ __launch_bounds__(256) void k_prep(Geo g, Phys ph, const T *__restric
 // (hyperdifvisc1r.cu:426-1300, hyperdifvisc1l.cu:180-1450).
 // Computed on the box [1, n-3] in every dim; R is stored as T.
 // ---------------------------------------------------------------------------
+// R of one window.  D3 is formed with one fused multiply-add from the two
+// first differences, so it carries a single rounding even where it is a
+// small remainder of nearly equal terms (smooth data); in FP32 the
+// differences of the FP32 state are then exact to one rounding, as in FP64.
 template <typename T>
 __device__ __forceinline__ T ratio6(const T q[6])
 {
+    T d[5];
+#pragma unroll
+    for (int f = 0; f < 5; f++) d[f] = q[f + 1] - q[f];
     T m1 = 0, m3 = 0;
 #pragma unroll
     for (int f = 0; f < 5; f++) {
-        T v = fabs(q[f + 1] - q[f]);
+        const T v = fabs(d[f]);
         m1 = v > m1 ? v : m1;
     }
 #pragma unroll
     for (int f = 1; f < 4; f++) {
-        T v = fabs((T)3.0 * (q[f + 1] - q[f]) - (q[f + 2] - q[f - 1]));
+        // 3 (q[f+1]-q[f]) - (q[f+2]-q[f-1])
+        const T v = fabs(fma((T)3, d[f], -(q[f + 2] - q[f - 1])));
         m3 = v > m3 ? v : m3;
     }
-    return m1 > (T)0 ? m3 / m1 : (T)0;
+    if (!(m1 > (T)0)) return (T)0;
+    if constexpr (sizeof(T) == 4)
+        return __fdividef(m3, m1);
+    else
+        return m3 / m1;
 }
 
 template <int D, typename T>
@@ -150,9 +161,7 @@ __device__ __forceinline__ void ratio_cell(const Geo &g, const T *__restrict__ u
         int x = xs[d], n = g.n[d];
         const ci_t sd = stride32<D>(g, d);
         const ci_t c0 = c - (ci_t)x * sd;
-        // differences and ratios in double in both paths: third differences
-        // of nearly smooth FP32 data sit near the FP32 rounding level
-        double qr[6], qe[6], qv[D][6], qb[D][6];
+        T qr[6], qe[6], qv[D][6], qb[D][6];
 #pragma unroll
         for (int p = 0; p < 6; p++) {
             int xx = x - 2 + p;
@@ -167,13 +176,13 @@ __device__ __forceinline__ void ratio_cell(const Geo &g, const T *__restrict__ u
                 qb[f][p] = ld(u + (LY::B0 + f) * nc, cp);
             }
         }
-        st(R + LY::prho(d) * nc, c, (T)ratio6<double>(qr));
-        st(R + LY::pe(d) * nc, c, (T)ratio6<double>(qe));
+        st(R + LY::prho(d) * nc, c, ratio6<T>(qr));
+        st(R + LY::pe(d) * nc, c, ratio6<T>(qe));
 #pragma unroll
-        for (int f = 0; f < D; f++) st(R + LY::pv(f, d) * nc, c, (T)ratio6<double>(qv[f]));
+        for (int f = 0; f < D; f++) st(R + LY::pv(f, d) * nc, c, ratio6<T>(qv[f]));
 #pragma unroll
         for (int f = 0; f < D; f++)
-            if (f != d) st(R + LY::pb(f, d) * nc, c, (T)ratio6<double>(qb[f]));
+            if (f != d) st(R + LY::pb(f, d) * nc, c, ratio6<T>(qb[f]));
     }
 }
 
@@ -267,6 +276,294 @@ struct StageArgs {
     double ct;     // stage time offset in units of dt (0, 1/2, 1/2, 1)
     const double *tm; // device [t, dt]
 };
+
+// Global-memory view of the hyperdiffusion inputs around cell c: field f of
+// the state, aux quantity a, rho_b and ratio pair p at c + (di, dj, dk).
+template <int D, typename T>
+struct HyGlobal {
+    const T *__restrict__ uin;
+    const T *__restrict__ aux;
+    const double *__restrict__ bg;
+    const T *__restrict__ R;
+    size_t nc;
+    ci_t c, s1, s2;
+    __device__ __forceinline__ ci_t at(int di, int dj, int dk) const
+    {
+        return c + (ci_t)di + s1 * (ci_t)dj + s2 * (ci_t)dk;
+    }
+    __device__ __forceinline__ T U(int f, int di, int dj, int dk) const
+    {
+        return ld(uin + f * nc, at(di, dj, dk));
+    }
+    __device__ __forceinline__ T A(int a, int di, int dj, int dk) const
+    {
+        return ld(aux + a * nc, at(di, dj, dk));
+    }
+    __device__ __forceinline__ double rhob(int di, int dj, int dk) const
+    {
+        return ld(bg + nc, at(di, dj, dk));
+    }
+    __device__ __forceinline__ T Rp(int p, int di, int dj, int dk) const
+    {
+        return ld(R + p * nc, at(di, dj, dk));
+    }
+};
+
+// unit displacement along dim d, scaled by s
+template <int D>
+__device__ __forceinline__ void unit(int d, int s, int &di, int &dj, int &dk)
+{
+    di = d == 0 ? s : 0;
+    dj = d == 1 ? s : 0;
+    dk = d == 2 ? s : 0;
+}
+
+// the hyperdiffusive sources of interior cell (x0, x1, x2) added to dU
+// (iosmaug.cpp:969-1146); inputs through the view V (HyGlobal or an LDS
+// staged view with the same members)
+template <int D, typename T, typename V>
+__device__ __forceinline__ void hyper_terms(const Geo &g, const Phys &ph, const V &vw,
+                                            const unsigned long long *__restrict__ cmb,
+                                            const int x0, const int x1, const int x2, T *dU)
+{
+    typedef Lay<D> LY;
+    typedef Aux<D> AX;
+    const int xs[3] = {x0, x1, x2};
+    const T rc = vw.U(LY::RHO, 0, 0, 0);
+    const double rbc = vw.rhob(0, 0, 0);
+    double cmax[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) cmax[d] = __longlong_as_double((long long)cmb[d]);
+    const T eic = vw.A(AX::EINT, 0, 0, 0);
+    const double rhoc = (double)rc + rbc;
+
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        int ri, rj, rk, li, lj, lk;
+        unit<D>(d, 1, ri, rj, rk);
+        unit<D>(d, -1, li, lj, lk);
+        const int xd = xs[d], nd = g.n[d];
+        const double dxd = g.dx[d][xd];
+        const T idxd = (T)g.idx[d][xd];
+        const bool z1 = zero1(g.per[d], xd, nd);
+
+        auto nuRL = [&](int pair, int field, T &nR, T &nL) {
+            const T K = (T)(dxd * cmax[d] * ph.chyp[field]);
+            nR = K * vw.Rp(pair, 0, 0, 0);
+            nL = K * vw.Rp(pair, li, lj, lk);
+        };
+        T nR, nL, gR, gL;
+        // density (hyperdifrhosource1.cu:15-120)
+        nuRL(LY::prho(d), LY::RHO, nR, nL);
+        gR = z1 ? (T)0 : (vw.U(LY::RHO, ri, rj, rk) - rc) * idxd;
+        gL = z1 ? (T)0 : (rc - vw.U(LY::RHO, li, lj, lk)) * idxd;
+        dU[LY::RHO] += (nR * gR - nL * gL) * idxd;
+        // energy (hyperdifesource1.cu:18-330)
+        nuRL(LY::pe(d), LY::EN, nR, nL);
+        gR = z1 ? (T)0 : (vw.A(AX::EINT, ri, rj, rk) - eic) * idxd;
+        gL = z1 ? (T)0 : (eic - vw.A(AX::EINT, li, lj, lk)) * idxd;
+        dU[LY::EN] += (nR * gR - nL * gL) * idxd;
+        // momentum, diagonal part for every f; twice when f == d
+        // (hyperdifmomsource1.cu:17-533 called for ii1 = 0 and, if f == d, ii1 = 1)
+        const T rR = (T)((rhoc + (double)vw.U(LY::RHO, ri, rj, rk) + vw.rhob(ri, rj, rk)) / 2);
+        const T rL = (T)((rhoc + (double)vw.U(LY::RHO, li, lj, lk) + vw.rhob(li, lj, lk)) / 2);
+#pragma unroll
+        for (int f = 0; f < D; f++) {
+            nuRL(LY::pv(f, d), LY::M0 + f, nR, nL);
+            const T vc = vw.A(AX::V0 + f, 0, 0, 0);
+            gR = z1 ? (T)0 : (vw.A(AX::V0 + f, ri, rj, rk) - vc) * idxd;
+            gL = z1 ? (T)0 : (vc - vw.A(AX::V0 + f, li, lj, lk)) * idxd;
+            const T mc = vw.U(LY::M0 + f, 0, 0, 0);
+            const T mR = (mc + vw.U(LY::M0 + f, ri, rj, rk)) / (T)2;
+            const T mL = (mc + vw.U(LY::M0 + f, li, lj, lk)) / (T)2;
+            const T k2 = (f == d) ? (T)1 : (T)0.5;
+            dU[LY::M0 + f] += k2 * (rR * nR * gR - rL * nL * gL) * idxd;
+            dU[LY::EN] += k2 * (mR * nR * gR - mL * nL * gL) * idxd;
+        }
+        // magnetic field, diagonal part for f != d (hyperdifbsource1.cu:21-440)
+#pragma unroll
+        for (int f = 0; f < D; f++) {
+            if (f == d) continue;
+            nuRL(LY::pb(f, d), LY::B0 + f, nR, nL);
+            const T bc_ = vw.U(LY::B0 + f, 0, 0, 0);
+            const T br = vw.U(LY::B0 + f, ri, rj, rk), bl = vw.U(LY::B0 + f, li, lj, lk);
+            gR = z1 ? (T)0 : (br - bc_) * idxd;
+            gL = z1 ? (T)0 : (bc_ - bl) * idxd;
+            const T bR = (bc_ + br) / (T)2, bL = (bc_ + bl) / (T)2;
+            dU[LY::B0 + f] += (gR * nR - gL * nL) * idxd;
+            dU[LY::EN] += (bR * gR * nR - bL * gL * nL) * idxd;
+        }
+
+        // off-diagonal terms for f != d: d/dx_f of products evaluated at
+        // c -/+ e_f (hyperdifmomsourcene1.cu:16-740, hyperdifbsourcene1.cu:18-820)
+#pragma unroll
+        for (int f = 0; f < D; f++) {
+            if (f == d) continue;
+            const int xf = xs[f], nf = g.n[f];
+            if (zero1(g.per[f], xf, nf)) continue;
+            T Xr[2], Xm[2], Hb[2], He[2];
+#pragma unroll
+            for (int side = 0; side < 2; side++) {
+                const int sg = side ? 1 : -1;
+                int ni, nj, nk;
+                unit<D>(f, sg, ni, nj, nk); // cn = c + sg e_f
+                Xr[side] = Xm[side] = Hb[side] = He[side] = 0;
+                const int xfn = xf + sg;
+                if (xfn < 2 || xfn > nf - 3) continue; // nu vanishes off the interior
+                // ca = cn - e_d, cb = cn + e_d
+                const int ai = ni + li, aj = nj + lj, ak = nk + lk;
+                const int bi = ni + ri, bj = nj + rj, bk = nk + rk;
+                const T dv = z1 ? (T)0
+                                : (vw.A(AX::V0 + f, bi, bj, bk) - vw.A(AX::V0 + f, ai, aj, ak)) *
+                                      idxd / (T)2;
+                const T db = z1 ? (T)0
+                                : (vw.U(LY::B0 + f, bi, bj, bk) - vw.U(LY::B0 + f, ai, aj, ak)) *
+                                      idxd / (T)2;
+                {
+                    const T K = (T)(dxd * cmax[d] * ph.chyp[LY::M0 + f]);
+                    const T s2 = K * (vw.Rp(LY::pv(f, d), ni, nj, nk) +
+                                      vw.Rp(LY::pv(f, d), ai, aj, ak));
+                    const T G = dv * s2 / (T)4;
+                    const T rhon = (T)((double)vw.U(LY::RHO, ni, nj, nk) + vw.rhob(ni, nj, nk));
+                    Xr[side] = rhon * G;
+                    Xm[side] = G * vw.U(LY::M0 + d, ni, nj, nk);
+                }
+                {
+                    const T K = (T)(dxd * cmax[d] * ph.chyp[LY::B0 + f]);
+                    const T s2 = K * (vw.Rp(LY::pb(f, d), ni, nj, nk) +
+                                      vw.Rp(LY::pb(f, d), ai, aj, ak));
+                    const T H = db * s2 / (T)2;
+                    Hb[side] = H;
+                    He[side] = H * vw.U(LY::B0 + d, ni, nj, nk);
+                }
+            }
+            const T h = (T)g.idx[f][xf] / (T)2;
+            dU[LY::M0 + d] += (Xr[1] - Xr[0]) * h;
+            dU[LY::EN] += (Xm[1] - Xm[0]) * h;
+            dU[LY::B0 + d] -= (Hb[1] - Hb[0]) * h;
+            dU[LY::EN] -= (He[1] - He[0]) * h;
+        }
+    }
+}
+
+template <int D, typename T>
+__device__ __forceinline__ void hyper_cell(const Geo &g, const Phys &ph, const T *__restrict__ uin,
+                                           const T *__restrict__ R, const T *__restrict__ aux,
+                                           const double *__restrict__ bg,
+                                           const unsigned long long *__restrict__ cmb,
+                                           const int x0, const int x1, const int x2, T *dU)
+{
+    HyGlobal<D, T> vw;
+    vw.uin = uin;
+    vw.aux = aux;
+    vw.bg = bg;
+    vw.R = R;
+    vw.nc = (size_t)g.nc;
+    vw.s1 = (ci_t)g.s1;
+    vw.s2 = (ci_t)g.s2;
+    vw.c = (ci_t)x0 + vw.s1 * (ci_t)x1 + vw.s2 * (ci_t)x2;
+    hyper_terms<D, T>(g, ph, vw, cmb, x0, x1, x2, dU);
+}
+
+// the user driver (vertical velocity, m_0 += rho v, e += rho v^2/2) at
+// interior cell (x0, x1, x2) added to dU (usersource.cu:4-92)
+template <int D, typename T>
+__device__ __forceinline__ void driver_cell(const Geo &g, const Phys &ph, const StageArgs &sa,
+                                            const T rc, const double rbc, const int x0,
+                                            const int x1, const int x2, T *dU)
+{
+    typedef Lay<D> LY;
+    const double t = sa.tm[0], dt = sa.tm[1];
+    if (ph.driver == 1) {
+        const double ts = t + sa.ct * dt;
+        const double tdep = sin(ts * 6.283185307179586 / ph.drv[1]);
+        double pr = ph.drv[0] * tdep * g.prof[0][x0] * g.prof[1][x1];
+        if (D == 3) pr *= g.prof[2][x2];
+        const T v = (T)pr;
+        const T rfull = (T)((double)rc + rbc);
+        dU[LY::M0] += v * rfull;
+        dU[LY::EN] += v * v * rfull / (T)2;
+    }
+
+}
+
+// the stage result of cell c from its L(w) = dU (mode 0: low-storage RK4
+// slot algebra, 1: forward Euler, 2: raw L(w))
+template <int D, typename T>
+__device__ __forceinline__ void rk_write(const StageArgs &sa, const T *__restrict__ un,
+                                         T *__restrict__ uout, T *__restrict__ acc, size_t nc,
+                                         ci_t c, const T *dU)
+{
+    typedef Lay<D> LY;
+    const double dt = sa.tm[1];
+    if (sa.mode == 2) {
+#pragma unroll
+        for (int f = 0; f < LY::NVE; f++) st(uout + f * nc, c, dU[f]);
+        return;
+    }
+    if (sa.mode == 1) {
+        const T tdt = (T)dt;
+#pragma unroll
+        for (int f = 0; f < LY::NVE; f++)
+            st(uout + f * nc, c, ld(un + f * nc, c) + tdt * dU[f]);
+        return;
+    }
+    // low-storage classical RK4:
+    //  s0: out = un + dt/2 k, acc = un + dt/6 k
+    //  s1: out = un + dt/2 k, acc += dt/3 k
+    //  s2: out = un + dt k,   acc += dt/3 k
+    //  s3: out = acc + dt/6 k
+    const int s = sa.stage;
+    const T a = (T)(s == 2 ? dt : (s == 3 ? dt / 6 : dt / 2));
+    const T b = (T)(s == 0 ? dt / 6 : dt / 3);
+#pragma unroll
+    for (int f = 0; f < LY::NVE; f++) {
+        const size_t o = f * nc;
+        if (s == 3) {
+            st(uout + o, c, ld(acc + o, c) + a * dU[f]);
+        } else {
+            const T base = ld(un + o, c);
+            st(uout + o, c, base + a * dU[f]);
+            st(acc + o, c, (s == 0 ? base : ld(acc + o, c)) + b * dU[f]);
+        }
+    }
+}
+
+
+// rk_write with the old values of the cell already loaded: un_v (modes 1
+// and RK stages 0-2) and acc_v (RK stages 1-3)
+template <int D, typename T>
+__device__ __forceinline__ void rk_write_pre(const StageArgs &sa, T *__restrict__ uout,
+                                             T *__restrict__ acc, size_t nc, ci_t c,
+                                             const T *dU, const T *un_v, const T *acc_v)
+{
+    typedef Lay<D> LY;
+    const double dt = sa.tm[1];
+    if (sa.mode == 2) {
+#pragma unroll
+        for (int f = 0; f < LY::NVE; f++) st(uout + f * nc, c, dU[f]);
+        return;
+    }
+    if (sa.mode == 1) {
+        const T tdt = (T)dt;
+#pragma unroll
+        for (int f = 0; f < LY::NVE; f++) st(uout + f * nc, c, un_v[f] + tdt * dU[f]);
+        return;
+    }
+    const int s = sa.stage;
+    const T a = (T)(s == 2 ? dt : (s == 3 ? dt / 6 : dt / 2));
+    const T b = (T)(s == 0 ? dt / 6 : dt / 3);
+#pragma unroll
+    for (int f = 0; f < LY::NVE; f++) {
+        const size_t o = f * nc;
+        if (s == 3) {
+            st(uout + o, c, acc_v[f] + a * dU[f]);
+        } else {
+            st(uout + o, c, un_v[f] + a * dU[f]);
+            st(acc + o, c, (s == 0 ? un_v[f] : acc_v[f]) + b * dU[f]);
+        }
+    }
+}
 
 // the whole stage update of interior cell (x0, x1, x2)
 template <int D, typename T>
@@ -376,159 +673,10 @@ __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const 
         dU[LY::EN] += rc * ge * A(AX::IRHO, c);
     }
 
-    // ---- hyperdiffusion (iosmaug.cpp:969-1146)
-    if (ph.hyperdif) {
-        double cmax[D];
-#pragma unroll
-        for (int d = 0; d < D; d++) cmax[d] = __longlong_as_double((long long)cmb[d]);
-        const T eic = A(AX::EINT, c);
-        const double rhoc = (double)rc + rbc;
+    if (ph.hyperdif) hyper_cell<D, T>(g, ph, uin, R, aux, bg, cmb, x0, x1, x2, dU);
 
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-            const ci_t sd = stride32<D>(g, d);
-            const int xd = xs[d], nd = g.n[d];
-            const double dxd = g.dx[d][xd];
-            const T idxd = (T)g.idx[d][xd];
-            const bool z1 = zero1(g.per[d], xd, nd);
-            const ci_t cl = c - sd, cr = c + sd;
-
-            auto nuRL = [&](int pair, int field, T &nR, T &nL) {
-                const T K = (T)(dxd * cmax[d] * ph.chyp[field]);
-                nR = K * ld(R + pair * nc, c);
-                nL = K * ld(R + pair * nc, cl);
-            };
-            T nR, nL, gR, gL;
-            // density (hyperdifrhosource1.cu:15-120)
-            nuRL(LY::prho(d), LY::RHO, nR, nL);
-            gR = z1 ? (T)0 : (U(LY::RHO, cr) - rc) * idxd;
-            gL = z1 ? (T)0 : (rc - U(LY::RHO, cl)) * idxd;
-            dU[LY::RHO] += (nR * gR - nL * gL) * idxd;
-            // energy (hyperdifesource1.cu:18-330)
-            nuRL(LY::pe(d), LY::EN, nR, nL);
-            gR = z1 ? (T)0 : (A(AX::EINT, cr) - eic) * idxd;
-            gL = z1 ? (T)0 : (eic - A(AX::EINT, cl)) * idxd;
-            dU[LY::EN] += (nR * gR - nL * gL) * idxd;
-            // momentum, diagonal part for every f; twice when f == d
-            // (hyperdifmomsource1.cu:17-533 called for ii1 = 0 and, if f == d, ii1 = 1)
-            const T rR = (T)((rhoc + (double)U(LY::RHO, cr) + BG(1, cr)) / 2);
-            const T rL = (T)((rhoc + (double)U(LY::RHO, cl) + BG(1, cl)) / 2);
-#pragma unroll
-            for (int f = 0; f < D; f++) {
-                nuRL(LY::pv(f, d), LY::M0 + f, nR, nL);
-                const T vc = A(AX::V0 + f, c);
-                gR = z1 ? (T)0 : (A(AX::V0 + f, cr) - vc) * idxd;
-                gL = z1 ? (T)0 : (vc - A(AX::V0 + f, cl)) * idxd;
-                const T mc = U(LY::M0 + f, c);
-                const T mR = (mc + U(LY::M0 + f, cr)) / (T)2, mL = (mc + U(LY::M0 + f, cl)) / (T)2;
-                const T k2 = (f == d) ? (T)1 : (T)0.5;
-                dU[LY::M0 + f] += k2 * (rR * nR * gR - rL * nL * gL) * idxd;
-                dU[LY::EN] += k2 * (mR * nR * gR - mL * nL * gL) * idxd;
-            }
-            // magnetic field, diagonal part for f != d (hyperdifbsource1.cu:21-440)
-#pragma unroll
-            for (int f = 0; f < D; f++) {
-                if (f == d) continue;
-                nuRL(LY::pb(f, d), LY::B0 + f, nR, nL);
-                const T bc_ = U(LY::B0 + f, c), br = U(LY::B0 + f, cr), bl = U(LY::B0 + f, cl);
-                gR = z1 ? (T)0 : (br - bc_) * idxd;
-                gL = z1 ? (T)0 : (bc_ - bl) * idxd;
-                const T bR = (bc_ + br) / (T)2, bL = (bc_ + bl) / (T)2;
-                dU[LY::B0 + f] += (gR * nR - gL * nL) * idxd;
-                dU[LY::EN] += (bR * gR * nR - bL * gL * nL) * idxd;
-            }
-
-            // off-diagonal terms for f != d: d/dx_f of products evaluated at
-            // c -/+ e_f (hyperdifmomsourcene1.cu:16-740, hyperdifbsourcene1.cu:18-820)
-#pragma unroll
-            for (int f = 0; f < D; f++) {
-                if (f == d) continue;
-                const ci_t sf = stride32<D>(g, f);
-                const int xf = xs[f], nf = g.n[f];
-                if (zero1(g.per[f], xf, nf)) continue;
-                T Xr[2], Xm[2], Hb[2], He[2];
-#pragma unroll
-                for (int side = 0; side < 2; side++) {
-                    const int sg = side ? 1 : -1;
-                    const ci_t cn = side ? c + sf : c - sf;
-                    Xr[side] = Xm[side] = Hb[side] = He[side] = 0;
-                    const int xfn = xf + sg;
-                    if (xfn < 2 || xfn > nf - 3) continue; // nu vanishes off the interior
-                    const ci_t ca = cn - sd, cb = cn + sd;
-                    const T dv = z1 ? (T)0 : (A(AX::V0 + f, cb) - A(AX::V0 + f, ca)) * idxd / (T)2;
-                    const T db = z1 ? (T)0 : (U(LY::B0 + f, cb) - U(LY::B0 + f, ca)) * idxd / (T)2;
-                    {
-                        const T K = (T)(dxd * cmax[d] * ph.chyp[LY::M0 + f]);
-                        const T s2 = K * (ld(R + LY::pv(f, d) * nc, cn) +
-                                          ld(R + LY::pv(f, d) * nc, cn - sd));
-                        const T G = dv * s2 / (T)4;
-                        const T rhon = (T)((double)U(LY::RHO, cn) + BG(1, cn));
-                        Xr[side] = rhon * G;
-                        Xm[side] = G * U(LY::M0 + d, cn);
-                    }
-                    {
-                        const T K = (T)(dxd * cmax[d] * ph.chyp[LY::B0 + f]);
-                        const T s2 = K * (ld(R + LY::pb(f, d) * nc, cn) +
-                                          ld(R + LY::pb(f, d) * nc, cn - sd));
-                        const T H = db * s2 / (T)2;
-                        Hb[side] = H;
-                        He[side] = H * U(LY::B0 + d, cn);
-                    }
-                }
-                const T h = (T)g.idx[f][xf] / (T)2;
-                dU[LY::M0 + d] += (Xr[1] - Xr[0]) * h;
-                dU[LY::EN] += (Xm[1] - Xm[0]) * h;
-                dU[LY::B0 + d] -= (Hb[1] - Hb[0]) * h;
-                dU[LY::EN] -= (He[1] - He[0]) * h;
-            }
-        }
-    }
-
-    // ---- user driver: vertical velocity, m_0 += rho v, e += rho v^2/2
-    const double t = sa.tm[0], dt = sa.tm[1];
-    if (ph.driver == 1) {
-        const double ts = t + sa.ct * dt;
-        const double tdep = sin(ts * 6.283185307179586 / ph.drv[1]);
-        double pr = ph.drv[0] * tdep * g.prof[0][x0] * g.prof[1][x1];
-        if (D == 3) pr *= g.prof[2][x2];
-        const T v = (T)pr;
-        const T rfull = (T)((double)rc + rbc);
-        dU[LY::M0] += v * rfull;
-        dU[LY::EN] += v * v * rfull / (T)2;
-    }
-
-    // ---- write
-    if (sa.mode == 2) {
-#pragma unroll
-        for (int f = 0; f < LY::NVE; f++) st(uout + f * nc, c, dU[f]);
-        return;
-    }
-    if (sa.mode == 1) {
-        const T tdt = (T)dt;
-#pragma unroll
-        for (int f = 0; f < LY::NVE; f++)
-            st(uout + f * nc, c, ld(un + f * nc, c) + tdt * dU[f]);
-        return;
-    }
-    // low-storage classical RK4:
-    //  s0: out = un + dt/2 k, acc = un + dt/6 k
-    //  s1: out = un + dt/2 k, acc += dt/3 k
-    //  s2: out = un + dt k,   acc += dt/3 k
-    //  s3: out = acc + dt/6 k
-    const int s = sa.stage;
-    const T a = (T)(s == 2 ? dt : (s == 3 ? dt / 6 : dt / 2));
-    const T b = (T)(s == 0 ? dt / 6 : dt / 3);
-#pragma unroll
-    for (int f = 0; f < LY::NVE; f++) {
-        const size_t o = f * nc;
-        if (s == 3) {
-            st(uout + o, c, ld(acc + o, c) + a * dU[f]);
-        } else {
-            const T base = ld(un + o, c);
-            st(uout + o, c, base + a * dU[f]);
-            st(acc + o, c, (s == 0 ? base : ld(acc + o, c)) + b * dU[f]);
-        }
-    }
+    driver_cell<D, T>(g, ph, sa, rc, rbc, x0, x1, x2, dU);
+    rk_write<D, T>(sa, un, uout, acc, nc, c, dU);
 }
 
 template <int D, typename T>

@@ -1,0 +1,605 @@
+// smaug_stream.hpp -- the 2.5-D z-streamed stage update for 3-D grids.
+//
+// A block owns a tile of 64 x 4 columns (i fastest, then j) and walks its
+// columns upward in k over a chunk of kz output planes.  For every plane it
+// stages, in LDS, the advective fluxes along i and j and the velocity of
+// the tile plus a 2-cell halo (one global read of each cell's state per
+// block and plane); the 4th-order differences along i and j then read LDS.
+// Along k each thread keeps a ring of five pending outputs in registers: the
+// k-flux of plane P is formed once and scattered into outputs P-2..P+2, so
+// no plane is read twice along the stream.  An output plane is complete two
+// planes behind the front, where the hyperdiffusive sources (formed by
+// k_hyper into their own buffer), the driver and the RK slot algebra are
+// applied and the result written.
+//
+// The per-cell arithmetic is the reference operator of centdiff1.cu:19-256
+// and centdiff2.cu:20-356 (fluxes and background work terms), addgrav_cd2
+// (centdiff2.cu:118-154), and the hyperdiffusion, driver and RK combination
+// of smaug_kernels.hpp.  Only the order in which the twelve neighbour
+// contributions are summed differs from the per-cell kernel.
+#pragma once
+#include "smaug_kernels.hpp"
+
+namespace smaug {
+
+constexpr int ZS_TI = 64, ZS_TJ = 4;                // output columns per block
+constexpr int ZS_BI = ZS_TI + 4, ZS_BJ = ZS_TJ + 4; // LDS box incl. 2-cell halo
+constexpr int ZS_NB = ZS_BI * ZS_BJ;
+constexpr int ZS_NT = ZS_TI * ZS_TJ;                // threads per block
+constexpr int ZS_NH = ZS_NB - ZS_NT;                // halo cells of the box
+constexpr int ZS_NHQ = (ZS_NH + ZS_NT - 1) / ZS_NT; // halo cells per thread
+
+// block -> (tile i, tile j, z-chunk).  The tiles are cut into 8 contiguous
+// groups, one per XCD (block b runs on XCD b % 8); consecutive blocks of an
+// XCD take neighbouring tiles of the same z-chunk, whose halos then meet in
+// that XCD's L2.
+struct ZMap {
+    int ntx, nty;   // tiles along i and j
+    int kz, nkz;    // output planes per chunk, chunks along k
+    int per_xcd;    // tiles per XCD group
+};
+
+__device__ __forceinline__ bool zmap_block(const ZMap &m, int &tx, int &ty, int &kc)
+{
+    const int b = blockIdx.x;
+    const int xcd = b & 7, local = b >> 3;
+    kc = local / m.per_xcd;
+    const int t = xcd * m.per_xcd + (local - kc * m.per_xcd);
+    if (kc >= m.nkz || t >= m.ntx * m.nty) return false;
+    tx = t % m.ntx;
+    ty = t / m.ntx;
+    return true;
+}
+
+// box coordinates of halo cell h of the box: rows 0,1 and the last two in
+// full, then columns 0,1 and the last two of the rows between
+__device__ __forceinline__ void zs_halo(int h, int &bi, int &bj)
+{
+    if (h < 2 * ZS_BI) {
+        bj = h / ZS_BI;
+        bi = h - bj * ZS_BI;
+    } else if (h < 4 * ZS_BI) {
+        const int q = h - 2 * ZS_BI;
+        bj = ZS_BJ - 2 + q / ZS_BI;
+        bi = q % ZS_BI;
+    } else {
+        const int q = h - 4 * ZS_BI;
+        bj = 2 + (q >> 2);
+        const int c = q & 3;
+        bi = c < 2 ? c : ZS_BI - 4 + c;
+    }
+}
+
+// one cell's state in the form the flux needs (T arithmetic, as the
+// per-cell kernel uses it: background rounded to T, 1/rho and the pressure
+// perturbation formed in double and rounded)
+template <typename T>
+struct ZCell {
+    T r, e, eb, ir, pt, m[3], b[3], bb[3], v[3];
+};
+
+// a cell's raw state as loaded (the loads of the next plane are issued
+// into it while the current plane is being processed)
+template <typename T>
+__device__ __forceinline__ void zs_raw(St<3, T> &s, const T *__restrict__ u,
+                                       const double *__restrict__ bg, size_t nc, ci_t c)
+{
+    load_st32<3, T>(s, u, bg, nc, c);
+}
+
+template <typename T>
+__device__ __forceinline__ void zs_derive(ZCell<T> &z, const St<3, T> &s, double gam,
+                                          double *rb_out = nullptr, double *ptb_out = nullptr)
+{
+    const double rho = rho_full(s);
+    const double ird = 1.0 / rho;
+    z.r = s.r;
+    z.e = s.e;
+    z.eb = (T)s.eb;
+    z.ir = (T)ird;
+    z.pt = (T)ptot_pert<3, T>(s, rho, gam);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        z.m[d] = s.m[d];
+        z.b[d] = s.b[d];
+        z.bb[d] = (T)s.bb[d];
+        z.v[d] = (T)((double)s.m[d] * ird);
+    }
+    if (rb_out) *rb_out = s.rb;
+    if (ptb_out) {
+        double bbb = 0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) bbb += s.bb[d] * s.bb[d];
+        *ptb_out = (gam - 1) * s.eb - 0.5 * (gam - 2) * bbb;
+    }
+}
+
+// the seven non-trivial components of the flux along a: rho, m_0..2, e,
+// and b_i for the two i != a (increasing i)
+template <typename T>
+__device__ __forceinline__ void zs_flux(int a, const ZCell<T> &z, T F[7])
+{
+    T bbm = 0, bm = 0;
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        bbm += z.bb[t] * z.m[t];
+        bm += z.b[t] * z.m[t];
+    }
+    const T ma = z.m[a];
+    F[0] = ma;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const T mag = -(z.b[i] * z.bb[a] + z.bb[i] * z.b[a]) - z.b[i] * z.b[a];
+        F[1 + i] = ma * z.m[i] * z.ir + ((i == a) ? z.pt + mag : mag);
+    }
+    F[4] = (ma * (z.e + z.eb + z.pt) - z.b[a] * bbm - z.bb[a] * bm - z.b[a] * bm) * z.ir;
+    int q = 5;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (i == a) continue;
+        F[q++] = (ma * z.b[i] - (z.b[a] + z.bb[a]) * z.m[i] + z.bb[i] * ma) * z.ir;
+    }
+}
+
+// evolved-field index of flux component q along a
+__device__ __forceinline__ int zs_field(int a, int q)
+{
+    if (q < 5) return q; // rho, m_0..2, e
+    const int first = a == 0 ? 1 : 0;
+    const int second = a == 2 ? 1 : 2;
+    return 5 + (q == 5 ? first : second);
+}
+
+template <typename T>
+__device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const StageArgs &sa,
+                                             const ZMap &zm, const T *__restrict__ uin,
+                                             const T *__restrict__ un, T *__restrict__ uout,
+                                             T *__restrict__ acc, const T *__restrict__ dUh,
+                                             const double *__restrict__ bg,
+                                             T (*sF)[7][ZS_NB], T (*sV)[ZS_NB])
+{
+    typedef Lay<3> LY;
+
+    int tx, ty, kc;
+    if (!zmap_block(zm, tx, ty, kc)) return;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    const size_t nc = (size_t)g.nc;
+    const ci_t s1 = (ci_t)g.s1, s2 = (ci_t)g.s2;
+    const double gam = ph.gamma;
+    const int ti = threadIdx.x % ZS_TI, tj = threadIdx.x / ZS_TI;
+    const int i0 = 2 + tx * ZS_TI, j0 = 2 + ty * ZS_TJ;
+    const int i = i0 + ti, j = j0 + tj;
+    const bool act = i <= n0 - 3 && j <= n1 - 3;
+    const int k0 = 2 + kc * zm.kz;
+    const int k1 = min(k0 + zm.kz, n2 - 2);
+    const int own = (tj + 2) * ZS_BI + ti + 2;
+    // this thread's halo cells of the box (one or two)
+    int hb[ZS_NHQ], hc[ZS_NHQ];
+    bool hv[ZS_NHQ];
+#pragma unroll
+    for (int q = 0; q < ZS_NHQ; q++) {
+        const int h = threadIdx.x + q * ZS_NT;
+        hv[q] = h < ZS_NH;
+        int bi = 0, bj = 0;
+        if (hv[q]) zs_halo(h, bi, bj);
+        const int gi = i0 - 2 + bi, gj = j0 - 2 + bj;
+        hv[q] = hv[q] && gi <= n0 - 1 && gj <= n1 - 1;
+        hb[q] = bj * ZS_BI + bi;
+        hc[q] = gi + (int)s1 * gj;
+    }
+    const bool own_in = i <= n0 - 1 && j <= n1 - 1; // own cell lies in the grid
+    const ci_t colc = (ci_t)i + s1 * (ci_t)j;
+
+    // Ring of pending outputs, slots 0..3 = planes P-2..P+1 at the start of
+    // front P: rU holds dU; rG the k-differences of v still waiting for
+    // the background-work coefficients of planes P and P+1 (slots 2, 3);
+    // rC those coefficients (-p_Tb, b_b0..2) of planes P-2 and P-1.
+    T rU[4][8], rG[2][3], rC[2][4];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int f = 0; f < 8; f++) rU[q][f] = 0;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) rG[q][d] = 0;
+#pragma unroll
+        for (int f = 0; f < 4; f++) rC[q][f] = 0;
+    }
+
+    const bool zper = g.per[2];
+    const bool need_un = sa.mode == 1 || (sa.mode == 0 && sa.stage != 3);
+    const bool need_acc = sa.mode == 0 && sa.stage != 0;
+    // raw state of the own cell and the halo cells of the current plane
+    St<3, T> ro, rh[ZS_NHQ];
+    if (own_in) zs_raw<T>(ro, uin, bg, nc, colc + s2 * (ci_t)(k0 - 2));
+    for (int P = k0 - 2; P < k1 + 2; P++) {
+        const bool outp = P >= k0 && P < k1;
+        ZCell<T> z;
+        double rbo = 0, ptbo = 0;
+        if (own_in) zs_derive<T>(z, ro, gam, &rbo, &ptbo);
+        T Fz[7];
+        if (act) zs_flux<T>(2, z, Fz);
+        if (outp) {
+            __syncthreads(); // the previous plane's LDS reads are done
+            if (own_in) {
+                T F[7];
+#pragma unroll
+                for (int a = 0; a < 2; a++) {
+                    zs_flux<T>(a, z, F);
+#pragma unroll
+                    for (int q = 0; q < 7; q++) sF[a][q][own] = F[q];
+                }
+#pragma unroll
+                for (int d = 0; d < 3; d++) sV[d][own] = z.v[d];
+            }
+#pragma unroll
+            for (int q = 0; q < ZS_NHQ; q++)
+                if (hv[q]) {
+                    ZCell<T> h;
+                    zs_derive<T>(h, rh[q], gam);
+                    T F[7];
+#pragma unroll
+                    for (int a = 0; a < 2; a++) {
+                        zs_flux<T>(a, h, F);
+#pragma unroll
+                        for (int f = 0; f < 7; f++) sF[a][f][hb[q]] = F[f];
+                    }
+#pragma unroll
+                    for (int d = 0; d < 3; d++) sV[d][hb[q]] = h.v[d];
+                }
+        }
+        // Loads for later, issued before the in-plane work so they are in
+        // flight while it runs: first the inputs of the output completed in
+        // this iteration, then the raw state of plane P+1 (vector memory
+        // operations retire in order, so waiting for the former does not
+        // wait for the latter).
+        const int O = P - 2;
+        const bool fin = act && O >= k0 && O < k1;
+        const ci_t co = colc + s2 * (ci_t)O;
+        T fu[8], fa[8], fh[8], fr = 0;
+        double frb = 0;
+        if (fin) {
+#pragma unroll
+            for (int f = 0; f < 8; f++) {
+                fu[f] = need_un ? ld(un + f * nc, co) : (T)0;
+                fa[f] = need_acc ? ld(acc + f * nc, co) : (T)0;
+                fh[f] = ph.hyperdif ? ld(dUh + f * nc, co) : (T)0;
+            }
+            if (ph.driver == 1) {
+                fr = ld(uin, co);
+                frb = ld(bg + nc, co);
+            }
+        }
+        if (P + 1 < k1 + 2) {
+            const ci_t pn = s2 * (ci_t)(P + 1);
+            if (own_in) zs_raw<T>(ro, uin, bg, nc, colc + pn);
+            if (P + 1 >= k0 && P + 1 < k1)
+#pragma unroll
+                for (int q = 0; q < ZS_NHQ; q++)
+                    if (hv[q]) zs_raw<T>(rh[q], uin, bg, nc, (ci_t)hc[q] + pn);
+        }
+        if (outp) {
+            __syncthreads();
+            if (act) {
+                // 4th-order differences along i and j from LDS, background
+                // work terms, gravity
+                T bbc[3];
+#pragma unroll
+                for (int d = 0; d < 3; d++) bbc[d] = z.bb[d];
+                const T ptb = (T)ptbo;
+                const int xs[2] = {i, j};
+#pragma unroll
+                for (int a = 0; a < 2; a++) {
+                    if (zero4(g.per[a], xs[a], g.n[a])) continue;
+                    const int st_ = a == 0 ? 1 : ZS_BI;
+                    const T i12 = (T)(1.0 / (12.0 * g.dx[a][xs[a]]));
+                    T gv[3] = {0, 0, 0};
+#pragma unroll
+                    for (int p = 0; p < 4; p++) {
+                        const int off = (p < 2) ? p - 2 : p - 1;
+                        const T cw = (p == 0 ? (T)1 : (p == 1 ? (T)-8 : (p == 2 ? (T)8 : (T)-1))) * i12;
+                        const int nb = own + off * st_;
+#pragma unroll
+                        for (int q = 0; q < 7; q++) rU[2][zs_field(a, q)] -= cw * sF[a][q][nb];
+#pragma unroll
+                        for (int d = 0; d < 3; d++) gv[d] += cw * sV[d][nb];
+                    }
+                    T w = -ptb * gv[a];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) w += bbc[a] * bbc[d] * gv[d];
+                    rU[2][LY::EN] += w;
+                }
+                T ge = 0;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    rU[2][LY::M0 + d] += (T)ph.g[d] * z.r;
+                    ge += (T)ph.g[d] * z.m[d];
+                }
+                rU[2][LY::EN] += z.r * ge * z.ir;
+            }
+        }
+        if (act) {
+            // background-work coefficients of plane P; the k-differences of
+            // v gathered for output P so far become its energy term
+            T cP[4];
+            cP[0] = -(T)ptbo;
+#pragma unroll
+            for (int d = 0; d < 3; d++) cP[1 + d] = z.bb[d];
+            auto bgwork = [&](const T *c, const T *gz) -> T {
+                T w = c[0] * gz[2];
+#pragma unroll
+                for (int d = 0; d < 3; d++) w += c[3] * c[1 + d] * gz[d];
+                return w;
+            };
+            if (outp) rU[2][LY::EN] += bgwork(cP, rG[0]);
+            // scatter the k-flux of plane P into the outputs P-2..P+2
+            const T *F = Fz;
+            T nU[8], nG[3]; // the new slot of output P+2
+#pragma unroll
+            for (int f = 0; f < 8; f++) nU[f] = 0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) nG[d] = 0;
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                if (q == 2) continue;
+                const int O = P - 2 + q;
+                if (O < k0 || O >= k1 || zero4(zper, O, n2)) continue;
+                // neighbour offset P - O = 2 - q; weights 1, -8, 8, -1 for
+                // offsets -2, -1, +1, +2
+                const int off = 2 - q;
+                const T base = off == -2 ? (T)1 : (off == -1 ? (T)-8 : (off == 1 ? (T)8 : (T)-1));
+                const T cw = base * (T)(1.0 / (12.0 * g.dx[2][O]));
+                T *dU = q < 4 ? rU[q] : nU;
+#pragma unroll
+                for (int f = 0; f < 7; f++) dU[zs_field(2, f)] -= cw * F[f];
+                T gz[3];
+#pragma unroll
+                for (int d = 0; d < 3; d++) gz[d] = cw * z.v[d];
+                if (q < 2)
+                    dU[LY::EN] += bgwork(rC[q], gz);
+                else if (q == 3)
+#pragma unroll
+                    for (int d = 0; d < 3; d++) rG[1][d] += gz[d];
+                else
+#pragma unroll
+                    for (int d = 0; d < 3; d++) nG[d] = gz[d];
+            }
+            // output P-2 is complete
+            if (fin) {
+                T *dU = rU[0];
+#pragma unroll
+                for (int f = 0; f < 8; f++) dU[f] += fh[f];
+                if (ph.driver == 1) driver_cell<3, T>(g, ph, sa, fr, frb, i, j, O, dU);
+                rk_write_pre<3, T>(sa, uout, acc, nc, co, dU, fu, fa);
+            }
+            // advance the ring
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+#pragma unroll
+                for (int f = 0; f < 8; f++) rU[q][f] = rU[q + 1][f];
+#pragma unroll
+            for (int f = 0; f < 8; f++) rU[3][f] = nU[f];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                rG[0][d] = rG[1][d];
+                rG[1][d] = nG[d];
+            }
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                rC[0][f] = rC[1][f];
+                rC[1][f] = cP[f];
+            }
+        }
+    }
+}
+
+// FP32 state: up to 256 VGPRs (the prefetched plane and the output ring
+// stay in registers without spills), two 256-thread blocks of 37 KB of LDS
+// per CU
+__global__ __launch_bounds__(ZS_NT) __attribute__((amdgpu_waves_per_eu(2))) void
+k_zstream_f(Geo g, Phys ph, StageArgs sa, ZMap zm, const float *__restrict__ uin,
+            const float *__restrict__ un, float *__restrict__ uout, float *__restrict__ acc,
+            const float *__restrict__ dUh, const double *__restrict__ bg)
+{
+    __shared__ float sF[2][7][ZS_NB]; // fluxes along i (0) and j (1)
+    __shared__ float sV[3][ZS_NB];    // velocity
+    zstream_body<float>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV);
+}
+
+// FP64 state: one wave per SIMD, so that nothing spills; 74 KB of LDS
+__global__ __launch_bounds__(ZS_NT) __attribute__((amdgpu_waves_per_eu(1))) void
+k_zstream_d(Geo g, Phys ph, StageArgs sa, ZMap zm, const double *__restrict__ uin,
+            const double *__restrict__ un, double *__restrict__ uout, double *__restrict__ acc,
+            const double *__restrict__ dUh, const double *__restrict__ bg)
+{
+    __shared__ double sF[2][7][ZS_NB];
+    __shared__ double sV[3][ZS_NB];
+    zstream_body<double>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV);
+}
+
+// ---------------------------------------------------------------------------
+// Hyperdiffusive sources with the plane staged in LDS.  A block takes a
+// 64 x 4 tile of one plane; the state, aux quantities, rho_b and 18 of the
+// 21 ratio pairs of the tile and a 1-cell halo are loaded once, coalesced,
+// into LDS, and every in-plane neighbour access of hyper_terms reads LDS.
+// Accesses to the planes above and below (and the three ratio pairs that
+// are only read at the cell itself or below it) go to global memory.
+// ---------------------------------------------------------------------------
+constexpr int HB_TI = 64, HB_TJ = 4; // the 64 x 4 tiles of make_map / map_block
+constexpr int HB_BI = HB_TI + 2, HB_BJ = HB_TJ + 2, HB_NB = HB_BI * HB_BJ;
+constexpr int HB_NH = HB_NB - HB_TI * HB_TJ;
+constexpr int HB_NR = 18; // ratio pairs staged: all but prho(2), pe(2), pv(2,2)
+
+__host__ __device__ constexpr int hb_rslot(int p)
+{
+    return (p == 2 || p == 5 || p == 14) ? -1 : p - (p > 2) - (p > 5) - (p > 14);
+}
+// state field -> LDS slot (rho, m_0..2, b_0..2; e is not read)
+__host__ __device__ constexpr int hb_uslot(int f) { return f < 4 ? f : f - 1; }
+
+template <typename T>
+struct HyLds {
+    HyGlobal<3, T> gm;
+    const T (*sU)[HB_NB];
+    const T (*sA)[HB_NB];
+    const double *sRb;
+    const T (*sR)[HB_NB];
+    int own;
+    __device__ __forceinline__ int bx(int di, int dj) const { return own + di + HB_BI * dj; }
+    __device__ __forceinline__ T U(int f, int di, int dj, int dk) const
+    {
+        return dk == 0 ? sU[hb_uslot(f)][bx(di, dj)] : gm.U(f, di, dj, dk);
+    }
+    __device__ __forceinline__ T A(int a, int di, int dj, int dk) const
+    {
+        return dk == 0 ? sA[a - Aux<3>::EINT][bx(di, dj)] : gm.A(a, di, dj, dk);
+    }
+    __device__ __forceinline__ double rhob(int di, int dj, int dk) const
+    {
+        return dk == 0 ? sRb[bx(di, dj)] : gm.rhob(di, dj, dk);
+    }
+    __device__ __forceinline__ T Rp(int p, int di, int dj, int dk) const
+    {
+        return (dk == 0 && hb_rslot(p) >= 0) ? sR[hb_rslot(p)][bx(di, dj)] : gm.Rp(p, di, dj, dk);
+    }
+};
+
+// box coordinates of halo cell h of the 66 x 6 box: rows 0 and 5 in full,
+// then columns 0 and 65 of rows 1..4
+__device__ __forceinline__ void hb_halo(int h, int &bi, int &bj)
+{
+    if (h < 2 * HB_BI) {
+        bj = h < HB_BI ? 0 : HB_BJ - 1;
+        bi = h < HB_BI ? h : h - HB_BI;
+    } else {
+        const int q = h - 2 * HB_BI;
+        bj = 1 + (q >> 1);
+        bi = (q & 1) ? HB_BI - 1 : 0;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void hyper_lds_body(const Geo &g, const TileMap &tm, const Phys &ph,
+                                               const T *__restrict__ uin,
+                                               const T *__restrict__ R,
+                                               const T *__restrict__ aux,
+                                               const double *__restrict__ bg,
+                                               const unsigned long long *__restrict__ cmb,
+                                               T *__restrict__ dUh, T (*sU)[HB_NB],
+                                               T (*sA)[HB_NB], double *sRb, T (*sR)[HB_NB])
+{
+    typedef Lay<3> LY;
+    int tx, ty, tk;
+    if (!map_block(tm, tx, ty, tk)) return;
+    const int n0 = g.n[0], n1 = g.n[1];
+    const size_t nc = (size_t)g.nc;
+    const ci_t s1 = (ci_t)g.s1, s2 = (ci_t)g.s2;
+    const int ti = threadIdx.x & 63, tj = threadIdx.x >> 6;
+    const int i0 = 2 + tx * HB_TI, j0 = 2 + ty * HB_TJ, k = 2 + tk;
+    const ci_t pc = s2 * (ci_t)k;
+    // stage the tile and its 1-cell halo
+    auto stage = [&](int bi, int bj) {
+        const int gi = i0 - 1 + bi, gj = j0 - 1 + bj;
+        if (gi > n0 - 1 || gj > n1 - 1) return;
+        const ci_t c = (ci_t)gi + s1 * (ci_t)gj + pc;
+        const int b = bj * HB_BI + bi;
+        sU[0][b] = ld(uin, c);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            sU[1 + d][b] = ld(uin + (LY::M0 + d) * nc, c);
+            sU[4 + d][b] = ld(uin + (LY::B0 + d) * nc, c);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; a++) sA[a][b] = ld(aux + (Aux<3>::EINT + a) * nc, c);
+        sRb[b] = ld(bg + nc, c);
+#pragma unroll
+        for (int p = 0; p < LY::NPAIR; p++)
+            if (hb_rslot(p) >= 0) sR[hb_rslot(p)][b] = ld(R + p * nc, c);
+    };
+    stage(ti + 1, tj + 1);
+    if (threadIdx.x < HB_NH) {
+        int bi, bj;
+        hb_halo(threadIdx.x, bi, bj);
+        stage(bi, bj);
+    }
+    __syncthreads();
+    const int i = i0 + ti, j = j0 + tj;
+    if (i > n0 - 3 || j > n1 - 3) return;
+    HyLds<T> vw;
+    vw.gm.uin = uin;
+    vw.gm.aux = aux;
+    vw.gm.bg = bg;
+    vw.gm.R = R;
+    vw.gm.nc = nc;
+    vw.gm.s1 = s1;
+    vw.gm.s2 = s2;
+    vw.gm.c = (ci_t)i + s1 * (ci_t)j + pc;
+    vw.sU = sU;
+    vw.sA = sA;
+    vw.sRb = sRb;
+    vw.sR = sR;
+    vw.own = (tj + 1) * HB_BI + ti + 1;
+    T dU[8];
+#pragma unroll
+    for (int f = 0; f < 8; f++) dU[f] = 0;
+    hyper_terms<3, T>(g, ph, vw, cmb, i, j, k, dU);
+#pragma unroll
+    for (int f = 0; f < 8; f++) st(dUh + f * nc, vw.gm.c, dU[f]);
+}
+
+__global__ __launch_bounds__(256) void k_hyper_lds_f(Geo g, TileMap tm, Phys ph,
+                                                      const float *__restrict__ uin,
+                                                      const float *__restrict__ R,
+                                                      const float *__restrict__ aux,
+                                                      const double *__restrict__ bg,
+                                                      const unsigned long long *__restrict__ cmb,
+                                                      float *__restrict__ dUh)
+{
+    __shared__ float sU[7][HB_NB], sA[4][HB_NB], sR[HB_NR][HB_NB];
+    __shared__ double sRb[HB_NB];
+    hyper_lds_body<float>(g, tm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRb, sR);
+}
+
+__global__ __launch_bounds__(256) void k_hyper_lds_d(Geo g, TileMap tm, Phys ph,
+                                                      const double *__restrict__ uin,
+                                                      const double *__restrict__ R,
+                                                      const double *__restrict__ aux,
+                                                      const double *__restrict__ bg,
+                                                      const unsigned long long *__restrict__ cmb,
+                                                      double *__restrict__ dUh)
+{
+    __shared__ double sU[7][HB_NB], sA[4][HB_NB], sR[HB_NR][HB_NB];
+    __shared__ double sRb[HB_NB];
+    hyper_lds_body<double>(g, tm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRb, sR);
+}
+
+// the hyperdiffusive sources of every interior cell into dUh (the stage
+// update adds them); same tiles and XCD panels as k_update
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_hyper(Geo g, TileMap tm, Phys ph,
+                                                const T *__restrict__ uin,
+                                                const T *__restrict__ R,
+                                                const T *__restrict__ aux,
+                                                const double *__restrict__ bg,
+                                                const unsigned long long *__restrict__ cmb,
+                                                T *__restrict__ dUh)
+{
+    typedef Lay<D> LY;
+    int tx, ty, tk;
+    if (!map_block(tm, tx, ty, tk)) return;
+    const int x0 = 2 + tx * 64 + (threadIdx.x & 63);
+    const int x1 = 2 + ty * 4 + (threadIdx.x >> 6);
+    const int x2 = (D == 3) ? 2 + tk : 0;
+    if (x0 > g.n[0] - 3 || x1 > g.n[1] - 3) return;
+    T dU[LY::NVE];
+#pragma unroll
+    for (int f = 0; f < LY::NVE; f++) dU[f] = 0;
+    hyper_cell<D, T>(g, ph, uin, R, aux, bg, cmb, x0, x1, x2, dU);
+    const size_t nc = (size_t)g.nc;
+    const ci_t c = (ci_t)x0 + (ci_t)g.s1 * (ci_t)x1 + (ci_t)g.s2 * (ci_t)x2;
+#pragma unroll
+    for (int f = 0; f < LY::NVE; f++) st(dUh + f * nc, c, dU[f]);
+}
+
+} // namespace smaug

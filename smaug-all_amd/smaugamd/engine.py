@@ -25,6 +25,7 @@ EXPORTS = [
     "smaug_rhs", "smaug_cmax", "smaug_maxcourant", "smaug_rk4", "smaug_euler",
     "smaug_time_rk4", "smaug_ncell", "smaug_ninterior", "smaug_copy_probe",
     "smaug_set_panel", "smaug_get_panel", "smaug_time_pass", "smaug_autotune",
+    "smaug_set_zstream", "smaug_get_zstream",
     "smaug_ens_create", "smaug_ens_destroy", "smaug_ens_nrun", "smaug_ens_ninterior",
     "smaug_ens_set_state", "smaug_ens_get_state", "smaug_ens_rk4", "smaug_ens_time_rk4",
 ]
@@ -84,6 +85,8 @@ def load_library(path: str = LIB_PATH):
         lib.smaug_get_panel.argtypes = [vp, ctypes.c_int, ip, ip]
         lib.smaug_time_pass.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
         lib.smaug_autotune.argtypes = [vp, ctypes.c_int, ip, dp]
+        lib.smaug_set_zstream.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        lib.smaug_get_zstream.argtypes = [vp, ip, ip]
         lib.smaug_ens_create.argtypes = [ctypes.POINTER(SmaugCfg), ctypes.c_int,
                                          ctypes.POINTER(vp)]
         lib.smaug_ens_destroy.argtypes = [vp]
@@ -201,6 +204,17 @@ class Engine:
         x, y = ctypes.c_int(), ctypes.c_int()
         self._check(self.lib.smaug_get_panel(self.h, pass_, ctypes.byref(x), ctypes.byref(y)))
         return x.value, y.value
+
+    def set_zstream(self, enable, kz=0):
+        """3-D: z-streamed stage update (default) or the per-cell one"""
+        self._check(self.lib.smaug_set_zstream(self.h, int(enable), kz))
+
+    def get_zstream(self):
+        """(form of the 3-D update pass: 0 per cell, 1 z-streamed, 2 z-streamed
+        with per-cell hyperdiffusion; z-stream depth)"""
+        en, kz = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.smaug_get_zstream(self.h, ctypes.byref(en), ctypes.byref(kz)))
+        return en.value, kz.value
 
     def time_pass(self, pass_, reps=5):
         ms = ctypes.c_double()

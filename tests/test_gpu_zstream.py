@@ -1,0 +1,73 @@
+"""The z-streamed 3-D stage update (k_zstream) against the per-cell stage
+update and against the FP64 oracle, on tiles and z-chunks that do not divide
+the grid."""
+import numpy as np
+import pytest
+
+from helpers import interior_mask, perturbed, rel_err
+
+pytestmark = pytest.mark.gpu
+
+from smaugamd import configs  # noqa: E402
+from smaugamd.engine import FP64, MIXED, Engine  # noqa: E402
+from oracle.oracle import Oracle  # noqa: E402
+
+# the two update kernels differ only in the summation order of the twelve
+# neighbour contributions
+TOL_RHS = {FP64: 1e-12, MIXED: 2e-5}
+TOL_ORACLE_FP64 = 1e-10
+
+
+def strat(nz, nx, ny):
+    return configs.stratified(nz=nz, nx=nx, ny=ny, lz=6e6 * nz / 256 * 4,
+                              lx=4e6 * nx / 256 * 4, ly=4e6 * ny / 256 * 4)
+
+
+CASES = [
+    ("70x9x13", strat(70, 9, 13), 0),      # two i-tiles, ragged j-tile
+    ("20x12x16_kz5", strat(20, 12, 16), 5),  # z-chunks of 5 over 16 planes
+    ("64x8x8_kz3", strat(64, 8, 8), 3),
+]
+
+
+@pytest.mark.parametrize("prec", [FP64, MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("name,setup,kz", CASES, ids=[c[0] for c in CASES])
+def test_rhs_matches_per_cell_kernel(name, setup, kz, prec):
+    print(f"[case] zstream rhs {name} kz={kz} prec={prec} n={setup.n}", flush=True)
+    o = Oracle(setup)
+    w = o.boundary(perturbed(setup, amp=1e-2, seed=5))
+    if prec == MIXED:
+        w = o.boundary(w.astype(np.float32).astype(np.float64))
+    e = Engine(setup, prec)
+    e.set_state(w)
+    e.set_zstream(False)
+    ref = e.rhs(t=2.0)
+    # 1: hyperdiffusive sources from the LDS-staged kernel, 2: per-cell
+    for mode in (1, 2):
+        e.set_zstream(mode, kz)
+        got = e.rhs(t=2.0)
+        err = rel_err(got, ref, interior_mask(setup))
+        print(name, f"zstream mode {mode} vs per-cell rel err", err)
+        assert np.all(np.isfinite(got))
+        assert np.all(err < TOL_RHS[prec]), err
+        # ghosts stay untouched in the raw-L(w) mode
+        assert np.all(got[:, ~interior_mask(setup)] == 0)
+    e.close()
+
+
+@pytest.mark.parametrize("name,setup,kz", CASES, ids=[c[0] for c in CASES])
+def test_rk4_fp64_vs_oracle(name, setup, kz):
+    print(f"[case] zstream rk4 fp64 vs oracle {name} kz={kz}", flush=True)
+    o = Oracle(setup)
+    w0 = o.boundary(perturbed(setup, amp=1e-3, seed=6))
+    dt = 0.5 * configs.hyperviscous_dt(setup)
+    ref, _ = o.run(w0, 0.25, dt, 4, rk4=True)
+    e = Engine(setup, FP64)
+    e.set_zstream(True, kz)
+    e.set_state(w0)
+    e.rk4(0.25, dt, 4)
+    got = e.get_state()
+    e.close()
+    err = rel_err(got, ref, interior_mask(setup))
+    print(name, "rk4 rel err vs oracle", err)
+    assert np.all(err < TOL_ORACLE_FP64), err
