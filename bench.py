@@ -50,22 +50,57 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--shape", default=None,
+                    help="fixed 3-D update form and z-stream depth 'FORM,KZ' instead of the "
+                         "autotuner (FORM 0 per-cell, 1 z-streamed), e.g. for profiling runs")
     ap.add_argument("--ens-steps", type=int, default=200)
     return ap.parse_args()
 
 
-def run_path(setup, precision, warmup, steps, dt_scale=1.0):
+def run_path(setup, precision, warmup, steps, dt_scale=1.0, tune=True, shape=None):
+    """time `steps` RK4 steps after `warmup`; the launch shapes (XCD panels,
+    z-streamed or per-cell update, z-stream depth) are picked by the
+    engine's autotuner first"""
     from smaugamd.engine import Engine
 
     dt = setup.dt * dt_scale
     e = Engine(setup, precision)
     e.set_state()
+    info = {}
+    if shape is not None:
+        form, kz = (int(x) for x in shape.split(","))
+        e.set_zstream(form, kz)
+        info["tuned"] = {"update_form": ["per-cell", "z-streamed"][form], "kz": kz, "fixed": True}
+    elif tune:
+        best, ms = e.autotune(3)
+        form, kz = e.get_zstream()
+        info["tuned"] = {"update_form": ["per-cell", "z-streamed", "z-streamed/per-cell-hyper"][form],
+                         "kz": kz, "panel_ratio": list(best[0]), "panel_update": list(best[1])}
+        e.set_state()  # the tuning launches leave the state untouched; re-fill anyway
     e.rk4(0.0, dt, warmup)
     t0 = warmup * dt
     tot_ms, stage_ms = e.time_rk4(t0, dt, steps)
+    info["passes_ms"] = e.time_passes(t0 + (steps + 1) * dt, dt)
+    info["form"] = e.get_zstream()[0]
     w = e.get_state()
     e.close()
-    return tot_ms, stage_ms, w
+    return tot_ms, stage_ms, w, info
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE passes scaled by the copy-kernel
+    calibration), or None"""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return k["fetch_bytes"] * t["fetch_scale"] + k["write_bytes"] * t["write_scale"]
 
 
 def cpu_baseline(n):
@@ -155,7 +190,7 @@ def main():
     cells = setup.ninterior
 
     print(f"[bench] set-up done: {setup.n} dt={setup.dt:.4e}", file=sys.stderr, flush=True)
-    tot, stage_ms, wm = run_path(setup, MIXED, a.warmup, a.steps)
+    tot, stage_ms, wm, info = run_path(setup, MIXED, a.warmup, a.steps, shape=a.shape)
     print(f"[bench] mixed path: {tot / a.steps:.3f} ms/step", file=sys.stderr, flush=True)
     ms_step = tot / a.steps
     value = cells * a.steps / (tot / 1e3) / 1e6
@@ -163,12 +198,14 @@ def main():
     out_fp64 = None
     acc = None
     if not a.no_fp64:
-        tot64, stage64, wd = run_path(setup, FP64, a.warmup, a.steps)
+        tot64, stage64, wd, info64 = run_path(setup, FP64, a.warmup, a.steps)
         print(f"[bench] fp64 path: {tot64 / a.steps:.3f} ms/step", file=sys.stderr, flush=True)
         out_fp64 = {
             "value": cells * a.steps / (tot64 / 1e3) / 1e6,
             "ms_per_step": tot64 / a.steps,
             "stage_kernel_ms": stage64,
+            "tuned": info64.get("tuned"),
+            "passes_ms": info64["passes_ms"],
             "speedup_mixed_over_fp64": (tot64 / tot),
         }
         n0, n1, n2 = setup.n
@@ -186,26 +223,31 @@ def main():
             linf[names[f]] = float(np.abs(d).max() / amp) if amp > 0 else 0.0
         # conditioning of the workload itself: the FP64 path against the FP64
         # path run with dt*(1+1e-9)
-        _, _, wd2 = run_path(setup, FP64, a.warmup, a.steps, dt_scale=1 + 1e-9)
+        _, _, wd2, _ = run_path(setup, FP64, a.warmup, a.steps, dt_scale=1 + 1e-9, tune=False)
         sens = {}
         for f in range(setup.nve):
             ref = wd[f][m]
             rms = np.sqrt(np.mean(ref ** 2))
             sens[names[f]] = float(np.sqrt(np.mean((wd2[f][m] - ref) ** 2)) / rms) if rms > 0 else 0.0
         acc = {"reference": "FP64 GPU path (parity-tested against the FP64 oracle)",
-               "steps": a.warmup + a.steps + 1, "rel_l2": l2, "rel_linf": linf,
+               "steps": a.warmup + a.steps + 2, "rel_l2": l2, "rel_linf": linf,
                "fp64_rel_l2_under_1e-9_dt_perturbation": sens}
 
     bytes_launch = BYTES_PER_CELL_STEP["mixed"] / STAGES_PER_STEP * cells
     achieved = bytes_launch / (stage_ms / 1e3) / 1e9
+    kname = "k_zstream_f" if info["form"] else "k_update<3, float>"
     roof = {
         "bound": "hbm",
-        "kernel": "k_update<3,float> (fused RK4 stage update)",
+        "kernel": kname + (" (z-streamed RK4 stage update)" if info["form"]
+                           else " (per-cell RK4 stage update)"),
+        "algorithmic_bytes_per_launch": bytes_launch,
+        "launch_ms": stage_ms,
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": profiled_traffic(kname),
+        "passes_ms": info["passes_ms"],
         "step_level_frac": (BYTES_PER_CELL_STEP["mixed"] * cells / (ms_step / 1e3) / 1e9)
         / HBM_PEAK_GBS,
     }
@@ -234,7 +276,8 @@ def main():
         "config": {"workload": f"3-D stratified solar atmosphere {n}x{n}x{n} cells over "
                                f"6x4x4 Mm, RK4, dt={setup.dt:.4e} s (min of CFL 0.3 and "
                                f"the hyperviscous limit)",
-                   "cells": cells, "precision": "mixed", "parallelism": "single GPU"},
+                   "cells": cells, "precision": "mixed", "parallelism": "single GPU",
+                   "launch_shapes": info.get("tuned")},
         "fp64": out_fp64,
         "accuracy": acc,
         "roofline": roof,

@@ -100,6 +100,12 @@ int smaug_set_zstream(smaug_engine *e, int enable, int kz);
 /* the form of the 3-D update pass in use (enable as above) and its depth */
 int smaug_get_zstream(smaug_engine *e, int *enable, int *kz);
 
+/* one more RK4 step, with HIP events between its passes: ms[p] is the mean
+ * duration over the four stages of pass p = 0 aux quantities and cmax,
+ * 1 hyperviscosity ratios, 2 hyperdiffusive sources (3-D z-streamed form
+ * only; 0 otherwise), 3 stage update, 4 ghost fill */
+int smaug_time_passes(smaug_engine *e, double t0, double dt, double ms[5]);
+
 /* measured copy-bandwidth ceiling: GB/s (read + write bytes) of a plain
  * streaming copy of nbytes, mean of reps launches */
 int smaug_copy_probe(long long nbytes, int reps, double *gbs);

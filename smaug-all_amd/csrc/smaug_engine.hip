@@ -171,12 +171,11 @@ static ZMap make_zmap(const smaug_engine *e, unsigned *nblocks)
     return m;
 }
 
+// 3-D z-streamed form: the hyperdiffusive sources of uin into HYP (the
+// stage update adds them)
 template <typename T>
-static int launch_zstream(smaug_engine *e, StageArgs sa, const void *uin, const void *un,
-                          void *uout)
+static int launch_hyper_sources(smaug_engine *e, const void *uin)
 {
-    unsigned nb;
-    ZMap m = make_zmap(e, &nb);
     if (e->phys.hyperdif) {
         unsigned nh;
         TileMap tm = make_map(e->n[0] - 4, e->n[1] - 4, e->n[2] - 4, e->panel[1][0],
@@ -200,6 +199,15 @@ static int launch_zstream(smaug_engine *e, StageArgs sa, const void *uin, const 
         }
         HIPCHK(hipGetLastError());
     }
+    return 0;
+}
+
+template <typename T>
+static int launch_zstream(smaug_engine *e, StageArgs sa, const void *uin, const void *un,
+                          void *uout)
+{
+    unsigned nb;
+    ZMap m = make_zmap(e, &nb);
     if constexpr (sizeof(T) == 4)
         hipLaunchKernelGGL(k_zstream_f, dim3(nb), dim3(ZS_NT), 0, e->stream, e->geo, e->phys, sa, m,
                            (const T *)uin, (const T *)un, (T *)uout, (T *)e->ACC,
@@ -209,6 +217,21 @@ static int launch_zstream(smaug_engine *e, StageArgs sa, const void *uin, const 
                            (const T *)uin, (const T *)un, (T *)uout, (T *)e->ACC,
                            (const T *)e->HYP, (const double *)e->bg);
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+template <int D, typename T>
+static bool zstreamed(const smaug_engine *e)
+{
+    return D == 3 && e->zstream;
+}
+
+// the hyperdiffusive-source pass that precedes the z-streamed update
+template <int D, typename T>
+static int launch_pre_update(smaug_engine *e, const void *uin)
+{
+    if constexpr (D == 3)
+        if (e->zstream && e->phys.hyperdif) return launch_hyper_sources<T>(e, uin);
     return 0;
 }
 
@@ -240,22 +263,30 @@ __global__ void k_advance_time(double *tm)
 // one stage: cmax(uin), ratios(uin), update -> uout, ghost fill of uout
 template <int D, typename T>
 static int stage(smaug_engine *e, StageArgs sa, const void *uin, const void *un, void *uout,
-                 hipEvent_t ev0, hipEvent_t ev1)
+                 hipEvent_t ev0, hipEvent_t ev1, const hipEvent_t *evk = nullptr)
 {
     int rc;
+    if (evk) HIPCHK(hipEventRecord(evk[0], e->stream));
     if ((rc = launch_cmax<D, T>(e, uin))) return rc;
+    if (evk) HIPCHK(hipEventRecord(evk[1], e->stream));
     if (e->phys.hyperdif)
         if ((rc = launch_ratio<D, T>(e, uin))) return rc;
+    if (evk) HIPCHK(hipEventRecord(evk[2], e->stream));
+    if ((rc = launch_pre_update<D, T>(e, uin))) return rc;
     if (ev0) HIPCHK(hipEventRecord(ev0, e->stream));
+    if (evk) HIPCHK(hipEventRecord(evk[3], e->stream));
     if ((rc = launch_stage_update<D, T>(e, sa, uin, un, uout))) return rc;
     if (ev1) HIPCHK(hipEventRecord(ev1, e->stream));
+    if (evk) HIPCHK(hipEventRecord(evk[4], e->stream));
     if (sa.mode != 2)
         if ((rc = launch_bc<D, T>(e, uout))) return rc;
+    if (evk) HIPCHK(hipEventRecord(evk[5], e->stream));
     return 0;
 }
 
 template <int D, typename T>
-static int rk4_step(smaug_engine *e, std::vector<hipEvent_t> *evs)
+static int rk4_step(smaug_engine *e, std::vector<hipEvent_t> *evs,
+                    std::vector<hipEvent_t> *evk = nullptr)
 {
     // buffers: U0 = u_n (and u_{n+1}), U1/U2 stage outputs, ACC accumulator
     const double ct[4] = {0.0, 0.5, 0.5, 1.0};
@@ -272,7 +303,8 @@ static int rk4_step(smaug_engine *e, std::vector<hipEvent_t> *evs)
             a = (*evs)[2 * s];
             b = (*evs)[2 * s + 1];
         }
-        int rc = stage<D, T>(e, sa, in[s], e->U[0], out[s], a, b);
+        int rc = stage<D, T>(e, sa, in[s], e->U[0], out[s], a, b,
+                             evk ? evk->data() + 6 * s : nullptr);
         if (rc) return rc;
     }
     hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(64), 0, e->stream, e->d_tm);
@@ -636,6 +668,36 @@ static int time_rk4(smaug_engine *e, double t0, double dt, int nsteps, double *t
     return 0;
 }
 
+// one eager RK4 step with events between the passes of every stage: the
+// mean duration of each pass over the four stages
+template <int D, typename T>
+static int time_passes(smaug_engine *e, double t0, double dt, double *ms)
+{
+    int rc = set_time(e, t0, dt);
+    if (rc) return rc;
+    std::vector<hipEvent_t> evk(24);
+    for (auto &v : evk) HIPCHK(hipEventCreate(&v));
+    if ((rc = rk4_step<D, T>(e, nullptr, &evk))) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int p = 0; p < 5; p++) {
+        double sum = 0;
+        for (int s = 0; s < 4; s++) {
+            float m = 0;
+            HIPCHK(hipEventElapsedTime(&m, evk[6 * s + p], evk[6 * s + p + 1]));
+            sum += m;
+        }
+        ms[p] = sum / 4;
+    }
+    for (auto &v : evk) hipEventDestroy(v);
+    return 0;
+}
+
+extern "C" int smaug_time_passes(smaug_engine *e, double t0, double dt, double ms[5])
+{
+    if (!e || !ms) return fail(SMAUG_EINVAL, "bad argument");
+    return DISPATCH(e, time_passes, e, t0, dt, ms);
+}
+
 extern "C" int smaug_time_rk4(smaug_engine *e, double t0, double dt, int nsteps,
                               double *total_ms, double *stage_kernel_ms)
 {
@@ -696,8 +758,12 @@ static int time_pass(smaug_engine *e, int pass, int reps, double *ms)
     HIPCHK(hipEventCreate(&b));
     for (int r = -1; r < reps; r++) {
         if (r == 0) HIPCHK(hipEventRecord(a, e->stream));
-        rc = pass == 0 ? launch_ratio<D, T>(e, e->U[0])
-                       : launch_stage_update<D, T>(e, sa, e->U[0], e->U[0], e->U[1]);
+        if (pass == 0) {
+            rc = launch_ratio<D, T>(e, e->U[0]);
+        } else {
+            rc = launch_pre_update<D, T>(e, e->U[0]);
+            if (!rc) rc = launch_stage_update<D, T>(e, sa, e->U[0], e->U[0], e->U[1]);
+        }
         if (rc) return rc;
     }
     HIPCHK(hipEventRecord(b, e->stream));
