@@ -91,6 +91,8 @@ struct smaug_engine {
     int zstream;             // 3-D: 1 = z-streamed stage update, 0 = per-cell stage update
     int kz;                  // z-stream depth (output planes per block)
     int hyper_lds;           // 3-D: hyperdiffusive sources with the plane staged in LDS
+    int ratio_zs;            // 3-D: z-streamed hyperviscosity ratios
+    int kzr;                 // their z-stream depth (0: 32)
 };
 
 // ---------------------------------------------------------------------------
@@ -131,9 +133,37 @@ static TileMap make_map(int e0, int e1, int e2, int tpx, int tpy, unsigned *nblo
     return m;
 }
 
+// tiles and z-chunks of the ratio box [1, n-3] for the z-streamed ratio kernel
+static ZMap make_rmap(const smaug_engine *e, unsigned *nblocks)
+{
+    ZMap m;
+    m.ntx = (e->n[0] - 3 + ZS_TI - 1) / ZS_TI;
+    m.nty = (e->n[1] - 3 + ZS_TJ - 1) / ZS_TJ;
+    const int nk = e->n[2] - 3;
+    m.kz = e->kzr > 0 ? (e->kzr < nk ? e->kzr : nk) : (nk < 32 ? nk : 32);
+    m.nkz = (nk + m.kz - 1) / m.kz;
+    m.per_xcd = (m.ntx * m.nty + 7) / 8;
+    *nblocks = (unsigned)(8 * m.per_xcd * m.nkz);
+    return m;
+}
+
 template <int D, typename T>
 static int launch_ratio(smaug_engine *e, const void *u)
 {
+    if constexpr (D == 3) {
+        if (e->ratio_zs) {
+            unsigned nb;
+            ZMap m = make_rmap(e, &nb);
+            if constexpr (sizeof(T) == 4)
+                hipLaunchKernelGGL(k_ratio_zs_f, dim3(nb), dim3(ZS_NT), 0, e->stream, e->geo, m,
+                                   (const T *)u, (const T *)e->AUX, (T *)e->R);
+            else
+                hipLaunchKernelGGL(k_ratio_zs_d, dim3(nb), dim3(ZS_NT), 0, e->stream, e->geo, m,
+                                   (const T *)u, (const T *)e->AUX, (T *)e->R);
+            HIPCHK(hipGetLastError());
+            return 0;
+        }
+    }
     unsigned nb;
     TileMap m = make_map(e->n[0] - 3, e->n[1] - 3, D == 3 ? e->n[2] - 3 : 1, e->panel[0][0],
                          e->panel[0][1], &nb); // box [1, n-3]
@@ -442,6 +472,8 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
     e->zstream = D == 3;
     e->kz = 0;
     e->hyper_lds = 1;
+    e->ratio_zs = D == 3;
+    e->kzr = 0;
     *out = e;
     return 0;
 }
@@ -796,14 +828,16 @@ extern "C" int smaug_time_pass(smaug_engine *e, int pass, int reps, double *ms)
 
 // picks, for the ratio and update passes, the fastest XCD panel among
 // tpx in {1, 2, 4, 8} and tpy in {1, 2, 4, 8, 16, 32}; for 3-D grids the
-// update pass also tries the z-streamed kernel at z-stream depths
-// 4, 8, 16, 32 and 64 and keeps whichever form is fastest
+// ratio pass also tries the z-streamed ratio kernel at depths 8..64 and
+// the update pass the z-streamed update at depths 4..64, and each keeps
+// whichever form is fastest
 extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double best_ms[2])
 {
     if (!e || reps < 1) return fail(SMAUG_EINVAL, "bad argument");
     static const int TX[] = {1, 2, 4, 8}, TY[] = {1, 2, 4, 8, 16, 32};
-    const int zs0 = e->zstream;
+    const int zs0 = e->zstream, rz0 = e->ratio_zs;
     e->zstream = 0;
+    e->ratio_zs = 0;
     for (int pass = 0; pass < 2; pass++) {
         if (pass == 0 && !e->phys.hyperdif) {
             if (best) best[0][0] = best[0][1] = 0;
@@ -830,6 +864,23 @@ extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double 
             }
         e->panel[pass][0] = bx;
         e->panel[pass][1] = by;
+        if (pass == 0 && e->D == 3) {
+            static const int KZ[] = {8, 16, 32, 64};
+            int bkz = -1;
+            e->ratio_zs = 1;
+            for (int kz : KZ) {
+                e->kzr = kz;
+                double ms;
+                int rc = DISPATCH(e, time_pass, e, pass, reps, &ms);
+                if (rc) return rc;
+                if (ms < bm) {
+                    bm = ms;
+                    bkz = kz;
+                }
+            }
+            e->ratio_zs = bkz > 0;
+            e->kzr = bkz > 0 ? bkz : 0;
+        }
         if (pass == 1 && e->D == 3) {
             static const int KZ[] = {4, 8, 16, 32, 64};
             int bkz = -1;
@@ -853,7 +904,20 @@ extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double 
         }
         if (best_ms) best_ms[pass] = bm;
     }
-    if (e->D != 3) e->zstream = zs0;
+    if (e->D != 3) {
+        e->zstream = zs0;
+        e->ratio_zs = rz0;
+    }
+    drop_graphs(e);
+    return 0;
+}
+
+extern "C" int smaug_set_ratio_form(smaug_engine *e, int zstreamed, int kz)
+{
+    if (!e || kz < 0) return fail(SMAUG_EINVAL, "bad argument");
+    if (zstreamed && e->D != 3) return fail(SMAUG_EINVAL, "the z-streamed ratios are for 3-D grids");
+    e->ratio_zs = zstreamed ? 1 : 0;
+    e->kzr = kz;
     drop_graphs(e);
     return 0;
 }

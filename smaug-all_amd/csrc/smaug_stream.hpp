@@ -419,6 +419,165 @@ k_zstream_d(Geo g, Phys ph, StageArgs sa, ZMap zm, const double *__restrict__ ui
 }
 
 // ---------------------------------------------------------------------------
+// Hyperviscosity ratios, z-streamed.  A block owns 64 x 4 columns of the box
+// [1, n-3] and walks a chunk of its planes.  Each plane's window quantities
+// (rho', e_int, v, b') over the tile plus the window halo (2 cells below, 3
+// above, pads mirrored as in ratio_cell) are staged in LDS once; the ratios
+// along i and j read their 6-point windows there.  Along k every thread
+// keeps the last six planes of its own column in registers and forms the
+// k-ratios of the plane three behind the front.  The next plane's loads are
+// issued into registers before the current plane is processed.  Same
+// arithmetic (ratio6) as k_ratio, so the two give identical ratios.
+// ---------------------------------------------------------------------------
+constexpr int RB_BI = ZS_TI + 5, RB_BJ = ZS_TJ + 5, RB_NB = RB_BI * RB_BJ;
+constexpr int RB_NQ = 8;                           // rho', e_int, v_0..2, b'_0..2
+constexpr int RB_PER = (RB_NB + ZS_NT - 1) / ZS_NT; // box cells per thread
+
+__device__ __forceinline__ int pad_map(int x, int n) { return x < 0 ? 5 : (x >= n ? n - 6 : x); }
+
+template <typename T>
+__device__ __forceinline__ void ratio_zs_body(const Geo &g, const ZMap &zm,
+                                              const T *__restrict__ u,
+                                              const T *__restrict__ aux, T *__restrict__ R,
+                                              T (*sQ)[RB_NB])
+{
+    typedef Lay<3> LY;
+    typedef Aux<3> AX;
+    int tx, ty, kc;
+    if (!zmap_block(zm, tx, ty, kc)) return;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    const size_t nc = (size_t)g.nc;
+    const ci_t s1 = (ci_t)g.s1, s2 = (ci_t)g.s2;
+    const int ti = threadIdx.x % ZS_TI, tj = threadIdx.x / ZS_TI;
+    const int i0 = 1 + tx * ZS_TI, j0 = 1 + ty * ZS_TJ;
+    const int i = i0 + ti, j = j0 + tj;
+    const bool act = i <= n0 - 3 && j <= n1 - 3;
+    const int k0 = 1 + kc * zm.kz;
+    const int k1 = min(k0 + zm.kz, n2 - 2);
+    const int own = (tj + 2) * RB_BI + ti + 2;
+    // in-plane offsets of this thread's box cells (pads mirrored)
+    ci_t bc[RB_PER];
+    bool bv[RB_PER];
+#pragma unroll
+    for (int q = 0; q < RB_PER; q++) {
+        const int b = threadIdx.x + q * ZS_NT;
+        bv[q] = b < RB_NB;
+        const int bi = b % RB_BI, bj = b / RB_BI;
+        const int gi = pad_map(i0 - 2 + bi, n0), gj = pad_map(j0 - 2 + bj, n1);
+        bc[q] = (ci_t)gi + s1 * (ci_t)gj;
+    }
+    const ci_t cell = (ci_t)i + s1 * (ci_t)j;
+    auto load = [&](T (*v)[RB_NQ], int P) {
+        const ci_t pc = s2 * (ci_t)pad_map(P, n2);
+#pragma unroll
+        for (int q = 0; q < RB_PER; q++)
+            if (bv[q]) {
+                const ci_t c = bc[q] + pc;
+                v[q][0] = ld(u, c);
+                v[q][1] = ld(aux + AX::EINT * nc, c);
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    v[q][2 + d] = ld(aux + (AX::V0 + d) * nc, c);
+                    v[q][5 + d] = ld(u + (LY::B0 + d) * nc, c);
+                }
+            }
+    };
+    // own-column window along k: planes P-5..P of rho', e_int, v_0..2, b'_0, b'_1
+    T zw[6][7];
+#pragma unroll
+    for (int p = 0; p < 6; p++)
+#pragma unroll
+        for (int q = 0; q < 7; q++) zw[p][q] = 0;
+    T nx[RB_PER][RB_NQ];
+    load(nx, k0 - 2);
+    for (int P = k0 - 2; P <= k1 + 2; P++) {
+        __syncthreads(); // the previous plane's LDS reads are done
+#pragma unroll
+        for (int q = 0; q < RB_PER; q++)
+            if (bv[q]) {
+                const int b = threadIdx.x + q * ZS_NT;
+#pragma unroll
+                for (int f = 0; f < RB_NQ; f++) sQ[f][b] = nx[q][f];
+            }
+        if (P + 1 <= k1 + 2) load(nx, P + 1);
+        __syncthreads();
+        if (!act) continue;
+#pragma unroll
+        for (int p = 0; p < 5; p++)
+#pragma unroll
+            for (int q = 0; q < 7; q++) zw[p][q] = zw[p + 1][q];
+#pragma unroll
+        for (int q = 0; q < 7; q++) zw[5][q] = sQ[q][own];
+        if (P >= k0 && P < k1) {
+            const ci_t c = cell + s2 * (ci_t)P;
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                const int st_ = d == 0 ? 1 : RB_BI;
+                auto win = [&](int f, T w[6]) {
+#pragma unroll
+                    for (int p = 0; p < 6; p++) w[p] = sQ[f][own + (p - 2) * st_];
+                };
+                T w[6];
+                win(0, w);
+                st(R + LY::prho(d) * nc, c, ratio6<T>(w));
+                win(1, w);
+                st(R + LY::pe(d) * nc, c, ratio6<T>(w));
+#pragma unroll
+                for (int f = 0; f < 3; f++) {
+                    win(2 + f, w);
+                    st(R + LY::pv(f, d) * nc, c, ratio6<T>(w));
+                }
+#pragma unroll
+                for (int f = 0; f < 3; f++) {
+                    if (f == d) continue;
+                    win(5 + f, w);
+                    st(R + LY::pb(f, d) * nc, c, ratio6<T>(w));
+                }
+            }
+        }
+        const int O = P - 3; // the window planes O-2..O+3 are in zw
+        if (O >= k0 && O < k1) {
+            const ci_t c = cell + s2 * (ci_t)O;
+            T w[6];
+            auto zwin = [&](int q) {
+#pragma unroll
+                for (int p = 0; p < 6; p++) w[p] = zw[p][q];
+            };
+            zwin(0);
+            st(R + LY::prho(2) * nc, c, ratio6<T>(w));
+            zwin(1);
+            st(R + LY::pe(2) * nc, c, ratio6<T>(w));
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                zwin(2 + f);
+                st(R + LY::pv(f, 2) * nc, c, ratio6<T>(w));
+            }
+#pragma unroll
+            for (int f = 0; f < 2; f++) {
+                zwin(5 + f);
+                st(R + LY::pb(f, 2) * nc, c, ratio6<T>(w));
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(ZS_NT) void k_ratio_zs_f(Geo g, ZMap zm, const float *__restrict__ u,
+                                                       const float *__restrict__ aux,
+                                                       float *__restrict__ R)
+{
+    __shared__ float sQ[RB_NQ][RB_NB];
+    ratio_zs_body<float>(g, zm, u, aux, R, sQ);
+}
+
+__global__ __launch_bounds__(ZS_NT) void k_ratio_zs_d(Geo g, ZMap zm, const double *__restrict__ u,
+                                                       const double *__restrict__ aux,
+                                                       double *__restrict__ R)
+{
+    __shared__ double sQ[RB_NQ][RB_NB];
+    ratio_zs_body<double>(g, zm, u, aux, R, sQ);
+}
+
+// ---------------------------------------------------------------------------
 // Hyperdiffusive sources with the plane staged in LDS.  A block takes a
 // 64 x 4 tile of one plane; the state, aux quantities, rho_b and 18 of the
 // 21 ratio pairs of the tile and a 1-cell halo are loaded once, coalesced,

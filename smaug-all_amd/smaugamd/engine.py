@@ -25,7 +25,7 @@ EXPORTS = [
     "smaug_rhs", "smaug_cmax", "smaug_maxcourant", "smaug_rk4", "smaug_euler",
     "smaug_time_rk4", "smaug_ncell", "smaug_ninterior", "smaug_copy_probe",
     "smaug_set_panel", "smaug_get_panel", "smaug_time_pass", "smaug_autotune",
-    "smaug_set_zstream", "smaug_get_zstream", "smaug_time_passes",
+    "smaug_set_zstream", "smaug_get_zstream", "smaug_time_passes", "smaug_set_ratio_form",
     "smaug_ens_create", "smaug_ens_destroy", "smaug_ens_nrun", "smaug_ens_ninterior",
     "smaug_ens_set_state", "smaug_ens_get_state", "smaug_ens_rk4", "smaug_ens_time_rk4",
 ]
@@ -87,6 +87,7 @@ def load_library(path: str = LIB_PATH):
         lib.smaug_autotune.argtypes = [vp, ctypes.c_int, ip, dp]
         lib.smaug_set_zstream.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         lib.smaug_get_zstream.argtypes = [vp, ip, ip]
+        lib.smaug_set_ratio_form.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         lib.smaug_time_passes.argtypes = [vp, ctypes.c_double, ctypes.c_double, dp]
         lib.smaug_ens_create.argtypes = [ctypes.POINTER(SmaugCfg), ctypes.c_int,
                                          ctypes.POINTER(vp)]
@@ -217,6 +218,10 @@ class Engine:
         ms = (ctypes.c_double * 5)()
         self._check(self.lib.smaug_time_passes(self.h, t0, dt, ms))
         return dict(zip(self.PASSES, list(ms)))
+
+    def set_ratio_form(self, zstreamed, kz=0):
+        """3-D: z-streamed hyperviscosity ratios (default) or per cell"""
+        self._check(self.lib.smaug_set_ratio_form(self.h, int(zstreamed), kz))
 
     def get_zstream(self):
         """(form of the 3-D update pass: 0 per cell, 1 z-streamed, 2 z-streamed

@@ -71,3 +71,21 @@ def test_rk4_fp64_vs_oracle(name, setup, kz):
     err = rel_err(got, ref, interior_mask(setup))
     print(name, "rk4 rel err vs oracle", err)
     assert np.all(err < TOL_ORACLE_FP64), err
+
+
+@pytest.mark.parametrize("prec", [FP64, MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("name,setup,kz", CASES, ids=[c[0] for c in CASES])
+def test_zstreamed_ratios_match_per_cell(name, setup, kz, prec):
+    # the z-streamed ratio kernel evaluates the same windows with the same
+    # arithmetic as the per-cell one, so L(w) is identical bit for bit
+    print(f"[case] zstream ratios {name} kz={kz} prec={prec}", flush=True)
+    o = Oracle(setup)
+    w = o.boundary(perturbed(setup, amp=1e-2, seed=8))
+    e = Engine(setup, prec)
+    e.set_state(w)
+    e.set_ratio_form(False)
+    ref = e.rhs(t=1.0)
+    e.set_ratio_form(True, kz)
+    got = e.rhs(t=1.0)
+    e.close()
+    assert np.array_equal(got, ref)
