@@ -93,6 +93,7 @@ struct smaug_engine {
     int hyper_lds;           // 3-D: hyperdiffusive sources with the plane staged in LDS
     int ratio_zs;            // 3-D: z-streamed hyperviscosity ratios
     int kzr;                 // their z-stream depth (0: 32)
+    int kzh;                 // z-stream depth of the z-streamed hyperdiffusive sources (0: 32)
 };
 
 // ---------------------------------------------------------------------------
@@ -203,9 +204,38 @@ static ZMap make_zmap(const smaug_engine *e, unsigned *nblocks)
 
 // 3-D z-streamed form: the hyperdiffusive sources of uin into HYP (the
 // stage update adds them)
+static ZMap make_hmap(const smaug_engine *e, unsigned *nblocks)
+{
+    ZMap m;
+    m.ntx = (e->n[0] - 4 + HB_TI - 1) / HB_TI;
+    m.nty = (e->n[1] - 4 + HB_TJ - 1) / HB_TJ;
+    const int nk = e->n[2] - 4;
+    m.kz = e->kzh > 0 ? (e->kzh < nk ? e->kzh : nk) : (nk < 32 ? nk : 32);
+    m.nkz = (nk + m.kz - 1) / m.kz;
+    m.per_xcd = (m.ntx * m.nty + 7) / 8;
+    *nblocks = (unsigned)(8 * m.per_xcd * m.nkz);
+    return m;
+}
+
 template <typename T>
 static int launch_hyper_sources(smaug_engine *e, const void *uin)
 {
+    if (e->phys.hyperdif && e->hyper_lds == 2) {
+        unsigned nb;
+        ZMap m = make_hmap(e, &nb);
+        if constexpr (sizeof(T) == 4)
+            hipLaunchKernelGGL(k_hyper_zs_f, dim3(nb), dim3(256), 0, e->stream, e->geo, m, e->phys,
+                               (const T *)uin, (const T *)e->R, (const T *)e->AUX,
+                               (const double *)e->bg, (const unsigned long long *)e->d_cmax,
+                               (T *)e->HYP);
+        else
+            hipLaunchKernelGGL(k_hyper_zs_d, dim3(nb), dim3(256), 0, e->stream, e->geo, m, e->phys,
+                               (const T *)uin, (const T *)e->R, (const T *)e->AUX,
+                               (const double *)e->bg, (const unsigned long long *)e->d_cmax,
+                               (T *)e->HYP);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
     if (e->phys.hyperdif) {
         unsigned nh;
         TileMap tm = make_map(e->n[0] - 4, e->n[1] - 4, e->n[2] - 4, e->panel[1][0],
@@ -471,7 +501,8 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
     p.driver = cfg->driver;
     e->zstream = D == 3;
     e->kz = 0;
-    e->hyper_lds = 1;
+    e->hyper_lds = 2;
+    e->kzh = 0;
     e->ratio_zs = D == 3;
     e->kzr = 0;
     *out = e;
@@ -813,7 +844,9 @@ extern "C" int smaug_set_zstream(smaug_engine *e, int enable, int kz)
     if (!e || kz < 0) return fail(SMAUG_EINVAL, "bad argument");
     if (enable && e->D != 3) return fail(SMAUG_EINVAL, "the z-streamed update is for 3-D grids");
     e->zstream = enable ? 1 : 0;
-    e->hyper_lds = enable != 2;
+    // hyperdiffusive sources: 1 z-streamed (the default), 2 per cell, 3 plane
+    // in LDS
+    e->hyper_lds = enable == 2 ? 0 : (enable == 3 ? 1 : 2);
     e->kz = kz;
     drop_graphs(e);
     return 0;
@@ -882,21 +915,29 @@ extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double 
             e->kzr = bkz > 0 ? bkz : 0;
         }
         if (pass == 1 && e->D == 3) {
+            // z-streamed update with the z-streamed (hyper_lds 2) or the
+            // LDS-plane (1) hyperdiffusive-source kernel
             static const int KZ[] = {4, 8, 16, 32, 64};
-            int bkz = -1;
+            int bkz = -1, bhy = 2;
             e->zstream = 1;
-            for (int kz : KZ) {
-                e->kz = kz;
-                double ms;
-                int rc = DISPATCH(e, time_pass, e, pass, reps, &ms);
-                if (rc) return rc;
-                if (ms < bm) {
-                    bm = ms;
-                    bkz = kz;
+            for (int hy = 2; hy >= 1; hy--) {
+                e->hyper_lds = hy;
+                for (int kz : KZ) {
+                    e->kz = kz;
+                    e->kzh = kz;
+                    double ms;
+                    int rc = DISPATCH(e, time_pass, e, pass, reps, &ms);
+                    if (rc) return rc;
+                    if (ms < bm) {
+                        bm = ms;
+                        bkz = kz;
+                        bhy = hy;
+                    }
                 }
             }
             e->zstream = bkz > 0;
-            e->kz = bkz > 0 ? bkz : 0;
+            e->kz = e->kzh = bkz > 0 ? bkz : 0;
+            e->hyper_lds = bhy;
         }
         if (best) {
             best[pass][0] = bx;
@@ -925,7 +966,7 @@ extern "C" int smaug_set_ratio_form(smaug_engine *e, int zstreamed, int kz)
 extern "C" int smaug_get_zstream(smaug_engine *e, int *enable, int *kz)
 {
     if (!e || !enable || !kz) return fail(SMAUG_EINVAL, "bad argument");
-    *enable = e->zstream ? (e->hyper_lds ? 1 : 2) : 0;
+    *enable = e->zstream ? (e->hyper_lds == 2 ? 1 : (e->hyper_lds == 1 ? 3 : 2)) : 0;
     unsigned nb;
     *kz = e->D == 3 ? make_zmap(e, &nb).kz : 0;
     return 0;

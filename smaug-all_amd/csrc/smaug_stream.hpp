@@ -733,6 +733,544 @@ __global__ __launch_bounds__(256) void k_hyper_lds_d(Geo g, TileMap tm, Phys ph,
     hyper_lds_body<double>(g, tm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRb, sR);
 }
 
+// ---------------------------------------------------------------------------
+// Hyperdiffusive sources, z-streamed.  A block owns 64 x 4 columns and walks
+// a chunk of output planes.  Per plane P (the "centre") the state, AUX,
+// rho_b and the 14 in-plane ratio pairs of the tile + 1-cell halo are staged
+// in LDS, and:
+//  - the terms along i and j of output P (diagonal, and the off-diagonal
+//    pair (i, j)) read LDS;
+//  - the k-coupled off-diagonal products X_{k,d}(P) (d in-plane) of the own
+//    column are formed from LDS and scattered into outputs P-1 and P+1;
+//  - the diagonal k-terms of output P-1 use the own column's values of
+//    planes P-2..P, kept in registers;
+//  - the products X_{f,k}(P-1) (f in-plane) are formed by the owner of each
+//    box column from its planes P-2..P (registers) and shared through LDS;
+// and output P-1 is written.  The box cells each thread loads stay fixed
+// along the stream, so every history is private to its thread.  The next
+// plane's loads are issued before the current plane is processed.  The
+// arithmetic of each term is that of hyper_terms; only the summation order
+// differs.
+// ---------------------------------------------------------------------------
+constexpr int HZ_NIP = 14; // in-plane ratio pairs staged in LDS (every pair with d < 2)
+
+__host__ __device__ constexpr int hz_islot(int p)
+{
+    // pairs along d = 2 (3-D numbering): 2, 5, 8, 11, 14, 16, 18
+    return (p == 2 || p == 5 || p == 8 || p == 11 || p == 14 || p == 16 || p == 18)
+               ? -1
+               : p - (p > 2) - (p > 5) - (p > 8) - (p > 11) - (p > 14) - (p > 16) - (p > 18);
+}
+
+template <typename T>
+struct HzCell {
+    T r, e, m[3], b[3], v[3];
+    double rb;
+    T R[21];
+};
+
+// all 21 ratio pairs for the own column (full = true); a halo column needs
+// only the in-plane pairs and pv/pb(f, 2), f in-plane
+template <typename T>
+__device__ __forceinline__ void hz_load(HzCell<T> &h, const T *__restrict__ u,
+                                        const T *__restrict__ aux, const double *__restrict__ bg,
+                                        const T *__restrict__ R, size_t nc, ci_t c, bool full)
+{
+    typedef Lay<3> LY;
+    typedef Aux<3> AX;
+    h.r = ld(u, c);
+    h.e = ld(aux + AX::EINT * nc, c);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        h.m[d] = ld(u + (LY::M0 + d) * nc, c);
+        h.b[d] = ld(u + (LY::B0 + d) * nc, c);
+        h.v[d] = ld(aux + (AX::V0 + d) * nc, c);
+    }
+    h.rb = ld(bg + nc, c);
+#pragma unroll
+    for (int p = 0; p < LY::NPAIR; p++)
+        h.R[p] = (full || hz_islot(p) >= 0 || p == LY::pv(0, 2) || p == LY::pv(1, 2) ||
+                  p == LY::pb(0, 2) || p == LY::pb(1, 2))
+                     ? ld(R + p * nc, c)
+                     : (T)0;
+}
+
+// LDS view of the centre plane for the in-plane terms (accesses with dk = 0)
+template <typename T>
+struct HzView {
+    const T (*sU)[HB_NB]; // rho', m_0..2, b'_0..2
+    const T (*sA)[HB_NB]; // e_int, v_0..2
+    const double *sRb;
+    const T (*sR)[HB_NB];
+    int own;
+    __device__ __forceinline__ int bx(int di, int dj) const { return own + di + HB_BI * dj; }
+    __device__ __forceinline__ T U(int f, int di, int dj) const { return sU[hb_uslot(f)][bx(di, dj)]; }
+    __device__ __forceinline__ T A(int a, int di, int dj) const { return sA[a - Aux<3>::EINT][bx(di, dj)]; }
+    __device__ __forceinline__ double rhob(int di, int dj) const { return sRb[bx(di, dj)]; }
+    __device__ __forceinline__ T Rp(int p, int di, int dj) const { return sR[hz_islot(p)][bx(di, dj)]; }
+};
+
+template <typename T>
+__device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, const Phys &ph,
+                                              const T *__restrict__ uin,
+                                              const T *__restrict__ R,
+                                              const T *__restrict__ aux,
+                                              const double *__restrict__ bg,
+                                              const unsigned long long *__restrict__ cmb,
+                                              T *__restrict__ dUh, T (*sU)[HB_NB],
+                                              T (*sA)[HB_NB], double *sRb, T (*sR)[HB_NB],
+                                              T (*sX)[HB_NB])
+{
+    typedef Lay<3> LY;
+    typedef Aux<3> AX;
+    int tx, ty, kc;
+    if (!zmap_block(zm, tx, ty, kc)) return;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    const size_t nc = (size_t)g.nc;
+    const ci_t s1 = (ci_t)g.s1, s2 = (ci_t)g.s2;
+    const int ti = threadIdx.x % HB_TI, tj = threadIdx.x / HB_TI;
+    const int i0 = 2 + tx * HB_TI, j0 = 2 + ty * HB_TJ;
+    const int i = i0 + ti, j = j0 + tj;
+    const bool act = i <= n0 - 3 && j <= n1 - 3;
+    const int k0 = 2 + kc * zm.kz;
+    const int k1 = min(k0 + zm.kz, n2 - 2);
+    double cmax[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) cmax[d] = __longlong_as_double((long long)cmb[d]);
+    // box cells of this thread: 0 = its own column, 1 = a halo column (if any)
+    int bb[2], bi_[2], bj_[2];
+    bool bv[2];
+    ci_t bc[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        int bi, bj;
+        if (q == 0) {
+            bi = ti + 1;
+            bj = tj + 1;
+            bv[q] = true;
+        } else {
+            bv[q] = threadIdx.x < HB_NH;
+            bi = bj = 0;
+            if (bv[q]) hb_halo(threadIdx.x, bi, bj);
+        }
+        const int gi = i0 - 1 + bi, gj = j0 - 1 + bj;
+        bv[q] = bv[q] && gi <= n0 - 1 && gj <= n1 - 1;
+        bi_[q] = gi;
+        bj_[q] = gj;
+        bb[q] = bj * HB_BI + bi;
+        bc[q] = (ci_t)gi + s1 * (ci_t)gj;
+    }
+    const int own = bb[0];
+    HzView<T> vw;
+    vw.sU = sU;
+    vw.sA = sA;
+    vw.sRb = sRb;
+    vw.sR = sR;
+    vw.own = own;
+
+    // per box column, planes P-1 and P-2 (index 0 and 1): v_0, v_1, b_0, b_1,
+    // and the k-pairs pv(0,2), pv(1,2), pb(0,2), pb(1,2); rho, m_2, b_2 of P-1
+    T hv[2][2][4], hr[2][2][4], hn[2][3];
+    // own column, planes P-1 and P-2: the diagonal k-term inputs not already
+    // in hv/hr (rho', e_int, v_2, m_0..2; prho(2), pe(2), pv(2,2); rho_b)
+    T ow[2][6], oz[2][3];
+    double orb[2];
+    // pending outputs P-1, P, P+1
+    T rA[8], rB[8], rC[8];
+#pragma unroll
+    for (int f = 0; f < 8; f++) rA[f] = rB[f] = rC[f] = 0;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+#pragma unroll
+        for (int t = 0; t < 2; t++)
+#pragma unroll
+            for (int f = 0; f < 4; f++) hv[q][t][f] = hr[q][t][f] = 0;
+#pragma unroll
+        for (int f = 0; f < 3; f++) hn[q][f] = 0;
+    }
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+#pragma unroll
+        for (int f = 0; f < 6; f++) ow[t][f] = 0;
+#pragma unroll
+        for (int f = 0; f < 3; f++) oz[t][f] = 0;
+        orb[t] = 0;
+    }
+    const bool zper = g.per[2];
+
+    HzCell<T> cur[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+        if (bv[q]) hz_load<T>(cur[q], uin, aux, bg, R, nc, bc[q] + s2 * (ci_t)(k0 - 1), q == 0);
+
+    for (int P = k0 - 1; P <= k1; P++) {
+        __syncthreads(); // the previous plane's LDS reads are done
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+            if (bv[q]) {
+                const int b = bb[q];
+                sU[0][b] = cur[q].r;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    sU[1 + d][b] = cur[q].m[d];
+                    sU[4 + d][b] = cur[q].b[d];
+                    sA[1 + d][b] = cur[q].v[d];
+                }
+                sA[0][b] = cur[q].e;
+                sRb[b] = cur[q].rb;
+#pragma unroll
+                for (int p = 0; p < LY::NPAIR; p++)
+                    if (hz_islot(p) >= 0) sR[hz_islot(p)][b] = cur[q].R[p];
+            }
+        // X_{f,k}(P-1) of this thread's box columns, from planes P-2, P-1
+        // (registers) and P (cur)
+        const int O = P - 1;
+        const bool outO = O >= k0 && O < k1;
+        if (outO) {
+            const double dxz = g.dx[2][O];
+            const T idz = (T)g.idx[2][O];
+            const bool z1 = zero1(zper, O, n2);
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                if (!bv[q]) continue;
+                const int xq[2] = {bi_[q], bj_[q]};
+#pragma unroll
+                for (int f = 0; f < 2; f++) {
+                    T Xr = 0, Xm = 0, Hb = 0, He = 0;
+                    if (xq[f] >= 2 && xq[f] <= g.n[f] - 3) {
+                        // ca = p - e_k (plane P-2), cb = p + e_k (plane P)
+                        const T dv = z1 ? (T)0 : (cur[q].v[f] - hv[q][1][f]) * idz / (T)2;
+                        const T db = z1 ? (T)0 : (cur[q].b[f] - hv[q][1][2 + f]) * idz / (T)2;
+                        const T Km = (T)(dxz * cmax[2] * ph.chyp[LY::M0 + f]);
+                        const T sv = Km * (hr[q][0][f] + hr[q][1][f]);
+                        const T G = dv * sv / (T)4;
+                        Xr = hn[q][0] * G;
+                        Xm = G * hn[q][1];
+                        const T Kb = (T)(dxz * cmax[2] * ph.chyp[LY::B0 + f]);
+                        const T sb = Kb * (hr[q][0][2 + f] + hr[q][1][2 + f]);
+                        const T H = db * sb / (T)2;
+                        Hb = H;
+                        He = H * hn[q][2];
+                    }
+                    sX[4 * f + 0][bb[q]] = Xr;
+                    sX[4 * f + 1][bb[q]] = Xm;
+                    sX[4 * f + 2][bb[q]] = Hb;
+                    sX[4 * f + 3][bb[q]] = He;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        // ---- diagonal k-terms of output P-1: own column planes P-2, P-1
+        // (registers) and P (cur)
+        if (act && outO) {
+            // plane t (0: P-1, 1: P-2) or P (t = 2) of quantity k: rho', e_int,
+            // v_0..2, m_0..2, b'_0, b'_1
+            auto W = [&](int t, int k) -> T {
+                if (t == 2) {
+                    if (k == 0) return cur[0].r;
+                    if (k == 1) return cur[0].e;
+                    if (k < 5) return cur[0].v[k - 2];
+                    if (k < 8) return cur[0].m[k - 5];
+                    return cur[0].b[k - 8];
+                }
+                if (k == 0) return ow[t][0];
+                if (k == 1) return ow[t][1];
+                if (k == 2 || k == 3) return hv[0][t][k - 2];
+                if (k == 4) return ow[t][2];
+                if (k < 8) return ow[t][3 + k - 5];
+                return hv[0][t][2 + k - 8];
+            };
+            // ratio pair t of the k-set (prho, pe, pv(0..2), pb(0..1) along k)
+            auto Z = [&](int t, int k) -> T {
+                if (k == 0 || k == 1) return oz[t][k];
+                if (k == 2 || k == 3) return hr[0][t][k - 2];
+                if (k == 4) return oz[t][2];
+                return hr[0][t][2 + k - 5];
+            };
+            const int xd = O;
+            const double dxd = g.dx[2][xd];
+            const T idxd = (T)g.idx[2][xd];
+            const bool z1 = zero1(zper, xd, n2);
+            auto nuRL = [&](int t, int field, T &nR, T &nL) {
+                const T K = (T)(dxd * cmax[2] * ph.chyp[field]);
+                nR = K * Z(0, t);
+                nL = K * Z(1, t);
+            };
+            T nR, nL, gR, gL;
+            const T rc = W(0, 0);
+            const double rhoc = (double)rc + orb[0];
+            nuRL(0, LY::RHO, nR, nL);
+            gR = z1 ? (T)0 : (W(2, 0) - rc) * idxd;
+            gL = z1 ? (T)0 : (rc - W(1, 0)) * idxd;
+            rA[LY::RHO] += (nR * gR - nL * gL) * idxd;
+            nuRL(1, LY::EN, nR, nL);
+            gR = z1 ? (T)0 : (W(2, 1) - W(0, 1)) * idxd;
+            gL = z1 ? (T)0 : (W(0, 1) - W(1, 1)) * idxd;
+            rA[LY::EN] += (nR * gR - nL * gL) * idxd;
+            const T rR = (T)((rhoc + (double)W(2, 0) + cur[0].rb) / 2);
+            const T rL = (T)((rhoc + (double)W(1, 0) + orb[1]) / 2);
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                nuRL(2 + f, LY::M0 + f, nR, nL);
+                const T vc = W(0, 2 + f);
+                gR = z1 ? (T)0 : (W(2, 2 + f) - vc) * idxd;
+                gL = z1 ? (T)0 : (vc - W(1, 2 + f)) * idxd;
+                const T mc = W(0, 5 + f);
+                const T mR = (mc + W(2, 5 + f)) / (T)2, mL = (mc + W(1, 5 + f)) / (T)2;
+                const T k2 = (f == 2) ? (T)1 : (T)0.5;
+                rA[LY::M0 + f] += k2 * (rR * nR * gR - rL * nL * gL) * idxd;
+                rA[LY::EN] += k2 * (mR * nR * gR - mL * nL * gL) * idxd;
+            }
+#pragma unroll
+            for (int f = 0; f < 2; f++) {
+                nuRL(5 + f, LY::B0 + f, nR, nL);
+                const T bc_ = W(0, 8 + f), br = W(2, 8 + f), bl = W(1, 8 + f);
+                gR = z1 ? (T)0 : (br - bc_) * idxd;
+                gL = z1 ? (T)0 : (bc_ - bl) * idxd;
+                const T bR = (bc_ + br) / (T)2, bL = (bc_ + bl) / (T)2;
+                rA[LY::B0 + f] += (gR * nR - gL * nL) * idxd;
+                rA[LY::EN] += (bR * gR * nR - bL * gL * nL) * idxd;
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        // the histories move up one plane
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+            if (bv[q]) {
+#pragma unroll
+                for (int f = 0; f < 2; f++) {
+                    hv[q][1][f] = hv[q][0][f];
+                    hv[q][1][2 + f] = hv[q][0][2 + f];
+                    hv[q][0][f] = cur[q].v[f];
+                    hv[q][0][2 + f] = cur[q].b[f];
+                    hr[q][1][f] = hr[q][0][f];
+                    hr[q][1][2 + f] = hr[q][0][2 + f];
+                    hr[q][0][f] = cur[q].R[LY::pv(f, 2)];
+                    hr[q][0][2 + f] = cur[q].R[LY::pb(f, 2)];
+                }
+                hn[q][0] = (T)((double)cur[q].r + cur[q].rb);
+                hn[q][1] = cur[q].m[2];
+                hn[q][2] = cur[q].b[2];
+            }
+        if (act) {
+#pragma unroll
+            for (int f = 0; f < 6; f++) ow[1][f] = ow[0][f];
+#pragma unroll
+            for (int f = 0; f < 3; f++) oz[1][f] = oz[0][f];
+            orb[1] = orb[0];
+            ow[0][0] = cur[0].r;
+            ow[0][1] = cur[0].e;
+            ow[0][2] = cur[0].v[2];
+#pragma unroll
+            for (int f = 0; f < 3; f++) ow[0][3 + f] = cur[0].m[f];
+            oz[0][0] = cur[0].R[LY::prho(2)];
+            oz[0][1] = cur[0].R[LY::pe(2)];
+            oz[0][2] = cur[0].R[LY::pv(2, 2)];
+            orb[0] = cur[0].rb;
+        }
+        __syncthreads();
+        if (act) {
+            const int xs[2] = {i, j};
+            // ---- in-plane terms of output P
+            if (P >= k0 && P < k1) {
+                const T rc = vw.U(LY::RHO, 0, 0);
+                const double rhoc = (double)rc + vw.rhob(0, 0);
+                const T eic = vw.A(AX::EINT, 0, 0);
+#pragma unroll
+                for (int d = 0; d < 2; d++) {
+                    const int ri = d == 0, rj = d == 1;
+                    const int xd = xs[d], nd = g.n[d];
+                    const double dxd = g.dx[d][xd];
+                    const T idxd = (T)g.idx[d][xd];
+                    const bool z1 = zero1(g.per[d], xd, nd);
+                    auto nuRL = [&](int pair, int field, T &nR, T &nL) {
+                        const T K = (T)(dxd * cmax[d] * ph.chyp[field]);
+                        nR = K * vw.Rp(pair, 0, 0);
+                        nL = K * vw.Rp(pair, -ri, -rj);
+                    };
+                    T nR, nL, gR, gL;
+                    nuRL(LY::prho(d), LY::RHO, nR, nL);
+                    gR = z1 ? (T)0 : (vw.U(LY::RHO, ri, rj) - rc) * idxd;
+                    gL = z1 ? (T)0 : (rc - vw.U(LY::RHO, -ri, -rj)) * idxd;
+                    rB[LY::RHO] += (nR * gR - nL * gL) * idxd;
+                    nuRL(LY::pe(d), LY::EN, nR, nL);
+                    gR = z1 ? (T)0 : (vw.A(AX::EINT, ri, rj) - eic) * idxd;
+                    gL = z1 ? (T)0 : (eic - vw.A(AX::EINT, -ri, -rj)) * idxd;
+                    rB[LY::EN] += (nR * gR - nL * gL) * idxd;
+                    const T rR = (T)((rhoc + (double)vw.U(LY::RHO, ri, rj) + vw.rhob(ri, rj)) / 2);
+                    const T rL = (T)((rhoc + (double)vw.U(LY::RHO, -ri, -rj) + vw.rhob(-ri, -rj)) / 2);
+#pragma unroll
+                    for (int f = 0; f < 3; f++) {
+                        nuRL(LY::pv(f, d), LY::M0 + f, nR, nL);
+                        const T vc = vw.A(AX::V0 + f, 0, 0);
+                        gR = z1 ? (T)0 : (vw.A(AX::V0 + f, ri, rj) - vc) * idxd;
+                        gL = z1 ? (T)0 : (vc - vw.A(AX::V0 + f, -ri, -rj)) * idxd;
+                        const T mc = vw.U(LY::M0 + f, 0, 0);
+                        const T mR = (mc + vw.U(LY::M0 + f, ri, rj)) / (T)2;
+                        const T mL = (mc + vw.U(LY::M0 + f, -ri, -rj)) / (T)2;
+                        const T k2 = (f == d) ? (T)1 : (T)0.5;
+                        rB[LY::M0 + f] += k2 * (rR * nR * gR - rL * nL * gL) * idxd;
+                        rB[LY::EN] += k2 * (mR * nR * gR - mL * nL * gL) * idxd;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int f = 0; f < 3; f++) {
+                        if (f == d) continue;
+                        nuRL(LY::pb(f, d), LY::B0 + f, nR, nL);
+                        const T bc_ = vw.U(LY::B0 + f, 0, 0);
+                        const T br = vw.U(LY::B0 + f, ri, rj), bl = vw.U(LY::B0 + f, -ri, -rj);
+                        gR = z1 ? (T)0 : (br - bc_) * idxd;
+                        gL = z1 ? (T)0 : (bc_ - bl) * idxd;
+                        const T bR = (bc_ + br) / (T)2, bL = (bc_ + bl) / (T)2;
+                        rB[LY::B0 + f] += (gR * nR - gL * nL) * idxd;
+                        rB[LY::EN] += (bR * gR * nR - bL * gL * nL) * idxd;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                    // off-diagonal pair (f, d) with f the other in-plane dim
+                    const int f = 1 - d;
+                    const int xf = xs[f], nf = g.n[f];
+                    if (!zero1(g.per[f], xf, nf)) {
+                        T Xr[2], Xm[2], Hb[2], He[2];
+#pragma unroll
+                        for (int side = 0; side < 2; side++) {
+                            const int sg = side ? 1 : -1;
+                            const int ni = f == 0 ? sg : 0, nj = f == 1 ? sg : 0;
+                            Xr[side] = Xm[side] = Hb[side] = He[side] = 0;
+                            const int xfn = xf + sg;
+                            if (xfn < 2 || xfn > nf - 3) continue;
+                            const int ai = ni - ri, aj = nj - rj, bi = ni + ri, bj = nj + rj;
+                            const T dv = z1 ? (T)0
+                                            : (vw.A(AX::V0 + f, bi, bj) - vw.A(AX::V0 + f, ai, aj)) *
+                                                  idxd / (T)2;
+                            const T db = z1 ? (T)0
+                                            : (vw.U(LY::B0 + f, bi, bj) - vw.U(LY::B0 + f, ai, aj)) *
+                                                  idxd / (T)2;
+                            {
+                                const T K = (T)(dxd * cmax[d] * ph.chyp[LY::M0 + f]);
+                                const T s2_ = K * (vw.Rp(LY::pv(f, d), ni, nj) + vw.Rp(LY::pv(f, d), ai, aj));
+                                const T G = dv * s2_ / (T)4;
+                                const T rhon = (T)((double)vw.U(LY::RHO, ni, nj) + vw.rhob(ni, nj));
+                                Xr[side] = rhon * G;
+                                Xm[side] = G * vw.U(LY::M0 + d, ni, nj);
+                            }
+                            {
+                                const T K = (T)(dxd * cmax[d] * ph.chyp[LY::B0 + f]);
+                                const T s2_ = K * (vw.Rp(LY::pb(f, d), ni, nj) + vw.Rp(LY::pb(f, d), ai, aj));
+                                const T H = db * s2_ / (T)2;
+                                Hb[side] = H;
+                                He[side] = H * vw.U(LY::B0 + d, ni, nj);
+                            }
+                        }
+                        const T h = (T)g.idx[f][xf] / (T)2;
+                        rB[LY::M0 + d] += (Xr[1] - Xr[0]) * h;
+                        rB[LY::EN] += (Xm[1] - Xm[0]) * h;
+                        rB[LY::B0 + d] -= (Hb[1] - Hb[0]) * h;
+                        rB[LY::EN] -= (He[1] - He[0]) * h;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        // ---- X_{k,d}(P) of the own column into outputs P-1 (+) and P+1 (-)
+            if (P >= 2 && P <= n2 - 3) {
+#pragma unroll
+                for (int d = 0; d < 2; d++) {
+                    const int ri = d == 0, rj = d == 1;
+                    const int xd = xs[d], nd = g.n[d];
+                    const double dxd = g.dx[d][xd];
+                    const T idxd = (T)g.idx[d][xd];
+                    const bool z1 = zero1(g.per[d], xd, nd);
+                    const T dv = z1 ? (T)0
+                                    : (vw.A(AX::V0 + 2, ri, rj) - vw.A(AX::V0 + 2, -ri, -rj)) * idxd / (T)2;
+                    const T db = z1 ? (T)0
+                                    : (vw.U(LY::B0 + 2, ri, rj) - vw.U(LY::B0 + 2, -ri, -rj)) * idxd / (T)2;
+                    const T Km = (T)(dxd * cmax[d] * ph.chyp[LY::M0 + 2]);
+                    const T sv = Km * (vw.Rp(LY::pv(2, d), 0, 0) + vw.Rp(LY::pv(2, d), -ri, -rj));
+                    const T G = dv * sv / (T)4;
+                    const T rhon = (T)((double)vw.U(LY::RHO, 0, 0) + vw.rhob(0, 0));
+                    const T Xr = rhon * G, Xm = G * vw.U(LY::M0 + d, 0, 0);
+                    const T Kb = (T)(dxd * cmax[d] * ph.chyp[LY::B0 + 2]);
+                    const T sb = Kb * (vw.Rp(LY::pb(2, d), 0, 0) + vw.Rp(LY::pb(2, d), -ri, -rj));
+                    const T H = db * sb / (T)2;
+                    const T He = H * vw.U(LY::B0 + d, 0, 0);
+                    // output P-1 sees this column as its c + e_k side
+                    if (P - 1 >= k0 && P - 1 < k1 && !zero1(zper, P - 1, n2)) {
+                        const T h = (T)g.idx[2][P - 1] / (T)2;
+                        rA[LY::M0 + d] += Xr * h;
+                        rA[LY::EN] += Xm * h;
+                        rA[LY::B0 + d] -= H * h;
+                        rA[LY::EN] -= He * h;
+                    }
+                    // output P+1 sees it as its c - e_k side
+                    if (P + 1 >= k0 && P + 1 < k1 && !zero1(zper, P + 1, n2)) {
+                        const T h = (T)g.idx[2][P + 1] / (T)2;
+                        rC[LY::M0 + d] -= Xr * h;
+                        rC[LY::EN] -= Xm * h;
+                        rC[LY::B0 + d] += H * h;
+                        rC[LY::EN] += He * h;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+        }
+        // the next plane's loads, in flight through the rest of this plane
+        if (P + 1 <= k1) {
+#pragma unroll
+            for (int q = 0; q < 2; q++)
+                if (bv[q]) hz_load<T>(cur[q], uin, aux, bg, R, nc, bc[q] + s2 * (ci_t)(P + 1), q == 0);
+        }
+        __syncthreads(); // the products X_{f,k}(P-1) are in LDS
+        if (act && outO) {
+            const int xs[2] = {i, j};
+#pragma unroll
+            for (int f = 0; f < 2; f++) {
+                if (zero1(g.per[f], xs[f], g.n[f])) continue;
+                const int sf = f == 0 ? 1 : HB_BI;
+                const T h = (T)g.idx[f][xs[f]] / (T)2;
+                const T Xr = sX[4 * f + 0][own + sf] - sX[4 * f + 0][own - sf];
+                const T Xm = sX[4 * f + 1][own + sf] - sX[4 * f + 1][own - sf];
+                const T Hb = sX[4 * f + 2][own + sf] - sX[4 * f + 2][own - sf];
+                const T He = sX[4 * f + 3][own + sf] - sX[4 * f + 3][own - sf];
+                rA[LY::M0 + 2] += Xr * h;
+                rA[LY::EN] += Xm * h;
+                rA[LY::B0 + 2] -= Hb * h;
+                rA[LY::EN] -= He * h;
+            }
+            const ci_t co = bc[0] + s2 * (ci_t)O;
+#pragma unroll
+            for (int f = 0; f < 8; f++) st(dUh + f * nc, co, rA[f]);
+        }
+        // advance the output ring and the own-column history
+#pragma unroll
+        for (int f = 0; f < 8; f++) {
+            rA[f] = rB[f];
+            rB[f] = rC[f];
+            rC[f] = 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void
+k_hyper_zs_f(Geo g, ZMap zm, Phys ph, const float *__restrict__ uin, const float *__restrict__ R,
+             const float *__restrict__ aux, const double *__restrict__ bg,
+             const unsigned long long *__restrict__ cmb, float *__restrict__ dUh)
+{
+    __shared__ float sU[7][HB_NB], sA[4][HB_NB], sR[HZ_NIP][HB_NB], sX[8][HB_NB];
+    __shared__ double sRb[HB_NB];
+    hyper_zs_body<float>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRb, sR, sX);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void
+k_hyper_zs_d(Geo g, ZMap zm, Phys ph, const double *__restrict__ uin,
+             const double *__restrict__ R, const double *__restrict__ aux,
+             const double *__restrict__ bg, const unsigned long long *__restrict__ cmb,
+             double *__restrict__ dUh)
+{
+    __shared__ double sU[7][HB_NB], sA[4][HB_NB], sR[HZ_NIP][HB_NB], sX[8][HB_NB];
+    __shared__ double sRb[HB_NB];
+    hyper_zs_body<double>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRb, sR, sX);
+}
+
 // the hyperdiffusive sources of every interior cell into dUh (the stage
 // update adds them); same tiles and XCD panels as k_update
 template <int D, typename T>

@@ -42,8 +42,8 @@ def test_rhs_matches_per_cell_kernel(name, setup, kz, prec):
     e.set_state(w)
     e.set_zstream(False)
     ref = e.rhs(t=2.0)
-    # 1: hyperdiffusive sources from the LDS-staged kernel, 2: per-cell
-    for mode in (1, 2):
+    # hyperdiffusive sources 1: z-streamed, 2: per cell, 3: plane staged in LDS
+    for mode in (1, 2, 3):
         e.set_zstream(mode, kz)
         got = e.rhs(t=2.0)
         err = rel_err(got, ref, interior_mask(setup))
