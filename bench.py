@@ -39,6 +39,9 @@ BYTES_PER_CELL_STEP = {"mixed": 640.0, "fp64": 1152.0}
 # fields read in each of the 4 stages: mixed 96 * 4 + 12 * 8 = 480
 BYTES_PER_CELL_STEP_2D_MIXED = 480.0
 STAGES_PER_STEP = 4
+# forms of the 3-D update pass (smaug_set_zstream / smaug_get_zstream)
+FORMS = {0: "per-cell", 1: "z-streamed update, z-streamed sources",
+         2: "z-streamed update, per-cell sources", 3: "z-streamed update, LDS-plane sources"}
 
 
 def parse():
@@ -52,7 +55,8 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--shape", default=None,
                     help="fixed 3-D update form and z-stream depth 'FORM,KZ' instead of the "
-                         "autotuner (FORM 0 per-cell, 1 z-streamed), e.g. for profiling runs")
+                         "autotuner (FORM as in smaug_set_zstream: 0 per-cell, 1 z-streamed, "
+                         "3 z-streamed with LDS-plane sources), e.g. for profiling runs")
     ap.add_argument("--ens-steps", type=int, default=200)
     return ap.parse_args()
 
@@ -70,11 +74,11 @@ def run_path(setup, precision, warmup, steps, dt_scale=1.0, tune=True, shape=Non
     if shape is not None:
         form, kz = (int(x) for x in shape.split(","))
         e.set_zstream(form, kz)
-        info["tuned"] = {"update_form": ["per-cell", "z-streamed"][form], "kz": kz, "fixed": True}
+        info["tuned"] = {"update_form": FORMS[form], "kz": kz, "fixed": True}
     elif tune:
         best, ms = e.autotune(3)
         form, kz = e.get_zstream()
-        info["tuned"] = {"update_form": ["per-cell", "z-streamed", "z-streamed/per-cell-hyper"][form],
+        info["tuned"] = {"update_form": FORMS[form],
                          "kz": kz, "panel_ratio": list(best[0]), "panel_update": list(best[1])}
         e.set_state()  # the tuning launches leave the state untouched; re-fill anyway
     e.rk4(0.0, dt, warmup)

@@ -86,7 +86,7 @@ int smaug_time_rk4(smaug_engine *e, double t0, double dt, int nsteps,
  * smaug_time_pass times reps launches of a pass on the current state (the
  * state is not changed), and smaug_autotune times every panel of
  * tpx in {1,2,4,8} x tpy in {1,..,32} and, for 3-D grids, the z-streamed
- * ratio kernel at depths 8..64 and the z-streamed update at depths 4..64
+ * ratio kernel at depths 8..128 and the z-streamed update at depths 8..128
  * with either form of the hyperdiffusive-source kernel, and keeps the
  * fastest form per pass. */
 int smaug_set_panel(smaug_engine *e, int pass, int tpx, int tpy);
@@ -94,7 +94,8 @@ int smaug_get_panel(smaug_engine *e, int pass, int *tpx, int *tpy);
 int smaug_time_pass(smaug_engine *e, int pass, int reps, double *ms);
 int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double best_ms[2]);
 /* 3-D grids: the stage update runs z-streamed (enable = 1, the default;
- * blocks of 64 x 4 columns walk kz output planes, kz = 0 picks 32) with the
+ * blocks of 64 x 4 columns walk kz output planes, kz = 0 picks 32; the
+ * z-streamed source kernel uses the same depth) with the
  * z-streamed hyperdiffusive-source kernel, z-streamed with the per-cell
  * (enable = 2) or the LDS-plane (enable = 3) source kernel, or per cell
  * (enable = 0; the update pass then uses the XCD panel) */

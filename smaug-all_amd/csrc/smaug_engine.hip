@@ -848,6 +848,7 @@ extern "C" int smaug_set_zstream(smaug_engine *e, int enable, int kz)
     // in LDS
     e->hyper_lds = enable == 2 ? 0 : (enable == 3 ? 1 : 2);
     e->kz = kz;
+    e->kzh = kz;
     drop_graphs(e);
     return 0;
 }
@@ -898,7 +899,7 @@ extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double 
         e->panel[pass][0] = bx;
         e->panel[pass][1] = by;
         if (pass == 0 && e->D == 3) {
-            static const int KZ[] = {8, 16, 32, 64};
+            static const int KZ[] = {8, 16, 32, 64, 128};
             int bkz = -1;
             e->ratio_zs = 1;
             for (int kz : KZ) {
@@ -917,7 +918,7 @@ extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double 
         if (pass == 1 && e->D == 3) {
             // z-streamed update with the z-streamed (hyper_lds 2) or the
             // LDS-plane (1) hyperdiffusive-source kernel
-            static const int KZ[] = {4, 8, 16, 32, 64};
+            static const int KZ[] = {8, 16, 32, 64, 128};
             int bkz = -1, bhy = 2;
             e->zstream = 1;
             for (int hy = 2; hy >= 1; hy--) {
