@@ -138,8 +138,8 @@ static TileMap make_map(int e0, int e1, int e2, int tpx, int tpy, unsigned *nblo
 static ZMap make_rmap(const smaug_engine *e, unsigned *nblocks)
 {
     ZMap m;
-    m.ntx = (e->n[0] - 3 + ZS_TI - 1) / ZS_TI;
-    m.nty = (e->n[1] - 3 + ZS_TJ - 1) / ZS_TJ;
+    m.ntx = (e->n[0] - 3 + RZ_TI - 1) / RZ_TI;
+    m.nty = (e->n[1] - 3 + RZ_TJ - 1) / RZ_TJ;
     const int nk = e->n[2] - 3;
     m.kz = e->kzr > 0 ? (e->kzr < nk ? e->kzr : nk) : (nk < 32 ? nk : 32);
     m.nkz = (nk + m.kz - 1) / m.kz;
@@ -156,10 +156,10 @@ static int launch_ratio(smaug_engine *e, const void *u)
             unsigned nb;
             ZMap m = make_rmap(e, &nb);
             if constexpr (sizeof(T) == 4)
-                hipLaunchKernelGGL(k_ratio_zs_f, dim3(nb), dim3(ZS_NT), 0, e->stream, e->geo, m,
+                hipLaunchKernelGGL(k_ratio_zs_f, dim3(nb), dim3(RZ_NT), 0, e->stream, e->geo, m,
                                    (const T *)u, (const T *)e->AUX, (T *)e->R);
             else
-                hipLaunchKernelGGL(k_ratio_zs_d, dim3(nb), dim3(ZS_NT), 0, e->stream, e->geo, m,
+                hipLaunchKernelGGL(k_ratio_zs_d, dim3(nb), dim3(RZ_NT), 0, e->stream, e->geo, m,
                                    (const T *)u, (const T *)e->AUX, (T *)e->R);
             HIPCHK(hipGetLastError());
             return 0;

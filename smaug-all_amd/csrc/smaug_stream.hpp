@@ -419,7 +419,7 @@ k_zstream_d(Geo g, Phys ph, StageArgs sa, ZMap zm, const double *__restrict__ ui
 }
 
 // ---------------------------------------------------------------------------
-// Hyperviscosity ratios, z-streamed.  A block owns 64 x 4 columns of the box
+// Hyperviscosity ratios, z-streamed.  A block owns 64 x 8 columns of the box
 // [1, n-3] and walks a chunk of its planes.  Each plane's window quantities
 // (rho', e_int, v, b') over the tile plus the window halo (2 cells below, 3
 // above, pads mirrored as in ratio_cell) are staged in LDS once; the ratios
@@ -429,9 +429,10 @@ k_zstream_d(Geo g, Phys ph, StageArgs sa, ZMap zm, const double *__restrict__ ui
 // issued into registers before the current plane is processed.  Same
 // arithmetic (ratio6) as k_ratio, so the two give identical ratios.
 // ---------------------------------------------------------------------------
-constexpr int RB_BI = ZS_TI + 5, RB_BJ = ZS_TJ + 5, RB_NB = RB_BI * RB_BJ;
+constexpr int RZ_TI = 64, RZ_TJ = 8, RZ_NT = RZ_TI * RZ_TJ; // columns (threads) per block
+constexpr int RB_BI = RZ_TI + 5, RB_BJ = RZ_TJ + 5, RB_NB = RB_BI * RB_BJ;
 constexpr int RB_NQ = 8;                           // rho', e_int, v_0..2, b'_0..2
-constexpr int RB_PER = (RB_NB + ZS_NT - 1) / ZS_NT; // box cells per thread
+constexpr int RB_PER = (RB_NB + RZ_NT - 1) / RZ_NT; // box cells per thread
 
 __device__ __forceinline__ int pad_map(int x, int n) { return x < 0 ? 5 : (x >= n ? n - 6 : x); }
 
@@ -448,8 +449,8 @@ __device__ __forceinline__ void ratio_zs_body(const Geo &g, const ZMap &zm,
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     const size_t nc = (size_t)g.nc;
     const ci_t s1 = (ci_t)g.s1, s2 = (ci_t)g.s2;
-    const int ti = threadIdx.x % ZS_TI, tj = threadIdx.x / ZS_TI;
-    const int i0 = 1 + tx * ZS_TI, j0 = 1 + ty * ZS_TJ;
+    const int ti = threadIdx.x % RZ_TI, tj = threadIdx.x / RZ_TI;
+    const int i0 = 1 + tx * RZ_TI, j0 = 1 + ty * RZ_TJ;
     const int i = i0 + ti, j = j0 + tj;
     const bool act = i <= n0 - 3 && j <= n1 - 3;
     const int k0 = 1 + kc * zm.kz;
@@ -460,7 +461,7 @@ __device__ __forceinline__ void ratio_zs_body(const Geo &g, const ZMap &zm,
     bool bv[RB_PER];
 #pragma unroll
     for (int q = 0; q < RB_PER; q++) {
-        const int b = threadIdx.x + q * ZS_NT;
+        const int b = threadIdx.x + q * RZ_NT;
         bv[q] = b < RB_NB;
         const int bi = b % RB_BI, bj = b / RB_BI;
         const int gi = pad_map(i0 - 2 + bi, n0), gj = pad_map(j0 - 2 + bj, n1);
@@ -495,7 +496,7 @@ __device__ __forceinline__ void ratio_zs_body(const Geo &g, const ZMap &zm,
 #pragma unroll
         for (int q = 0; q < RB_PER; q++)
             if (bv[q]) {
-                const int b = threadIdx.x + q * ZS_NT;
+                const int b = threadIdx.x + q * RZ_NT;
 #pragma unroll
                 for (int f = 0; f < RB_NQ; f++) sQ[f][b] = nx[q][f];
             }
@@ -561,7 +562,7 @@ __device__ __forceinline__ void ratio_zs_body(const Geo &g, const ZMap &zm,
     }
 }
 
-__global__ __launch_bounds__(ZS_NT) void k_ratio_zs_f(Geo g, ZMap zm, const float *__restrict__ u,
+__global__ __launch_bounds__(RZ_NT) void k_ratio_zs_f(Geo g, ZMap zm, const float *__restrict__ u,
                                                        const float *__restrict__ aux,
                                                        float *__restrict__ R)
 {
@@ -569,7 +570,7 @@ __global__ __launch_bounds__(ZS_NT) void k_ratio_zs_f(Geo g, ZMap zm, const floa
     ratio_zs_body<float>(g, zm, u, aux, R, sQ);
 }
 
-__global__ __launch_bounds__(ZS_NT) void k_ratio_zs_d(Geo g, ZMap zm, const double *__restrict__ u,
+__global__ __launch_bounds__(RZ_NT) void k_ratio_zs_d(Geo g, ZMap zm, const double *__restrict__ u,
                                                        const double *__restrict__ aux,
                                                        double *__restrict__ R)
 {
