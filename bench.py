@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256, help="interior cells per axis")
+    ap.add_argument("--large", action="store_true",
+                    help="the large stratified atmosphere: 768 x 512 x 512 interior cells over "
+                         "12 x 8 x 8 Mm (vertical first) instead of n^3 over 6 x 4 x 4 Mm")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
@@ -91,15 +94,17 @@ def run_path(setup, precision, warmup, steps, dt_scale=1.0, tune=True, shape=Non
     return tot_ms, stage_ms, w, info
 
 
-def profiled_traffic(kernel):
+def profiled_traffic(kernel, cells):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE passes scaled by the copy-kernel
-    calibration), or None"""
+    calibration) when it was taken on a grid of the same size, else None"""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
+        return None
+    if t.get("cells") != cells:
         return None
     k = t.get("kernels", {}).get(kernel)
     if not k:
@@ -190,7 +195,12 @@ def main():
     from smaugamd.engine import FP64, MIXED
 
     n = a.n
-    setup = configs.stratified(nz=n, nx=n, ny=n)
+    if a.large:
+        setup = configs.stratified(nz=768, nx=512, ny=512, lz=12.0e6, lx=8.0e6, ly=8.0e6)
+        shape_txt = "768x512x512 cells over 12x8x8 Mm"
+    else:
+        setup = configs.stratified(nz=n, nx=n, ny=n)
+        shape_txt = f"{n}x{n}x{n} cells over 6x4x4 Mm"
     cells = setup.ninterior
 
     print(f"[bench] set-up done: {setup.n} dt={setup.dt:.4e}", file=sys.stderr, flush=True)
@@ -213,7 +223,7 @@ def main():
             "speedup_mixed_over_fp64": (tot64 / tot),
         }
         n0, n1, n2 = setup.n
-        m = np.zeros((n2, n1, n0), dtype=bool)
+        m = np.zeros((n2, n1, n0), dtype=bool)  # interior cells
         m[2:-2, 2:-2, 2:-2] = True
         m = m.reshape(-1)
         names = ["rho", "m1", "m2", "m3", "e", "b1", "b2", "b3"]
@@ -250,7 +260,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": profiled_traffic(kname),
+        "traffic": profiled_traffic(kname, cells),
         "passes_ms": info["passes_ms"],
         "step_level_frac": (BYTES_PER_CELL_STEP["mixed"] * cells / (ms_step / 1e3) / 1e9)
         / HBM_PEAK_GBS,
@@ -277,9 +287,9 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32-perturbation/fp64-background (mixed)",
         "data": "synthetic",
-        "config": {"workload": f"3-D stratified solar atmosphere {n}x{n}x{n} cells over "
-                               f"6x4x4 Mm, RK4, dt={setup.dt:.4e} s (min of CFL 0.3 and "
-                               f"the hyperviscous limit)",
+        "config": {"workload": f"3-D stratified solar atmosphere {shape_txt}, RK4, "
+                               f"dt={setup.dt:.4e} s (min of CFL 0.3 and the hyperviscous "
+                               f"limit)",
                    "cells": cells, "precision": "mixed", "parallelism": "single GPU",
                    "launch_shapes": info.get("tuned")},
         "fp64": out_fp64,
