@@ -106,6 +106,11 @@ int smaug_set_ratio_form(smaug_engine *e, int zstreamed, int kz);
 /* the form of the 3-D update pass in use (enable as above) and its depth */
 int smaug_get_zstream(smaug_engine *e, int *enable, int *kz);
 
+/* conservation diagnostics: sums[f] = sum over the interior cells of evolved
+ * field f of the current state, accumulated in FP64 with compensated
+ * (Neumaier) summation on the device and over the per-block partials */
+int smaug_sums(smaug_engine *e, double *sums);
+
 /* one more RK4 step, with HIP events between its passes: ms[p] is the mean
  * duration over the four stages of pass p = 0 aux quantities and cmax,
  * 1 hyperviscosity ratios, 2 hyperdiffusive sources (3-D z-streamed form
@@ -139,6 +144,8 @@ long long smaug_ens_ninterior(const smaug_ensemble *e);
 /* state of one run, arrays as for smaug_set_state / smaug_get_state */
 int smaug_ens_set_state(smaug_ensemble *e, int run, const double *w, const double *bg);
 int smaug_ens_get_state(smaug_ensemble *e, int run, double *w);
+/* conservation diagnostics of one run, as smaug_sums */
+int smaug_ens_sums(smaug_ensemble *e, int run, double *sums);
 /* nsteps of classical RK4; t0 (may be NULL for all zero) and dt hold one
  * value per run */
 int smaug_ens_rk4(smaug_ensemble *e, const double *t0, const double *dt, int nsteps);

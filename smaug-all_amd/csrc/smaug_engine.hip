@@ -79,6 +79,7 @@ struct smaug_engine {
     void *R;      // hyperviscosity ratios
     void *AUX;    // per-cell 1/rho, pressure, internal energy, velocity
     void *HYP;    // 3-D: hyperdiffusive part of L(w) for the z-streamed update
+    double *d_part; // per-block partial sums of the conservation diagnostics
     double *bg;
     double *d_dx[3], *d_idx[3], *d_prof[3];
     unsigned long long *d_cmax;
@@ -416,6 +417,7 @@ static void free_engine(smaug_engine *e)
     if (e->R) hipFree(e->R);
     if (e->AUX) hipFree(e->AUX);
     if (e->HYP) hipFree(e->HYP);
+    if (e->d_part) hipFree(e->d_part);
     if (e->bg) hipFree(e->bg);
     if (e->d_cmax) hipFree(e->d_cmax);
     if (e->d_tm) hipFree(e->d_tm);
@@ -971,6 +973,52 @@ extern "C" int smaug_get_zstream(smaug_engine *e, int *enable, int *kz)
     unsigned nb;
     *kz = e->D == 3 ? make_zmap(e, &nb).kz : 0;
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// conservation diagnostics
+// ---------------------------------------------------------------------------
+int smaug::field_sums(int D, int prec, const Geo &g, const void *u, hipStream_t st, double *part,
+                      double *sums)
+{
+    const int nve = 2 + 2 * D, nb = sums_blocks();
+    if (D == 2) {
+        if (prec == SMAUG_FP64)
+            hipLaunchKernelGGL((k_sums<2, double>), dim3(nb), dim3(256), 0, st, g, (const double *)u, part);
+        else
+            hipLaunchKernelGGL((k_sums<2, float>), dim3(nb), dim3(256), 0, st, g, (const float *)u, part);
+    } else {
+        if (prec == SMAUG_FP64)
+            hipLaunchKernelGGL((k_sums<3, double>), dim3(nb), dim3(256), 0, st, g, (const double *)u, part);
+        else
+            hipLaunchKernelGGL((k_sums<3, float>), dim3(nb), dim3(256), 0, st, g, (const float *)u, part);
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<double> h((size_t)2 * nve * nb);
+    HIPCHK(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int f = 0; f < nve; f++) {
+        // Neumaier over the per-block (sum, compensation) pairs
+        double s = 0, c = 0;
+        auto add = [&](double x) {
+            const double t = s + x;
+            c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+            s = t;
+        };
+        for (int b = 0; b < nb; b++) {
+            add(h[(size_t)b * 2 * nve + 2 * f]);
+            c += h[(size_t)b * 2 * nve + 2 * f + 1];
+        }
+        sums[f] = s + c;
+    }
+    return 0;
+}
+
+extern "C" int smaug_sums(smaug_engine *e, double *sums)
+{
+    if (!e || !sums) return fail(SMAUG_EINVAL, "null argument");
+    if (!e->d_part) HIPCHK(hipMalloc((void **)&e->d_part, sizeof(double) * 2 * 8 * sums_blocks()));
+    return field_sums(e->D, e->prec, e->geo, e->U[0], e->stream, e->d_part, sums);
 }
 
 // ---------------------------------------------------------------------------

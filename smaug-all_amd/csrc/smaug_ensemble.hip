@@ -213,6 +213,7 @@ struct smaug_ensemble {
     int *base[NPASS];              // per-pass item prefix sums (device)
     int nitems[NPASS];
     unsigned grid[NPASS];          // persistent grid per pass
+    double *d_part;                // conservation-diagnostic partial sums
     Phys phys;
     hipGraphExec_t gexec;
     bool gvalid;
@@ -222,7 +223,8 @@ static void free_ens(smaug_ensemble *e)
 {
     if (!e) return;
     if (e->gvalid) hipGraphExecDestroy(e->gexec);
-    void *ptrs[] = {e->state, e->ratio, e->aux, e->bg, e->axes, e->tm, e->cm, e->ctr, e->descs};
+    void *ptrs[] = {e->state, e->ratio, e->aux, e->bg, e->axes, e->tm, e->cm, e->ctr, e->descs,
+                    e->d_part};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (int k = 0; k < NPASS; k++)
@@ -348,6 +350,7 @@ extern "C" int smaug_ens_create(const smaug_cfg *cfgs, int nrun, smaug_ensemble 
     e->descs = nullptr;
     for (int k = 0; k < NPASS; k++) e->base[k] = nullptr;
     e->stream = nullptr;
+    e->d_part = nullptr;
     e->gvalid = false;
 #define ECHK(x)                                                                            \
     do {                                                                                   \
@@ -495,6 +498,14 @@ extern "C" int smaug_ens_get_state(smaug_ensemble *e, int run, double *w)
         for (idx_t i = 0; i < n; i++) w[i] = b[i];
     }
     return 0;
+}
+
+extern "C" int smaug_ens_sums(smaug_ensemble *e, int run, double *sums)
+{
+    if (!e || !sums || run < 0 || run >= e->nrun) return fail(SMAUG_EINVAL, "bad argument");
+    if (!e->d_part) HIPCHK(hipMalloc((void **)&e->d_part, sizeof(double) * 2 * 8 * sums_blocks()));
+    return field_sums(e->D, e->prec, e->geo[run], e->state + e->off_u[run], e->stream, e->d_part,
+                      sums);
 }
 
 static Queue queue(smaug_ensemble *e, int pass, int launch)

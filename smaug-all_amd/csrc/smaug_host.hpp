@@ -2,6 +2,7 @@
 // the ensemble: error text, HIP status checks, per-axis metric arrays.
 #pragma once
 #include "../../include/smaug_engine.h"
+#include "smaug_phys.hpp"
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -23,6 +24,13 @@ int check_cfg(const smaug_cfg *cfg);
 // Gaussian footprint (usersource.cu:4-92).
 void axis_arrays(const smaug_cfg *cfg, int d, int nd, std::vector<double> &dx,
                  std::vector<double> &idx, std::vector<double> &prof);
+
+// compensated FP64 sums of the evolved fields of u over the interior cells
+// of grid g (k_sums plus a compensated host pass), nve values into sums;
+// part is device scratch of at least 2 * nve * sums_blocks() doubles
+int field_sums(int D, int prec, const Geo &g, const void *u, hipStream_t st, double *part,
+               double *sums);
+constexpr int sums_blocks() { return 512; }
 
 } // namespace smaug
 

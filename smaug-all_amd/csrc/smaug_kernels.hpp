@@ -268,6 +268,79 @@ __global__ __launch_bounds__(256) void k_bc(Geo g, T *__restrict__ u)
 }
 
 // ---------------------------------------------------------------------------
+// conservation diagnostics: compensated FP64 sums of every evolved field over
+// the interior cells.  Each thread accumulates its cells with Neumaier's
+// compensated summation; the block combines the per-thread (sum,
+// compensation) pairs with error-free two-sums and writes one pair per field;
+// the host adds the per-block pairs, again compensated.
+// ---------------------------------------------------------------------------
+struct CSum {
+    double s, c;
+};
+
+__device__ __forceinline__ void csum_add(CSum &a, double x)
+{
+    const double t = a.s + x;
+    if (fabs(a.s) >= fabs(x))
+        a.c += (a.s - t) + x;
+    else
+        a.c += (x - t) + a.s;
+    a.s = t;
+}
+
+__device__ __forceinline__ void csum_merge(CSum &a, const CSum &b)
+{
+    csum_add(a, b.s);
+    a.c += b.c;
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_sums(Geo g, const T *__restrict__ u, double *part)
+{
+    typedef Lay<D> LY;
+    __shared__ double red[2][LY::NVE][256];
+    const int e0 = g.n[0] - 4, e1 = g.n[1] - 4, e2 = (D == 3) ? g.n[2] - 4 : 1;
+    const long long ni = (long long)e0 * e1 * e2;
+    const size_t nc = (size_t)g.nc;
+    CSum acc[LY::NVE];
+#pragma unroll
+    for (int f = 0; f < LY::NVE; f++) acc[f].s = acc[f].c = 0.0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ni;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % e0);
+        const long long r = t / e0;
+        const int j = (int)(r % e1), k = (int)(r / e1);
+        const ci_t c = (ci_t)(2 + i) + (ci_t)g.s1 * (ci_t)(2 + j) +
+                       (D == 3 ? (ci_t)g.s2 * (ci_t)(2 + k) : 0u);
+#pragma unroll
+        for (int f = 0; f < LY::NVE; f++) csum_add(acc[f], (double)ld(u + f * nc, c));
+    }
+#pragma unroll
+    for (int f = 0; f < LY::NVE; f++) {
+        red[0][f][threadIdx.x] = acc[f].s;
+        red[1][f][threadIdx.x] = acc[f].c;
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int f = 0; f < LY::NVE; f++) {
+                CSum a = {red[0][f][threadIdx.x], red[1][f][threadIdx.x]};
+                const CSum b = {red[0][f][threadIdx.x + w], red[1][f][threadIdx.x + w]};
+                csum_merge(a, b);
+                red[0][f][threadIdx.x] = a.s;
+                red[1][f][threadIdx.x] = a.c;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < LY::NVE) {
+        part[(size_t)blockIdx.x * 2 * LY::NVE + 2 * threadIdx.x] = red[0][threadIdx.x][0];
+        part[(size_t)blockIdx.x * 2 * LY::NVE + 2 * threadIdx.x + 1] = red[1][threadIdx.x][0];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // stage update
 // ---------------------------------------------------------------------------
 struct StageArgs {

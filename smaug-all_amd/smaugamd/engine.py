@@ -26,6 +26,7 @@ EXPORTS = [
     "smaug_time_rk4", "smaug_ncell", "smaug_ninterior", "smaug_copy_probe",
     "smaug_set_panel", "smaug_get_panel", "smaug_time_pass", "smaug_autotune",
     "smaug_set_zstream", "smaug_get_zstream", "smaug_time_passes", "smaug_set_ratio_form",
+    "smaug_sums", "smaug_ens_sums",
     "smaug_ens_create", "smaug_ens_destroy", "smaug_ens_nrun", "smaug_ens_ninterior",
     "smaug_ens_set_state", "smaug_ens_get_state", "smaug_ens_rk4", "smaug_ens_time_rk4",
 ]
@@ -88,6 +89,8 @@ def load_library(path: str = LIB_PATH):
         lib.smaug_set_zstream.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         lib.smaug_get_zstream.argtypes = [vp, ip, ip]
         lib.smaug_set_ratio_form.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        lib.smaug_sums.argtypes = [vp, dp]
+        lib.smaug_ens_sums.argtypes = [vp, ctypes.c_int, dp]
         lib.smaug_time_passes.argtypes = [vp, ctypes.c_double, ctypes.c_double, dp]
         lib.smaug_ens_create.argtypes = [ctypes.POINTER(SmaugCfg), ctypes.c_int,
                                          ctypes.POINTER(vp)]
@@ -211,6 +214,12 @@ class Engine:
         """3-D: z-streamed stage update (default) or the per-cell one"""
         self._check(self.lib.smaug_set_zstream(self.h, int(enable), kz))
 
+    def sums(self):
+        """compensated FP64 sums of the evolved fields over the interior"""
+        out = np.zeros(self.setup.nve)
+        self._check(self.lib.smaug_sums(self.h, _dptr(out)))
+        return out
+
     PASSES = ("aux+cmax", "ratio", "hyperdiffusive sources", "stage update", "ghost fill")
 
     def time_passes(self, t0, dt):
@@ -294,6 +303,12 @@ class Ensemble:
         w = np.empty((s.nve, s.ncell))
         self._check(self.lib.smaug_ens_get_state(self.h, run, _dptr(w)))
         return w
+
+    def sums(self, run):
+        """compensated FP64 sums of run's evolved fields over its interior"""
+        out = np.zeros(self.setups[run].nve)
+        self._check(self.lib.smaug_ens_sums(self.h, run, _dptr(out)))
+        return out
 
     def _times(self, t0, dt):
         n = len(self.setups)
