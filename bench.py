@@ -14,6 +14,8 @@ Printed JSON line (rank 0): the contract fields plus
                 field's amplitude)
   roofline    : the stage-update kernel against HBM peak
   cpu_baseline: the CPU oracle (OpenMP) on a bounded sample of the workload
+  orszag_tang : the 1024^2 Orszag-Tang vortex, 500 RK4 steps at dt = 1e-4,
+                mixed and FP64 paths and the mixed error against FP64
   ensemble    : the batched 2-D ensemble (64 Orszag-Tang-type periodic runs,
                 sides drawn from {96,...,512}, 200 RK4 steps, mixed precision)
                 advanced by the persistent queue-driven kernels, and the same
@@ -56,6 +58,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-ot", action="store_true")
     ap.add_argument("--shape", default=None,
                     help="fixed 3-D update form and z-stream depth 'FORM,KZ' instead of the "
                          "autotuner (FORM as in smaug_set_zstream: 0 per-cell, 1 z-streamed, "
@@ -196,6 +199,42 @@ def ensemble_leg(steps, warmup, seq_steps=20):
     }
 
 
+def orszag_tang_leg(nsteps=500):
+    """the Orszag-Tang vortex, 1024^2 periodic, fixed dt = 1e-4, mixed and FP64
+    paths: Mcell-updates/s of each and the per-field error of the mixed
+    result against the FP64 one after nsteps"""
+    import numpy as np
+
+    from smaugamd import configs
+    from smaugamd.engine import FP64, MIXED, Engine
+
+    s = configs.orszag_tang(nx=1024)
+    out, res = {}, {}
+    for prec, name in ((MIXED, "mixed"), (FP64, "fp64")):
+        e = Engine(s, prec)
+        e.set_state()
+        e.autotune(3)
+        e.rk4(0.0, s.dt, 2)
+        e.set_state()
+        tot, _ = e.time_rk4(0.0, s.dt, nsteps)
+        res[name] = e.get_state()
+        e.close()
+        out[name] = {"ms_per_step": tot / nsteps, "value": s.ninterior * nsteps / (tot / 1e3) / 1e6}
+    m = np.zeros((s.n[1], s.n[0]), dtype=bool)
+    m[2:-2, 2:-2] = True
+    m = m.reshape(-1)
+    names = ["rho", "m1", "m2", "e", "b1", "b2"]
+    l2 = {}
+    for f in range(s.nve):
+        ref = res["fp64"][f][m]
+        rms = np.sqrt(np.mean(ref ** 2))
+        l2[names[f]] = float(np.sqrt(np.mean((res["mixed"][f][m] - ref) ** 2)) / rms)
+    return {"workload": "Orszag-Tang vortex 1024x1024 periodic on [0,2pi]^2, dt = 1e-4",
+            "steps": nsteps + 1, "unit": "Mcell-updates/s", "mixed": out["mixed"],
+            "fp64": out["fp64"], "speedup_mixed_over_fp64": out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"],
+            "mixed_rel_l2_vs_fp64": l2}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -285,6 +324,11 @@ def main():
     if not a.no_ensemble:
         ens = ensemble_leg(a.ens_steps, a.warmup)
         print(f"[bench] ensemble: {ens['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    ot = None
+    if not a.no_ot:
+        ot = orszag_tang_leg()
+        print(f"[bench] orszag-tang: {ot['mixed']['ms_per_step']:.3f} ms/step", file=sys.stderr,
+              flush=True)
     cpu = None if a.no_cpu else cpu_baseline(n)
     rec = {
         "metric": "Mcell-updates/s per RK4 step, mixed vs FP64 at stated L2 error; % HBM peak",
@@ -309,6 +353,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "ensemble": ens,
+        "orszag_tang": ot,
     }
     print(json.dumps(rec))
 
