@@ -156,6 +156,7 @@ def ensemble_leg(steps, warmup, seq_steps=20):
     s0 = [ens.sums(r) for r in range(len(setups))]
     ens.rk4(0.0, dts, warmup)
     ms = ens.time_rk4([warmup * d for d in dts], dts, steps)
+    passes = ens.time_passes([(warmup + steps) * d for d in dts], dts)
     finite = all(np.isfinite(ens.get_state(r)).all() for r in range(len(setups)))
     # drift of the compensated FP64 sums relative to the initial sum of
     # |field| (mass rho, energy e); the hyperviscosity windows are padded at
@@ -193,6 +194,7 @@ def ensemble_leg(steps, warmup, seq_steps=20):
         "hbm_gbs_algorithmic": gbs,
         "hbm_frac": gbs / HBM_PEAK_GBS,
         "finite": bool(finite),
+        "passes_ms": passes,
         "conservation_max_rel_drift": drift,
         "one_grid_at_a_time_ms_per_step": seq_ms,
         "speedup_over_one_grid_at_a_time": seq_ms / ms_step,

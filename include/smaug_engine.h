@@ -149,6 +149,10 @@ int smaug_ens_sums(smaug_ensemble *e, int run, double *sums);
 /* nsteps of classical RK4; t0 (may be NULL for all zero) and dt hold one
  * value per run */
 int smaug_ens_rk4(smaug_ensemble *e, const double *t0, const double *dt, int nsteps);
+/* one more RK4 step of every run with HIP events between the passes: ms[p]
+ * is the mean duration over the four stages of pass p = 0 aux and cmax,
+ * 1 ratios, 2 stage update, 3 ghost fill */
+int smaug_ens_time_passes(smaug_ensemble *e, const double *t0, const double *dt, double ms[4]);
 /* as smaug_ens_rk4, and the milliseconds of the step loop (HIP events) */
 int smaug_ens_time_rk4(smaug_ensemble *e, const double *t0, const double *dt, int nsteps,
                        double *total_ms);

@@ -2,9 +2,9 @@
 // together by persistent, queue-driven kernels.
 //
 // Every pass of an RK stage (prep, ratio, update, ghost fill) is one
-// persistent launch over the whole ensemble: its work items are the
-// (run, tile) pairs of all runs, numbered run by run, and the blocks pull
-// them from a device queue.  The item range is cut into 8 contiguous
+// persistent launch over the whole ensemble: its work items are runs of
+// consecutive cells of every grid, numbered grid by grid, and the blocks
+// pull them from a device queue.  The item range is cut into 8 contiguous
 // slices, one per XCD; a block first drains the slice of its own XCD
 // (blocks are dispatched to XCDs round-robin, so block b runs on XCD b % 8)
 // and then steals from the others.  Neighbouring tiles of one run are thus
@@ -28,6 +28,8 @@ using namespace smaug;
 namespace {
 
 constexpr int PREP_CELLS = 1024; // cells per prep item (4 per thread)
+constexpr int BOX_CELLS = 512;   // ratio / update cells per item (2 per thread)
+constexpr int BC_CELLS = 1024;   // ghost-slab elements per item (4 per thread)
 constexpr int NPASS = 4;         // prep, ratio, update, ghost fill
 constexpr int NQ = 8;            // queue slices, one per XCD
 
@@ -39,11 +41,12 @@ struct RunDesc {
     const double *bg;
     unsigned long long *cm; // [4 stages][4] cmax bit patterns
     double *tm;             // [t, dt]
-    int ntx[2], nty[2];     // 64 x 4 tiles of the ratio box (0) and the interior (1)
+    int ext[2][3];          // extents of the ratio box [1, n-3] (0) and the interior (1)
 };
 
 struct Queue {
     const int *base;   // nrun + 1 prefix sums of the items of each run
+    const int *run;    // the run of every item
     int nrun, nitems;
     unsigned *ctr;     // NQ slice counters
 };
@@ -63,29 +66,44 @@ __device__ __forceinline__ int pull(const Queue &q, int xcd, int &v)
     return -1;
 }
 
-// run that owns item it: the largest r with base[r] <= it
-__device__ __forceinline__ int run_of(const int *__restrict__ base, int nrun, int it)
+// The block-uniform next item and its run.  Thread 0 keeps one item (and
+// the load of its run) pulled ahead, so the latency of the queue atomic and
+// of the run lookup overlaps the processing of the current item; s_it[1]
+// starts at -2 ("nothing pulled yet").  Two barriers so that s_it is not
+// rewritten before every thread has read it.
+__device__ __forceinline__ int block_pull(const Queue &q, int &v, int *s_it, int &run)
 {
-    int lo = 0, hi = nrun;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (base[mid] <= it)
-            lo = mid;
-        else
-            hi = mid;
+    if (threadIdx.x == 0) {
+        const int xcd = blockIdx.x & (NQ - 1);
+        int cur = s_it[1], cr = s_it[3];
+        if (cur == -2) {
+            cur = pull(q, xcd, v);
+            cr = cur >= 0 ? q.run[cur] : 0;
+        }
+        s_it[0] = cur;
+        s_it[2] = cr;
+        const int nx = cur >= 0 ? pull(q, xcd, v) : -1;
+        s_it[1] = nx;
+        s_it[3] = nx >= 0 ? q.run[nx] : 0;
     }
-    return lo;
-}
-
-// the block-uniform next item; two barriers so that s_it is not rewritten
-// before every thread has read it
-__device__ __forceinline__ int block_pull(const Queue &q, int &v, int *s_it)
-{
-    if (threadIdx.x == 0) *s_it = pull(q, blockIdx.x & (NQ - 1), v);
     __syncthreads();
-    const int it = __builtin_amdgcn_readfirstlane(*s_it);
+    const int it = __builtin_amdgcn_readfirstlane(s_it[0]);
+    run = __builtin_amdgcn_readfirstlane(s_it[2]);
     __syncthreads();
     return it;
+}
+
+// cell t (linear, i fastest) of a box of extents ext starting at lo; a run
+// has fewer than 2^31 cells, so 32-bit division suffices
+__device__ __forceinline__ void box_cell(unsigned t, const int *ext, int lo, int &x0, int &x1,
+                                         int &x2)
+{
+    const unsigned e0 = (unsigned)ext[0], e1 = (unsigned)ext[1];
+    const unsigned r = t / e0;
+    x0 = lo + (int)(t - r * e0);
+    const unsigned k = r / e1;
+    x1 = lo + (int)(r - k * e1);
+    x2 = lo + (int)k;
 }
 
 __global__ void ke_reset(unsigned *ctr, int nctr, unsigned long long *cm, int ncm)
@@ -109,13 +127,14 @@ template <int D, typename T>
 __global__ __launch_bounds__(256) void ke_prep(const RunDesc<T> *__restrict__ runs, Queue q,
                                                 Phys ph, int stage, int src)
 {
-    __shared__ int s_it;
+    __shared__ int s_it[4];
     __shared__ double red[3][4];
     int v = 0;
+    if (threadIdx.x == 0) s_it[1] = -2;
     for (;;) {
-        const int it = block_pull(q, v, &s_it);
+        int r;
+        const int it = block_pull(q, v, s_it, r);
         if (it < 0) break;
-        const int r = run_of(q.base, q.nrun, it);
         const RunDesc<T> &rd = runs[r];
         const idx_t c0 = (idx_t)(it - q.base[r]) * PREP_CELLS;
         double mx[3] = {0.0, 0.0, 0.0};
@@ -132,21 +151,23 @@ template <int D, typename T>
 __global__ __launch_bounds__(256) void ke_ratio(const RunDesc<T> *__restrict__ runs, Queue q,
                                                  int src)
 {
-    __shared__ int s_it;
+    __shared__ int s_it[4];
     int v = 0;
+    if (threadIdx.x == 0) s_it[1] = -2;
     for (;;) {
-        const int it = block_pull(q, v, &s_it);
+        int r;
+        const int it = block_pull(q, v, s_it, r);
         if (it < 0) break;
-        const int r = run_of(q.base, q.nrun, it);
         const RunDesc<T> &rd = runs[r];
-        const int l = it - q.base[r];
-        const int tx = l % rd.ntx[0], rest = l / rd.ntx[0];
-        const int ty = rest % rd.nty[0], tk = rest / rd.nty[0];
-        const int x0 = 1 + tx * 64 + (threadIdx.x & 63);
-        const int x1 = 1 + ty * 4 + (threadIdx.x >> 6);
-        const int x2 = (D == 3) ? 1 + tk : 0;
-        if (x0 <= rd.geo.n[0] - 3 && x1 <= rd.geo.n[1] - 3)
-            ratio_cell<D, T>(rd.geo, rd.U[src], rd.AUX, rd.R, x0, x1, x2);
+        const unsigned nb = (unsigned)rd.ext[0][0] * rd.ext[0][1] * rd.ext[0][2];
+#pragma unroll
+        for (int k = 0; k < BOX_CELLS / 256; k++) {
+            const unsigned t = (unsigned)(it - q.base[r]) * BOX_CELLS + k * 256 + threadIdx.x;
+            if (t >= nb) break;
+            int x0, x1, x2;
+            box_cell(t, rd.ext[0], 1, x0, x1, x2);
+            ratio_cell<D, T>(rd.geo, rd.U[src], rd.AUX, rd.R, x0, x1, D == 3 ? x2 : 0);
+        }
     }
 }
 
@@ -154,20 +175,22 @@ template <int D, typename T>
 __global__ __launch_bounds__(256) void ke_update(const RunDesc<T> *__restrict__ runs, Queue q,
                                                   Phys ph, int stage, double ct, int src, int dst)
 {
-    __shared__ int s_it;
+    __shared__ int s_it[4];
     int v = 0;
+    if (threadIdx.x == 0) s_it[1] = -2;
     for (;;) {
-        const int it = block_pull(q, v, &s_it);
+        int r;
+        const int it = block_pull(q, v, s_it, r);
         if (it < 0) break;
-        const int r = run_of(q.base, q.nrun, it);
         const RunDesc<T> &rd = runs[r];
-        const int l = it - q.base[r];
-        const int tx = l % rd.ntx[1], rest = l / rd.ntx[1];
-        const int ty = rest % rd.nty[1], tk = rest / rd.nty[1];
-        const int x0 = 2 + tx * 64 + (threadIdx.x & 63);
-        const int x1 = 2 + ty * 4 + (threadIdx.x >> 6);
-        const int x2 = (D == 3) ? 2 + tk : 0;
-        if (x0 <= rd.geo.n[0] - 3 && x1 <= rd.geo.n[1] - 3) {
+        const unsigned nb = (unsigned)rd.ext[1][0] * rd.ext[1][1] * rd.ext[1][2];
+#pragma unroll
+        for (int k = 0; k < BOX_CELLS / 256; k++) {
+            const unsigned t = (unsigned)(it - q.base[r]) * BOX_CELLS + k * 256 + threadIdx.x;
+            if (t >= nb) break;
+            int x0, x1, x2;
+            box_cell(t, rd.ext[1], 2, x0, x1, x2);
+            if (D != 3) x2 = 0;
             StageArgs sa;
             sa.mode = 0;
             sa.stage = stage;
@@ -182,15 +205,20 @@ __global__ __launch_bounds__(256) void ke_update(const RunDesc<T> *__restrict__ 
 template <int D, typename T>
 __global__ __launch_bounds__(256) void ke_bc(const RunDesc<T> *__restrict__ runs, Queue q, int dst)
 {
-    __shared__ int s_it;
+    __shared__ int s_it[4];
     int v = 0;
+    if (threadIdx.x == 0) s_it[1] = -2;
     for (;;) {
-        const int it = block_pull(q, v, &s_it);
+        int r;
+        const int it = block_pull(q, v, s_it, r);
         if (it < 0) break;
-        const int r = run_of(q.base, q.nrun, it);
         const RunDesc<T> &rd = runs[r];
-        const idx_t t = (idx_t)(it - q.base[r]) * 256 + threadIdx.x;
-        if (t < bc_count<D>(rd.geo.n[0], rd.geo.n[1], rd.geo.n[2])) bc_elem<D, T>(rd.geo, rd.U[dst], t);
+        const idx_t nbc = bc_count<D>(rd.geo.n[0], rd.geo.n[1], rd.geo.n[2]);
+#pragma unroll
+        for (int k = 0; k < BC_CELLS / 256; k++) {
+            const idx_t t = (idx_t)(it - q.base[r]) * BC_CELLS + k * 256 + threadIdx.x;
+            if (t < nbc) bc_elem<D, T>(rd.geo, rd.U[dst], t);
+        }
     }
 }
 
@@ -211,6 +239,7 @@ struct smaug_ensemble {
     unsigned *ctr;                 // 16 launches x NQ
     void *descs;                   // RunDesc<T>[nrun]
     int *base[NPASS];              // per-pass item prefix sums (device)
+    int *irun[NPASS];              // per-pass run of every item (device)
     int nitems[NPASS];
     unsigned grid[NPASS];          // persistent grid per pass
     double *d_part;                // conservation-diagnostic partial sums
@@ -227,8 +256,10 @@ static void free_ens(smaug_ensemble *e)
                     e->d_part};
     for (void *p : ptrs)
         if (p) hipFree(p);
-    for (int k = 0; k < NPASS; k++)
+    for (int k = 0; k < NPASS; k++) {
         if (e->base[k]) hipFree(e->base[k]);
+        if (e->irun[k]) hipFree(e->irun[k]);
+    }
     if (e->stream) hipStreamDestroy(e->stream);
     delete e;
 }
@@ -284,16 +315,18 @@ static int setup_typed(smaug_ensemble *e)
         rd.tm = e->tm + 2 * r;
         const int n2 = D == 3 ? g.n[2] : 1;
         // ratio box [1, n-3], interior [2, n-3]
-        rd.ntx[0] = (g.n[0] - 3 + 63) / 64;
-        rd.nty[0] = (g.n[1] - 3 + 3) / 4;
-        rd.ntx[1] = (g.n[0] - 4 + 63) / 64;
-        rd.nty[1] = (g.n[1] - 4 + 3) / 4;
-        const int nk0 = D == 3 ? n2 - 3 : 1, nk1 = D == 3 ? n2 - 4 : 1;
+        for (int d = 0; d < 3; d++) {
+            rd.ext[0][d] = d < D ? g.n[d] - 3 : 1;
+            rd.ext[1][d] = d < D ? g.n[d] - 4 : 1;
+        }
+        const long long nbox[2] = {(long long)rd.ext[0][0] * rd.ext[0][1] * rd.ext[0][2],
+                                   (long long)rd.ext[1][0] * rd.ext[1][1] * rd.ext[1][2]};
+        if (g.nc >= (1ll << 31)) return fail(SMAUG_EINVAL, "run %d: too many cells for a member", r);
         long long items[NPASS] = {
             (long long)((g.nc + PREP_CELLS - 1) / PREP_CELLS),
-            e->phys.hyperdif ? (long long)rd.ntx[0] * rd.nty[0] * nk0 : 0,
-            (long long)rd.ntx[1] * rd.nty[1] * nk1,
-            (long long)((bc_count<D>(g.n[0], g.n[1], n2) + 255) / 256)};
+            e->phys.hyperdif ? (nbox[0] + BOX_CELLS - 1) / BOX_CELLS : 0,
+            (nbox[1] + BOX_CELLS - 1) / BOX_CELLS,
+            (long long)((bc_count<D>(g.n[0], g.n[1], n2) + BC_CELLS - 1) / BC_CELLS)};
         for (int k = 0; k < NPASS; k++) {
             long long nb = (long long)base[k][r] + items[k];
             if (nb > 0x7fffffffLL) return fail(SMAUG_EINVAL, "ensemble too large for the work queue");
@@ -307,12 +340,22 @@ static int setup_typed(smaug_ensemble *e)
         HIPCHK(hipMalloc((void **)&e->base[k], sizeof(int) * (e->nrun + 1)));
         HIPCHK(hipMemcpy(e->base[k], base[k].data(), sizeof(int) * (e->nrun + 1),
                          hipMemcpyHostToDevice));
+        std::vector<int> ir(e->nitems[k] > 0 ? e->nitems[k] : 1, 0);
+        for (int r = 0; r < e->nrun; r++)
+            for (int it = base[k][r]; it < base[k][r + 1]; it++) ir[it] = r;
+        HIPCHK(hipMalloc((void **)&e->irun[k], sizeof(int) * ir.size()));
+        HIPCHK(hipMemcpy(e->irun[k], ir.data(), sizeof(int) * ir.size(), hipMemcpyHostToDevice));
     }
     int rc;
     if ((rc = pass_grid(ke_prep<D, T>, &e->grid[0]))) return rc;
     if ((rc = pass_grid(ke_ratio<D, T>, &e->grid[1]))) return rc;
     if ((rc = pass_grid(ke_update<D, T>, &e->grid[2]))) return rc;
     if ((rc = pass_grid(ke_bc<D, T>, &e->grid[3]))) return rc;
+    // no more blocks than items (a multiple of 8 so every XCD slice has blocks)
+    for (int k = 0; k < NPASS; k++) {
+        const unsigned need = ((unsigned)e->nitems[k] + 7) & ~7u;
+        if (need > 0 && need < e->grid[k]) e->grid[k] = need;
+    }
     return 0;
 }
 
@@ -348,7 +391,7 @@ extern "C" int smaug_ens_create(const smaug_cfg *cfgs, int nrun, smaug_ensemble 
     e->cm = nullptr;
     e->ctr = nullptr;
     e->descs = nullptr;
-    for (int k = 0; k < NPASS; k++) e->base[k] = nullptr;
+    for (int k = 0; k < NPASS; k++) e->base[k] = e->irun[k] = nullptr;
     e->stream = nullptr;
     e->d_part = nullptr;
     e->gvalid = false;
@@ -512,6 +555,7 @@ static Queue queue(smaug_ensemble *e, int pass, int launch)
 {
     Queue q;
     q.base = e->base[pass];
+    q.run = e->irun[pass];
     q.nrun = e->nrun;
     q.nitems = e->nitems[pass];
     q.ctr = e->ctr + (size_t)launch * NQ;
@@ -519,8 +563,9 @@ static Queue queue(smaug_ensemble *e, int pass, int launch)
 }
 
 // one RK4 step of every run (same slot algebra as the single-grid engine)
+// evk (optional): 5 events per stage, recorded between its passes
 template <int D, typename T>
-static int ens_step(smaug_ensemble *e)
+static int ens_step(smaug_ensemble *e, const hipEvent_t *evk = nullptr)
 {
     const double ct[4] = {0.0, 0.5, 0.5, 1.0};
     const int in[4] = {0, 1, 2, 1}, out[4] = {1, 2, 1, 0};
@@ -528,15 +573,20 @@ static int ens_step(smaug_ensemble *e)
     hipLaunchKernelGGL(ke_reset, dim3(8), dim3(256), 0, e->stream, e->ctr, 4 * NPASS * NQ, e->cm,
                        16 * e->nrun);
     for (int s = 0; s < 4; s++) {
+        if (evk) HIPCHK(hipEventRecord(evk[5 * s + 0], e->stream));
         hipLaunchKernelGGL((ke_prep<D, T>), dim3(e->grid[0]), dim3(256), 0, e->stream, rd,
                            queue(e, 0, 4 * s + 0), e->phys, s, in[s]);
+        if (evk) HIPCHK(hipEventRecord(evk[5 * s + 1], e->stream));
         if (e->phys.hyperdif)
             hipLaunchKernelGGL((ke_ratio<D, T>), dim3(e->grid[1]), dim3(256), 0, e->stream, rd,
                                queue(e, 1, 4 * s + 1), in[s]);
+        if (evk) HIPCHK(hipEventRecord(evk[5 * s + 2], e->stream));
         hipLaunchKernelGGL((ke_update<D, T>), dim3(e->grid[2]), dim3(256), 0, e->stream, rd,
                            queue(e, 2, 4 * s + 2), e->phys, s, ct[s], in[s], out[s]);
+        if (evk) HIPCHK(hipEventRecord(evk[5 * s + 3], e->stream));
         hipLaunchKernelGGL((ke_bc<D, T>), dim3(e->grid[3]), dim3(256), 0, e->stream, rd,
                            queue(e, 3, 4 * s + 3), out[s]);
+        if (evk) HIPCHK(hipEventRecord(evk[5 * s + 4], e->stream));
     }
     hipLaunchKernelGGL(ke_advance_time, dim3((e->nrun + 63) / 64), dim3(64), 0, e->stream, e->tm,
                        e->nrun);
@@ -604,6 +654,35 @@ extern "C" int smaug_ens_rk4(smaug_ensemble *e, const double *t0, const double *
 {
     if (!e || !dt || nsteps < 0) return fail(SMAUG_EINVAL, "bad argument");
     return EDISPATCH(e, ens_run, e, t0, dt, nsteps, (double *)nullptr);
+}
+
+template <int D, typename T>
+static int ens_time_passes(smaug_ensemble *e, const double *t0, const double *dt, double *ms)
+{
+    int rc = ens_set_time(e, t0, dt);
+    if (rc) return rc;
+    std::vector<hipEvent_t> evk(20);
+    for (auto &v : evk) HIPCHK(hipEventCreate(&v));
+    if ((rc = ens_step<D, T>(e, evk.data()))) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int p = 0; p < 4; p++) {
+        double sum = 0;
+        for (int s = 0; s < 4; s++) {
+            float m = 0;
+            HIPCHK(hipEventElapsedTime(&m, evk[5 * s + p], evk[5 * s + p + 1]));
+            sum += m;
+        }
+        ms[p] = sum / 4;
+    }
+    for (auto &v : evk) hipEventDestroy(v);
+    return 0;
+}
+
+extern "C" int smaug_ens_time_passes(smaug_ensemble *e, const double *t0, const double *dt,
+                                     double ms[4])
+{
+    if (!e || !dt || !ms) return fail(SMAUG_EINVAL, "bad argument");
+    return EDISPATCH(e, ens_time_passes, e, t0, dt, ms);
 }
 
 extern "C" int smaug_ens_time_rk4(smaug_ensemble *e, const double *t0, const double *dt,

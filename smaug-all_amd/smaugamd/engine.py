@@ -26,7 +26,7 @@ EXPORTS = [
     "smaug_time_rk4", "smaug_ncell", "smaug_ninterior", "smaug_copy_probe",
     "smaug_set_panel", "smaug_get_panel", "smaug_time_pass", "smaug_autotune",
     "smaug_set_zstream", "smaug_get_zstream", "smaug_time_passes", "smaug_set_ratio_form",
-    "smaug_sums", "smaug_ens_sums",
+    "smaug_sums", "smaug_ens_sums", "smaug_ens_time_passes",
     "smaug_ens_create", "smaug_ens_destroy", "smaug_ens_nrun", "smaug_ens_ninterior",
     "smaug_ens_set_state", "smaug_ens_get_state", "smaug_ens_rk4", "smaug_ens_time_rk4",
 ]
@@ -91,6 +91,7 @@ def load_library(path: str = LIB_PATH):
         lib.smaug_set_ratio_form.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         lib.smaug_sums.argtypes = [vp, dp]
         lib.smaug_ens_sums.argtypes = [vp, ctypes.c_int, dp]
+        lib.smaug_ens_time_passes.argtypes = [vp, dp, dp, dp]
         lib.smaug_time_passes.argtypes = [vp, ctypes.c_double, ctypes.c_double, dp]
         lib.smaug_ens_create.argtypes = [ctypes.POINTER(SmaugCfg), ctypes.c_int,
                                          ctypes.POINTER(vp)]
@@ -319,6 +320,13 @@ class Ensemble:
     def rk4(self, t0, dt, nsteps):
         t0, dt = self._times(t0, dt)
         self._check(self.lib.smaug_ens_rk4(self.h, _dptr(t0), _dptr(dt), nsteps))
+
+    def time_passes(self, t0, dt):
+        """one more RK4 step; mean ms per launch of each pass"""
+        t0, dt = self._times(t0, dt)
+        ms = (ctypes.c_double * 4)()
+        self._check(self.lib.smaug_ens_time_passes(self.h, _dptr(t0), _dptr(dt), ms))
+        return dict(zip(("aux+cmax", "ratio", "stage update", "ghost fill"), list(ms)))
 
     def time_rk4(self, t0, dt, nsteps):
         t0, dt = self._times(t0, dt)
