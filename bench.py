@@ -343,12 +343,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-perturbation/fp64-background (mixed)",
+        "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": f"3-D stratified solar atmosphere {shape_txt}, RK4, "
                                f"dt={setup.dt:.4e} s (min of CFL 0.3 and the hyperviscous "
                                f"limit)",
-                   "cells": cells, "precision": "mixed", "parallelism": "single GPU",
+                   "cells": cells,
+                   "precision": "mixed: FP32 perturbation state and stencil arithmetic; FP64 "
+                                "background, pressure recovery, cmax/dt and conservation sums",
+                   "parallelism": "single GPU",
                    "launch_shapes": info.get("tuned")},
         "fp64": out_fp64,
         "accuracy": acc,
