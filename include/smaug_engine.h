@@ -94,8 +94,9 @@ int smaug_get_panel(smaug_engine *e, int pass, int *tpx, int *tpy);
 int smaug_time_pass(smaug_engine *e, int pass, int reps, double *ms);
 int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double best_ms[2]);
 /* 3-D grids: the stage update runs z-streamed (enable = 1, the default;
- * blocks of 64 x 4 columns walk kz output planes, kz = 0 picks 32; the
- * z-streamed source kernel uses the same depth) with the
+ * blocks of 64 x 4 columns walk kz output planes, kz = 0 picks 32, depths
+ * above 256 run as 256; the z-streamed source kernel uses the same depth)
+ * with the
  * z-streamed hyperdiffusive-source kernel, z-streamed with the per-cell
  * (enable = 2) or the LDS-plane (enable = 3) source kernel, or per cell
  * (enable = 0; the update pass then uses the XCD panel) */

@@ -197,6 +197,7 @@ static ZMap make_zmap(const smaug_engine *e, unsigned *nblocks)
     m.nty = (e->n[1] - 4 + ZS_TJ - 1) / ZS_TJ;
     const int nk = e->n[2] - 4;
     m.kz = e->kz > 0 ? (e->kz < nk ? e->kz : nk) : (nk < 32 ? nk : 32);
+    if (m.kz > ZS_KZMAX) m.kz = ZS_KZMAX;
     m.nkz = (nk + m.kz - 1) / m.kz;
     m.per_xcd = (m.ntx * m.nty + 7) / 8;
     *nblocks = (unsigned)(8 * m.per_xcd * m.nkz);
@@ -212,6 +213,7 @@ static ZMap make_hmap(const smaug_engine *e, unsigned *nblocks)
     m.nty = (e->n[1] - 4 + HB_TJ - 1) / HB_TJ;
     const int nk = e->n[2] - 4;
     m.kz = e->kzh > 0 ? (e->kzh < nk ? e->kzh : nk) : (nk < 32 ? nk : 32);
+    if (m.kz > ZS_KZMAX) m.kz = ZS_KZMAX;
     m.nkz = (nk + m.kz - 1) / m.kz;
     m.per_xcd = (m.ntx * m.nty + 7) / 8;
     *nblocks = (unsigned)(8 * m.per_xcd * m.nkz);

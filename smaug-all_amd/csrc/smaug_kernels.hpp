@@ -538,26 +538,39 @@ __device__ __forceinline__ void hyper_cell(const Geo &g, const Phys &ph, const T
     hyper_terms<D, T>(g, ph, vw, cmb, x0, x1, x2, dU);
 }
 
-// the user driver (vertical velocity, m_0 += rho v, e += rho v^2/2) at
-// interior cell (x0, x1, x2) added to dU (usersource.cu:4-92)
+// the driver's time factor drv_0 sin(2 pi (t + ct dt) / drv_1) at the stage
+// time (usersource.cu:4-92)
+__device__ __forceinline__ double driver_amp(const Phys &ph, const StageArgs &sa, double t,
+                                             double dt)
+{
+    const double ts = t + sa.ct * dt;
+    return ph.drv[0] * sin(ts * 6.283185307179586 / ph.drv[1]);
+}
+
+// the driver's sources (vertical velocity v = pr: m_0 += rho v,
+// e += rho v^2/2) of a cell with total density rc + rbc, added to dU
+template <int D, typename T>
+__device__ __forceinline__ void driver_add(double pr, const T rc, const double rbc, T *dU)
+{
+    typedef Lay<D> LY;
+    const T v = (T)pr;
+    const T rfull = (T)((double)rc + rbc);
+    dU[LY::M0] += v * rfull;
+    dU[LY::EN] += v * v * rfull / (T)2;
+}
+
+// the user driver at interior cell (x0, x1, x2) added to dU; its spatial
+// factor is the product of the per-axis profiles (usersource.cu:4-92)
 template <int D, typename T>
 __device__ __forceinline__ void driver_cell(const Geo &g, const Phys &ph, const StageArgs &sa,
                                             const T rc, const double rbc, const int x0,
                                             const int x1, const int x2, T *dU)
 {
-    typedef Lay<D> LY;
-    const double t = sa.tm[0], dt = sa.tm[1];
     if (ph.driver == 1) {
-        const double ts = t + sa.ct * dt;
-        const double tdep = sin(ts * 6.283185307179586 / ph.drv[1]);
-        double pr = ph.drv[0] * tdep * g.prof[0][x0] * g.prof[1][x1];
+        double pr = driver_amp(ph, sa, sa.tm[0], sa.tm[1]) * g.prof[0][x0] * g.prof[1][x1];
         if (D == 3) pr *= g.prof[2][x2];
-        const T v = (T)pr;
-        const T rfull = (T)((double)rc + rbc);
-        dU[LY::M0] += v * rfull;
-        dU[LY::EN] += v * v * rfull / (T)2;
+        driver_add<D, T>(pr, rc, rbc, dU);
     }
-
 }
 
 // the stage result of cell c from its L(w) = dU (mode 0: low-storage RK4
@@ -602,41 +615,6 @@ __device__ __forceinline__ void rk_write(const StageArgs &sa, const T *__restric
     }
 }
 
-
-// rk_write with the old values of the cell already loaded: un_v (modes 1
-// and RK stages 0-2) and acc_v (RK stages 1-3)
-template <int D, typename T>
-__device__ __forceinline__ void rk_write_pre(const StageArgs &sa, T *__restrict__ uout,
-                                             T *__restrict__ acc, size_t nc, ci_t c,
-                                             const T *dU, const T *un_v, const T *acc_v)
-{
-    typedef Lay<D> LY;
-    const double dt = sa.tm[1];
-    if (sa.mode == 2) {
-#pragma unroll
-        for (int f = 0; f < LY::NVE; f++) st(uout + f * nc, c, dU[f]);
-        return;
-    }
-    if (sa.mode == 1) {
-        const T tdt = (T)dt;
-#pragma unroll
-        for (int f = 0; f < LY::NVE; f++) st(uout + f * nc, c, un_v[f] + tdt * dU[f]);
-        return;
-    }
-    const int s = sa.stage;
-    const T a = (T)(s == 2 ? dt : (s == 3 ? dt / 6 : dt / 2));
-    const T b = (T)(s == 0 ? dt / 6 : dt / 3);
-#pragma unroll
-    for (int f = 0; f < LY::NVE; f++) {
-        const size_t o = f * nc;
-        if (s == 3) {
-            st(uout + o, c, acc_v[f] + a * dU[f]);
-        } else {
-            st(uout + o, c, un_v[f] + a * dU[f]);
-            st(acc + o, c, (s == 0 ? un_v[f] : acc_v[f]) + b * dU[f]);
-        }
-    }
-}
 
 // the whole stage update of interior cell (x0, x1, x2)
 template <int D, typename T>

@@ -28,6 +28,7 @@ constexpr int ZS_NB = ZS_BI * ZS_BJ;
 constexpr int ZS_NT = ZS_TI * ZS_TJ;                // threads per block
 constexpr int ZS_NH = ZS_NB - ZS_NT;                // halo cells of the box
 constexpr int ZS_NHQ = (ZS_NH + ZS_NT - 1) / ZS_NT; // halo cells per thread
+constexpr int ZS_KZMAX = 256; // deepest z-stream chunk (per-plane factors staged in LDS)
 
 // block -> (tile i, tile j, z-chunk).  The tiles are cut into 8 contiguous
 // groups, one per XCD (block b runs on XCD b % 8); consecutive blocks of an
@@ -156,7 +157,8 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
                                              const T *__restrict__ un, T *__restrict__ uout,
                                              T *__restrict__ acc, const T *__restrict__ dUh,
                                              const double *__restrict__ bg,
-                                             T (*sF)[7][ZS_NB], T (*sV)[ZS_NB])
+                                             T (*sF)[7][ZS_NB], T (*sV)[ZS_NB], T *sZ,
+                                             double *sP)
 {
     typedef Lay<3> LY;
 
@@ -190,6 +192,32 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
     const bool own_in = i <= n0 - 1 && j <= n1 - 1; // own cell lies in the grid
     const ci_t colc = (ci_t)i + s1 * (ci_t)j;
 
+    // Metric and driver factors, read once per block: the in-plane ones per
+    // thread into registers, those of the chunk's output planes into LDS (a
+    // global load inside the plane loop would be waited for together with
+    // the prefetched next plane, as vector memory operations retire in
+    // order).  Same arithmetic as the per-cell kernel.
+    const double dt = sa.tm[1];
+    // stage result = old + rk_a L(w) (old = the accumulator in RK stage 3),
+    // accumulator = (u_n or accumulator) + rk_b L(w) in RK stages 0-2
+    const int rs = sa.stage;
+    const T rk_a = sa.mode == 1 ? (T)dt : (T)(rs == 2 ? dt : (rs == 3 ? dt / 6 : dt / 2));
+    const T rk_b = (T)(rs == 0 ? dt / 6 : dt / 3);
+    const bool rk_old_acc = sa.mode == 0 && rs == 3;
+    T i12x[2] = {0, 0};
+    double pxy = 0;
+    if (act) {
+        i12x[0] = (T)(1.0 / (12.0 * g.dx[0][i]));
+        i12x[1] = (T)(1.0 / (12.0 * g.dx[1][j]));
+        if (ph.driver == 1)
+            pxy = driver_amp(ph, sa, sa.tm[0], dt) * g.prof[0][i] * g.prof[1][j];
+    }
+    for (int t = threadIdx.x; t < k1 - k0; t += ZS_NT) {
+        sZ[t] = (T)(1.0 / (12.0 * g.dx[2][k0 + t]));
+        sP[t] = g.prof[2][k0 + t];
+    }
+    __syncthreads();
+
     // Ring of pending outputs, slots 0..3 = planes P-2..P+1 at the start of
     // front P: rU holds dU; rG the k-differences of v still waiting for
     // the background-work coefficients of planes P and P+1 (slots 2, 3);
@@ -212,8 +240,26 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
     const bool need_acc = sa.mode == 0 && sa.stage != 0;
     // raw state of the own cell and the halo cells of the current plane
     St<3, T> ro, rh[ZS_NHQ];
-    if (own_in) zs_raw<T>(ro, uin, bg, nc, colc + s2 * (ci_t)(k0 - 2));
+    zs_raw<T>(ro, uin, bg, nc, own_in ? colc + s2 * (ci_t)(k0 - 2) : 0u);
+    // The output completed in one iteration (stage result ov, accumulator
+    // av) is stored at the head of the next one, behind the loads of the own
+    // cell's plane, which the head then waits for without waiting for the
+    // stores.  Every lane stores on every path: without an output it writes
+    // 0 into cell 0, a corner ghost cell that no kernel of the stage reads
+    // and the ghost fill rewrites.
+    const bool wacc = sa.mode == 0 && sa.stage != 3;
+    T ov[8], av[8];
+#pragma unroll
+    for (int f = 0; f < 8; f++) ov[f] = av[f] = 0;
+    ci_t cs = 0, ca = 0;
+    auto store_out = [&]() {
+#pragma unroll
+        for (int f = 0; f < 8; f++) st(uout + f * nc, cs, ov[f]);
+#pragma unroll
+        for (int f = 0; f < 8; f++) st(acc + f * nc, ca, av[f]);
+    };
     for (int P = k0 - 2; P < k1 + 2; P++) {
+        store_out();
         const bool outp = P >= k0 && P < k1;
         ZCell<T> z;
         double rbo = 0, ptbo = 0;
@@ -257,28 +303,30 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
         const int O = P - 2;
         const bool fin = act && O >= k0 && O < k1;
         const ci_t co = colc + s2 * (ci_t)O;
-        T fu[8], fa[8], fh[8], fr = 0;
-        double frb = 0;
-        if (fin) {
+        // (issued by every lane: an input the stage does not use, or a lane
+        // without an output, reads cell 0)
+        T fu[8], fa[8], fh[8], fr;
+        double frb;
+        {
+            const ci_t cu = fin && need_un ? co : 0u, cc = fin && need_acc ? co : 0u;
+            const ci_t chy = fin && ph.hyperdif ? co : 0u, cd = fin && ph.driver == 1 ? co : 0u;
 #pragma unroll
             for (int f = 0; f < 8; f++) {
-                fu[f] = need_un ? ld(un + f * nc, co) : (T)0;
-                fa[f] = need_acc ? ld(acc + f * nc, co) : (T)0;
-                fh[f] = ph.hyperdif ? ld(dUh + f * nc, co) : (T)0;
+                fu[f] = ld(un + f * nc, cu);
+                fa[f] = ld(acc + f * nc, cc);
+                fh[f] = ld(dUh + f * nc, chy);
             }
-            if (ph.driver == 1) {
-                fr = ld(uin, co);
-                frb = ld(bg + nc, co);
-            }
+            fr = ld(uin, cd);
+            frb = ld(bg + nc, cd);
         }
-        if (P + 1 < k1 + 2) {
-            const ci_t pn = s2 * (ci_t)(P + 1);
-            if (own_in) zs_raw<T>(ro, uin, bg, nc, colc + pn);
-            if (P + 1 >= k0 && P + 1 < k1)
+        // The own cell's loads come last and are issued on every path (the
+        // last iteration reloads its plane, lanes off the grid read cell 0):
+        // the compiler's waits for the loads above then count past them.
+        if (P + 1 >= k0 && P + 1 < k1)
 #pragma unroll
-                for (int q = 0; q < ZS_NHQ; q++)
-                    if (hv[q]) zs_raw<T>(rh[q], uin, bg, nc, (ci_t)hc[q] + pn);
-        }
+            for (int q = 0; q < ZS_NHQ; q++)
+                if (hv[q]) zs_raw<T>(rh[q], uin, bg, nc, (ci_t)hc[q] + s2 * (ci_t)(P + 1));
+        zs_raw<T>(ro, uin, bg, nc, own_in ? colc + s2 * (ci_t)min(P + 1, k1 + 1) : 0u);
         if (outp) {
             __syncthreads();
             if (act) {
@@ -293,7 +341,7 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
                 for (int a = 0; a < 2; a++) {
                     if (zero4(g.per[a], xs[a], g.n[a])) continue;
                     const int st_ = a == 0 ? 1 : ZS_BI;
-                    const T i12 = (T)(1.0 / (12.0 * g.dx[a][xs[a]]));
+                    const T i12 = i12x[a];
                     T gv[3] = {0, 0, 0};
 #pragma unroll
                     for (int p = 0; p < 4; p++) {
@@ -319,6 +367,10 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
                 rU[2][LY::EN] += z.r * ge * z.ir;
             }
         }
+#pragma unroll
+        for (int f = 0; f < 8; f++) ov[f] = av[f] = 0;
+        cs = fin ? co : 0u;
+        ca = fin && wacc ? co : 0u;
         if (act) {
             // background-work coefficients of plane P; the k-differences of
             // v gathered for output P so far become its energy term
@@ -349,7 +401,7 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
                 // offsets -2, -1, +1, +2
                 const int off = 2 - q;
                 const T base = off == -2 ? (T)1 : (off == -1 ? (T)-8 : (off == 1 ? (T)8 : (T)-1));
-                const T cw = base * (T)(1.0 / (12.0 * g.dx[2][O]));
+                const T cw = base * sZ[O - k0];
                 T *dU = q < 4 ? rU[q] : nU;
 #pragma unroll
                 for (int f = 0; f < 7; f++) dU[zs_field(2, f)] -= cw * F[f];
@@ -368,10 +420,17 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
             // output P-2 is complete
             if (fin) {
                 T *dU = rU[0];
+                if (ph.hyperdif) {
 #pragma unroll
-                for (int f = 0; f < 8; f++) dU[f] += fh[f];
-                if (ph.driver == 1) driver_cell<3, T>(g, ph, sa, fr, frb, i, j, O, dU);
-                rk_write_pre<3, T>(sa, uout, acc, nc, co, dU, fu, fa);
+                    for (int f = 0; f < 8; f++) dU[f] += fh[f];
+                }
+                if (ph.driver == 1) driver_add<3, T>(pxy * sP[O - k0], fr, frb, dU);
+                // the stage result and accumulator as rk_write forms them
+#pragma unroll
+                for (int f = 0; f < 8; f++) {
+                    ov[f] = sa.mode == 2 ? dU[f] : (rk_old_acc ? fa[f] : fu[f]) + rk_a * dU[f];
+                    av[f] = wacc ? (sa.stage == 0 ? fu[f] : fa[f]) + rk_b * dU[f] : (T)0;
+                }
             }
             // advance the ring
 #pragma unroll
@@ -392,6 +451,7 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
             }
         }
     }
+    store_out();
 }
 
 // FP32 state: up to 256 VGPRs (the prefetched plane and the output ring
@@ -404,7 +464,9 @@ k_zstream_f(Geo g, Phys ph, StageArgs sa, ZMap zm, const float *__restrict__ uin
 {
     __shared__ float sF[2][7][ZS_NB]; // fluxes along i (0) and j (1)
     __shared__ float sV[3][ZS_NB];    // velocity
-    zstream_body<float>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV);
+    __shared__ float sZ[ZS_KZMAX];    // 1/(12 dz) of the output planes
+    __shared__ double sP[ZS_KZMAX];   // driver profile along k
+    zstream_body<float>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV, sZ, sP);
 }
 
 // FP64 state: one wave per SIMD, so that nothing spills; 74 KB of LDS
@@ -415,7 +477,9 @@ k_zstream_d(Geo g, Phys ph, StageArgs sa, ZMap zm, const double *__restrict__ ui
 {
     __shared__ double sF[2][7][ZS_NB];
     __shared__ double sV[3][ZS_NB];
-    zstream_body<double>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV);
+    __shared__ double sZ[ZS_KZMAX];
+    __shared__ double sP[ZS_KZMAX];
+    zstream_body<double>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV, sZ, sP);
 }
 
 // ---------------------------------------------------------------------------
@@ -820,7 +884,7 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                                               const unsigned long long *__restrict__ cmb,
                                               T *__restrict__ dUh, T (*sU)[HB_NB],
                                               T (*sA)[HB_NB], double *sRb, T (*sR)[HB_NB],
-                                              T (*sX)[HB_NB])
+                                              T (*sX)[HB_NB], double *sDz, T *sIz)
 {
     typedef Lay<3> LY;
     typedef Aux<3> AX;
@@ -838,6 +902,24 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
     double cmax[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) cmax[d] = __longlong_as_double((long long)cmb[d]);
+    // Metric factors, read once per block: the in-plane ones per thread into
+    // registers (dc = dx cmax, id = 1/dx), those of the chunk's output planes
+    // into LDS (a global load inside the plane loop would be waited for
+    // together with the prefetched next plane, as vector memory operations
+    // retire in order).  nu = (dx cmax) chyp R as in hyper_terms.
+    double dc[2] = {0, 0};
+    T id[2] = {0, 0};
+    if (act) {
+        dc[0] = g.dx[0][i] * cmax[0];
+        dc[1] = g.dx[1][j] * cmax[1];
+        id[0] = (T)g.idx[0][i];
+        id[1] = (T)g.idx[1][j];
+    }
+    for (int t = threadIdx.x; t < k1 - k0; t += 256) {
+        sDz[t] = g.dx[2][k0 + t] * cmax[2];
+        sIz[t] = (T)g.idx[2][k0 + t];
+    }
+    __syncthreads();
     // box cells of this thread: 0 = its own column, 1 = a halo column (if any)
     int bb[2], bi_[2], bj_[2];
     bool bv[2];
@@ -904,7 +986,18 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
     for (int q = 0; q < 2; q++)
         if (bv[q]) hz_load<T>(cur[q], uin, aux, bg, R, nc, bc[q] + s2 * (ci_t)(k0 - 1), q == 0);
 
+    // The output completed in one iteration is stored at the head of the
+    // next one, behind the loads of the own column's plane, which the head
+    // then waits for without waiting for the stores.  Every lane stores on
+    // every path: without an output it writes 0 into cell 0, a corner ghost
+    // cell that the stage update does not read.
+    T hs[8];
+#pragma unroll
+    for (int f = 0; f < 8; f++) hs[f] = 0;
+    ci_t hco = 0;
     for (int P = k0 - 1; P <= k1; P++) {
+#pragma unroll
+        for (int f = 0; f < 8; f++) st(dUh + f * nc, hco, hs[f]);
         __syncthreads(); // the previous plane's LDS reads are done
 #pragma unroll
         for (int q = 0; q < 2; q++)
@@ -928,8 +1021,8 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
         const int O = P - 1;
         const bool outO = O >= k0 && O < k1;
         if (outO) {
-            const double dxz = g.dx[2][O];
-            const T idz = (T)g.idx[2][O];
+            const double dcz = sDz[O - k0];
+            const T idz = sIz[O - k0];
             const bool z1 = zero1(zper, O, n2);
 #pragma unroll
             for (int q = 0; q < 2; q++) {
@@ -942,12 +1035,12 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                         // ca = p - e_k (plane P-2), cb = p + e_k (plane P)
                         const T dv = z1 ? (T)0 : (cur[q].v[f] - hv[q][1][f]) * idz / (T)2;
                         const T db = z1 ? (T)0 : (cur[q].b[f] - hv[q][1][2 + f]) * idz / (T)2;
-                        const T Km = (T)(dxz * cmax[2] * ph.chyp[LY::M0 + f]);
+                        const T Km = (T)(dcz * ph.chyp[LY::M0 + f]);
                         const T sv = Km * (hr[q][0][f] + hr[q][1][f]);
                         const T G = dv * sv / (T)4;
                         Xr = hn[q][0] * G;
                         Xm = G * hn[q][1];
-                        const T Kb = (T)(dxz * cmax[2] * ph.chyp[LY::B0 + f]);
+                        const T Kb = (T)(dcz * ph.chyp[LY::B0 + f]);
                         const T sb = Kb * (hr[q][0][2 + f] + hr[q][1][2 + f]);
                         const T H = db * sb / (T)2;
                         Hb = H;
@@ -989,11 +1082,11 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                 return hr[0][t][2 + k - 5];
             };
             const int xd = O;
-            const double dxd = g.dx[2][xd];
-            const T idxd = (T)g.idx[2][xd];
+            const double dcz = sDz[O - k0];
+            const T idxd = sIz[O - k0];
             const bool z1 = zero1(zper, xd, n2);
             auto nuRL = [&](int t, int field, T &nR, T &nL) {
-                const T K = (T)(dxd * cmax[2] * ph.chyp[field]);
+                const T K = (T)(dcz * ph.chyp[field]);
                 nR = K * Z(0, t);
                 nL = K * Z(1, t);
             };
@@ -1081,11 +1174,11 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                 for (int d = 0; d < 2; d++) {
                     const int ri = d == 0, rj = d == 1;
                     const int xd = xs[d], nd = g.n[d];
-                    const double dxd = g.dx[d][xd];
-                    const T idxd = (T)g.idx[d][xd];
+                    const double dcd = dc[d];
+                    const T idxd = id[d];
                     const bool z1 = zero1(g.per[d], xd, nd);
                     auto nuRL = [&](int pair, int field, T &nR, T &nL) {
-                        const T K = (T)(dxd * cmax[d] * ph.chyp[field]);
+                        const T K = (T)(dcd * ph.chyp[field]);
                         nR = K * vw.Rp(pair, 0, 0);
                         nL = K * vw.Rp(pair, -ri, -rj);
                     };
@@ -1147,7 +1240,7 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                                             : (vw.U(LY::B0 + f, bi, bj) - vw.U(LY::B0 + f, ai, aj)) *
                                                   idxd / (T)2;
                             {
-                                const T K = (T)(dxd * cmax[d] * ph.chyp[LY::M0 + f]);
+                                const T K = (T)(dcd * ph.chyp[LY::M0 + f]);
                                 const T s2_ = K * (vw.Rp(LY::pv(f, d), ni, nj) + vw.Rp(LY::pv(f, d), ai, aj));
                                 const T G = dv * s2_ / (T)4;
                                 const T rhon = (T)((double)vw.U(LY::RHO, ni, nj) + vw.rhob(ni, nj));
@@ -1155,14 +1248,14 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                                 Xm[side] = G * vw.U(LY::M0 + d, ni, nj);
                             }
                             {
-                                const T K = (T)(dxd * cmax[d] * ph.chyp[LY::B0 + f]);
+                                const T K = (T)(dcd * ph.chyp[LY::B0 + f]);
                                 const T s2_ = K * (vw.Rp(LY::pb(f, d), ni, nj) + vw.Rp(LY::pb(f, d), ai, aj));
                                 const T H = db * s2_ / (T)2;
                                 Hb[side] = H;
                                 He[side] = H * vw.U(LY::B0 + d, ni, nj);
                             }
                         }
-                        const T h = (T)g.idx[f][xf] / (T)2;
+                        const T h = id[f] / (T)2;
                         rB[LY::M0 + d] += (Xr[1] - Xr[0]) * h;
                         rB[LY::EN] += (Xm[1] - Xm[0]) * h;
                         rB[LY::B0 + d] -= (Hb[1] - Hb[0]) * h;
@@ -1178,25 +1271,25 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                 for (int d = 0; d < 2; d++) {
                     const int ri = d == 0, rj = d == 1;
                     const int xd = xs[d], nd = g.n[d];
-                    const double dxd = g.dx[d][xd];
-                    const T idxd = (T)g.idx[d][xd];
+                    const double dcd = dc[d];
+                    const T idxd = id[d];
                     const bool z1 = zero1(g.per[d], xd, nd);
                     const T dv = z1 ? (T)0
                                     : (vw.A(AX::V0 + 2, ri, rj) - vw.A(AX::V0 + 2, -ri, -rj)) * idxd / (T)2;
                     const T db = z1 ? (T)0
                                     : (vw.U(LY::B0 + 2, ri, rj) - vw.U(LY::B0 + 2, -ri, -rj)) * idxd / (T)2;
-                    const T Km = (T)(dxd * cmax[d] * ph.chyp[LY::M0 + 2]);
+                    const T Km = (T)(dcd * ph.chyp[LY::M0 + 2]);
                     const T sv = Km * (vw.Rp(LY::pv(2, d), 0, 0) + vw.Rp(LY::pv(2, d), -ri, -rj));
                     const T G = dv * sv / (T)4;
                     const T rhon = (T)((double)vw.U(LY::RHO, 0, 0) + vw.rhob(0, 0));
                     const T Xr = rhon * G, Xm = G * vw.U(LY::M0 + d, 0, 0);
-                    const T Kb = (T)(dxd * cmax[d] * ph.chyp[LY::B0 + 2]);
+                    const T Kb = (T)(dcd * ph.chyp[LY::B0 + 2]);
                     const T sb = Kb * (vw.Rp(LY::pb(2, d), 0, 0) + vw.Rp(LY::pb(2, d), -ri, -rj));
                     const T H = db * sb / (T)2;
                     const T He = H * vw.U(LY::B0 + d, 0, 0);
                     // output P-1 sees this column as its c + e_k side
                     if (P - 1 >= k0 && P - 1 < k1 && !zero1(zper, P - 1, n2)) {
-                        const T h = (T)g.idx[2][P - 1] / (T)2;
+                        const T h = sIz[P - 1 - k0] / (T)2;
                         rA[LY::M0 + d] += Xr * h;
                         rA[LY::EN] += Xm * h;
                         rA[LY::B0 + d] -= H * h;
@@ -1204,7 +1297,7 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                     }
                     // output P+1 sees it as its c - e_k side
                     if (P + 1 >= k0 && P + 1 < k1 && !zero1(zper, P + 1, n2)) {
-                        const T h = (T)g.idx[2][P + 1] / (T)2;
+                        const T h = sIz[P + 1 - k0] / (T)2;
                         rC[LY::M0 + d] -= Xr * h;
                         rC[LY::EN] -= Xm * h;
                         rC[LY::B0 + d] += H * h;
@@ -1214,12 +1307,14 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                 }
             }
         }
-        // the next plane's loads, in flight through the rest of this plane
-        if (P + 1 <= k1) {
-#pragma unroll
-            for (int q = 0; q < 2; q++)
-                if (bv[q]) hz_load<T>(cur[q], uin, aux, bg, R, nc, bc[q] + s2 * (ci_t)(P + 1), q == 0);
-        }
+        // The next plane's loads, in flight through the rest of this plane.
+        // The own column's come last and are issued on every path (the last
+        // iteration reloads its plane, a column off the grid reads cell 0),
+        // so that the compiler's waits for earlier loads count past them.
+        if (P + 1 <= k1 && bv[1])
+            hz_load<T>(cur[1], uin, aux, bg, R, nc, bc[1] + s2 * (ci_t)(P + 1), false);
+        hz_load<T>(cur[0], uin, aux, bg, R, nc, bv[0] ? bc[0] + s2 * (ci_t)min(P + 1, k1) : 0u,
+                   true);
         __syncthreads(); // the products X_{f,k}(P-1) are in LDS
         if (act && outO) {
             const int xs[2] = {i, j};
@@ -1227,7 +1322,7 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
             for (int f = 0; f < 2; f++) {
                 if (zero1(g.per[f], xs[f], g.n[f])) continue;
                 const int sf = f == 0 ? 1 : HB_BI;
-                const T h = (T)g.idx[f][xs[f]] / (T)2;
+                const T h = id[f] / (T)2;
                 const T Xr = sX[4 * f + 0][own + sf] - sX[4 * f + 0][own - sf];
                 const T Xm = sX[4 * f + 1][own + sf] - sX[4 * f + 1][own - sf];
                 const T Hb = sX[4 * f + 2][own + sf] - sX[4 * f + 2][own - sf];
@@ -1237,10 +1332,11 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                 rA[LY::B0 + 2] -= Hb * h;
                 rA[LY::EN] -= He * h;
             }
-            const ci_t co = bc[0] + s2 * (ci_t)O;
-#pragma unroll
-            for (int f = 0; f < 8; f++) st(dUh + f * nc, co, rA[f]);
         }
+        const bool wr = act && outO;
+        hco = wr ? bc[0] + s2 * (ci_t)O : 0u;
+#pragma unroll
+        for (int f = 0; f < 8; f++) hs[f] = wr ? rA[f] : (T)0;
         // advance the output ring and the own-column history
 #pragma unroll
         for (int f = 0; f < 8; f++) {
@@ -1249,6 +1345,8 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
             rC[f] = 0;
         }
     }
+#pragma unroll
+    for (int f = 0; f < 8; f++) st(dUh + f * nc, hco, hs[f]);
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void
@@ -1258,7 +1356,9 @@ k_hyper_zs_f(Geo g, ZMap zm, Phys ph, const float *__restrict__ uin, const float
 {
     __shared__ float sU[7][HB_NB], sA[4][HB_NB], sR[HZ_NIP][HB_NB], sX[8][HB_NB];
     __shared__ double sRb[HB_NB];
-    hyper_zs_body<float>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRb, sR, sX);
+    __shared__ double sDz[ZS_KZMAX]; // dz cmax_z of the output planes
+    __shared__ float sIz[ZS_KZMAX];  // 1/dz of the output planes
+    hyper_zs_body<float>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRb, sR, sX, sDz, sIz);
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void
@@ -1269,7 +1369,8 @@ k_hyper_zs_d(Geo g, ZMap zm, Phys ph, const double *__restrict__ uin,
 {
     __shared__ double sU[7][HB_NB], sA[4][HB_NB], sR[HZ_NIP][HB_NB], sX[8][HB_NB];
     __shared__ double sRb[HB_NB];
-    hyper_zs_body<double>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRb, sR, sX);
+    __shared__ double sDz[ZS_KZMAX], sIz[ZS_KZMAX];
+    hyper_zs_body<double>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRb, sR, sX, sDz, sIz);
 }
 
 // the hyperdiffusive sources of every interior cell into dUh (the stage
