@@ -127,23 +127,28 @@ __device__ __forceinline__ T ratio6(const T q[6])
     T d[5];
 #pragma unroll
     for (int f = 0; f < 5; f++) d[f] = q[f + 1] - q[f];
+    // running maxima of non-negative values: fmax(v, m) equals
+    // (v > m ? v : m) here, a NaN v included (both keep m)
     T m1 = 0, m3 = 0;
 #pragma unroll
-    for (int f = 0; f < 5; f++) {
-        const T v = fabs(d[f]);
-        m1 = v > m1 ? v : m1;
-    }
+    for (int f = 0; f < 5; f++) m1 = fmax(fabs(d[f]), m1);
 #pragma unroll
     for (int f = 1; f < 4; f++) {
         // 3 (q[f+1]-q[f]) - (q[f+2]-q[f-1])
-        const T v = fabs(fma((T)3, d[f], -(q[f + 2] - q[f - 1])));
-        m3 = v > m3 ? v : m3;
+        m3 = fmax(fabs(fma((T)3, d[f], -(q[f + 2] - q[f - 1]))), m3);
     }
     if (!(m1 > (T)0)) return (T)0;
-    if constexpr (sizeof(T) == 4)
-        return __fdividef(m3, m1);
-    else
+    if constexpr (sizeof(T) == 4) {
+        // FP32: the hardware reciprocal (1 ulp) and one multiply, within
+        // 2 ulp of m3/m1.  Below 2^-62 both are scaled by 2^64 first
+        // (exactly, as m3 <= 4 m1), so that the reciprocal's argument and
+        // result are normal numbers; m1 above 2^126 does not arise in data
+        // that the FP32 flux arithmetic can carry.
+        const T sc = m1 < 0x1p-62f ? 0x1p64f : 1.0f;
+        return (m3 * sc) * __builtin_amdgcn_rcpf(m1 * sc);
+    } else {
         return m3 / m1;
+    }
 }
 
 template <int D, typename T>

@@ -827,6 +827,14 @@ __host__ __device__ constexpr int hz_islot(int p)
                : p - (p > 2) - (p > 5) - (p > 8) - (p > 11) - (p > 14) - (p > 16) - (p > 18);
 }
 
+// z ? 0 : v with v evaluated either way, so that the loads inside v are not
+// put behind a branch
+template <typename T>
+__device__ __forceinline__ T sel0(bool z, T v)
+{
+    return z ? (T)0 : v;
+}
+
 template <typename T>
 struct HzCell {
     T r, e, m[3], b[3], v[3];
@@ -1033,8 +1041,8 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                     T Xr = 0, Xm = 0, Hb = 0, He = 0;
                     if (xq[f] >= 2 && xq[f] <= g.n[f] - 3) {
                         // ca = p - e_k (plane P-2), cb = p + e_k (plane P)
-                        const T dv = z1 ? (T)0 : (cur[q].v[f] - hv[q][1][f]) * idz / (T)2;
-                        const T db = z1 ? (T)0 : (cur[q].b[f] - hv[q][1][2 + f]) * idz / (T)2;
+                        const T dv = sel0<T>(z1, (cur[q].v[f] - hv[q][1][f]) * idz / (T)2);
+                        const T db = sel0<T>(z1, (cur[q].b[f] - hv[q][1][2 + f]) * idz / (T)2);
                         const T Km = (T)(dcz * ph.chyp[LY::M0 + f]);
                         const T sv = Km * (hr[q][0][f] + hr[q][1][f]);
                         const T G = dv * sv / (T)4;
@@ -1094,12 +1102,12 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
             const T rc = W(0, 0);
             const double rhoc = (double)rc + orb[0];
             nuRL(0, LY::RHO, nR, nL);
-            gR = z1 ? (T)0 : (W(2, 0) - rc) * idxd;
-            gL = z1 ? (T)0 : (rc - W(1, 0)) * idxd;
+            gR = sel0<T>(z1, (W(2, 0) - rc) * idxd);
+            gL = sel0<T>(z1, (rc - W(1, 0)) * idxd);
             rA[LY::RHO] += (nR * gR - nL * gL) * idxd;
             nuRL(1, LY::EN, nR, nL);
-            gR = z1 ? (T)0 : (W(2, 1) - W(0, 1)) * idxd;
-            gL = z1 ? (T)0 : (W(0, 1) - W(1, 1)) * idxd;
+            gR = sel0<T>(z1, (W(2, 1) - W(0, 1)) * idxd);
+            gL = sel0<T>(z1, (W(0, 1) - W(1, 1)) * idxd);
             rA[LY::EN] += (nR * gR - nL * gL) * idxd;
             const T rR = (T)((rhoc + (double)W(2, 0) + cur[0].rb) / 2);
             const T rL = (T)((rhoc + (double)W(1, 0) + orb[1]) / 2);
@@ -1107,8 +1115,8 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
             for (int f = 0; f < 3; f++) {
                 nuRL(2 + f, LY::M0 + f, nR, nL);
                 const T vc = W(0, 2 + f);
-                gR = z1 ? (T)0 : (W(2, 2 + f) - vc) * idxd;
-                gL = z1 ? (T)0 : (vc - W(1, 2 + f)) * idxd;
+                gR = sel0<T>(z1, (W(2, 2 + f) - vc) * idxd);
+                gL = sel0<T>(z1, (vc - W(1, 2 + f)) * idxd);
                 const T mc = W(0, 5 + f);
                 const T mR = (mc + W(2, 5 + f)) / (T)2, mL = (mc + W(1, 5 + f)) / (T)2;
                 const T k2 = (f == 2) ? (T)1 : (T)0.5;
@@ -1119,8 +1127,8 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
             for (int f = 0; f < 2; f++) {
                 nuRL(5 + f, LY::B0 + f, nR, nL);
                 const T bc_ = W(0, 8 + f), br = W(2, 8 + f), bl = W(1, 8 + f);
-                gR = z1 ? (T)0 : (br - bc_) * idxd;
-                gL = z1 ? (T)0 : (bc_ - bl) * idxd;
+                gR = sel0<T>(z1, (br - bc_) * idxd);
+                gL = sel0<T>(z1, (bc_ - bl) * idxd);
                 const T bR = (bc_ + br) / (T)2, bL = (bc_ + bl) / (T)2;
                 rA[LY::B0 + f] += (gR * nR - gL * nL) * idxd;
                 rA[LY::EN] += (bR * gR * nR - bL * gL * nL) * idxd;
@@ -1162,6 +1170,14 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
             oz[0][2] = cur[0].R[LY::pv(2, 2)];
             orb[0] = cur[0].rb;
         }
+        // The next plane's loads, in flight through the rest of this plane.
+        // The own column's come last and are issued on every path (the last
+        // iteration reloads its plane, a column off the grid reads cell 0),
+        // so that the compiler's waits for earlier loads count past them.
+        if (P + 1 <= k1 && bv[1])
+            hz_load<T>(cur[1], uin, aux, bg, R, nc, bc[1] + s2 * (ci_t)(P + 1), false);
+        hz_load<T>(cur[0], uin, aux, bg, R, nc, bv[0] ? bc[0] + s2 * (ci_t)min(P + 1, k1) : 0u,
+                   true);
         __syncthreads();
         if (act) {
             const int xs[2] = {i, j};
@@ -1184,12 +1200,12 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                     };
                     T nR, nL, gR, gL;
                     nuRL(LY::prho(d), LY::RHO, nR, nL);
-                    gR = z1 ? (T)0 : (vw.U(LY::RHO, ri, rj) - rc) * idxd;
-                    gL = z1 ? (T)0 : (rc - vw.U(LY::RHO, -ri, -rj)) * idxd;
+                    gR = sel0<T>(z1, (vw.U(LY::RHO, ri, rj) - rc) * idxd);
+                    gL = sel0<T>(z1, (rc - vw.U(LY::RHO, -ri, -rj)) * idxd);
                     rB[LY::RHO] += (nR * gR - nL * gL) * idxd;
                     nuRL(LY::pe(d), LY::EN, nR, nL);
-                    gR = z1 ? (T)0 : (vw.A(AX::EINT, ri, rj) - eic) * idxd;
-                    gL = z1 ? (T)0 : (eic - vw.A(AX::EINT, -ri, -rj)) * idxd;
+                    gR = sel0<T>(z1, (vw.A(AX::EINT, ri, rj) - eic) * idxd);
+                    gL = sel0<T>(z1, (eic - vw.A(AX::EINT, -ri, -rj)) * idxd);
                     rB[LY::EN] += (nR * gR - nL * gL) * idxd;
                     const T rR = (T)((rhoc + (double)vw.U(LY::RHO, ri, rj) + vw.rhob(ri, rj)) / 2);
                     const T rL = (T)((rhoc + (double)vw.U(LY::RHO, -ri, -rj) + vw.rhob(-ri, -rj)) / 2);
@@ -1197,8 +1213,8 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                     for (int f = 0; f < 3; f++) {
                         nuRL(LY::pv(f, d), LY::M0 + f, nR, nL);
                         const T vc = vw.A(AX::V0 + f, 0, 0);
-                        gR = z1 ? (T)0 : (vw.A(AX::V0 + f, ri, rj) - vc) * idxd;
-                        gL = z1 ? (T)0 : (vc - vw.A(AX::V0 + f, -ri, -rj)) * idxd;
+                        gR = sel0<T>(z1, (vw.A(AX::V0 + f, ri, rj) - vc) * idxd);
+                        gL = sel0<T>(z1, (vc - vw.A(AX::V0 + f, -ri, -rj)) * idxd);
                         const T mc = vw.U(LY::M0 + f, 0, 0);
                         const T mR = (mc + vw.U(LY::M0 + f, ri, rj)) / (T)2;
                         const T mL = (mc + vw.U(LY::M0 + f, -ri, -rj)) / (T)2;
@@ -1213,8 +1229,8 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                         nuRL(LY::pb(f, d), LY::B0 + f, nR, nL);
                         const T bc_ = vw.U(LY::B0 + f, 0, 0);
                         const T br = vw.U(LY::B0 + f, ri, rj), bl = vw.U(LY::B0 + f, -ri, -rj);
-                        gR = z1 ? (T)0 : (br - bc_) * idxd;
-                        gL = z1 ? (T)0 : (bc_ - bl) * idxd;
+                        gR = sel0<T>(z1, (br - bc_) * idxd);
+                        gL = sel0<T>(z1, (bc_ - bl) * idxd);
                         const T bR = (bc_ + br) / (T)2, bL = (bc_ + bl) / (T)2;
                         rB[LY::B0 + f] += (gR * nR - gL * nL) * idxd;
                         rB[LY::EN] += (bR * gR * nR - bL * gL * nL) * idxd;
@@ -1233,12 +1249,10 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                             const int xfn = xf + sg;
                             if (xfn < 2 || xfn > nf - 3) continue;
                             const int ai = ni - ri, aj = nj - rj, bi = ni + ri, bj = nj + rj;
-                            const T dv = z1 ? (T)0
-                                            : (vw.A(AX::V0 + f, bi, bj) - vw.A(AX::V0 + f, ai, aj)) *
-                                                  idxd / (T)2;
-                            const T db = z1 ? (T)0
-                                            : (vw.U(LY::B0 + f, bi, bj) - vw.U(LY::B0 + f, ai, aj)) *
-                                                  idxd / (T)2;
+                            const T dv = sel0<T>(z1, (vw.A(AX::V0 + f, bi, bj) - vw.A(AX::V0 + f, ai, aj)) *
+                                                  idxd / (T)2);
+                            const T db = sel0<T>(z1, (vw.U(LY::B0 + f, bi, bj) - vw.U(LY::B0 + f, ai, aj)) *
+                                                  idxd / (T)2);
                             {
                                 const T K = (T)(dcd * ph.chyp[LY::M0 + f]);
                                 const T s2_ = K * (vw.Rp(LY::pv(f, d), ni, nj) + vw.Rp(LY::pv(f, d), ai, aj));
@@ -1274,10 +1288,8 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                     const double dcd = dc[d];
                     const T idxd = id[d];
                     const bool z1 = zero1(g.per[d], xd, nd);
-                    const T dv = z1 ? (T)0
-                                    : (vw.A(AX::V0 + 2, ri, rj) - vw.A(AX::V0 + 2, -ri, -rj)) * idxd / (T)2;
-                    const T db = z1 ? (T)0
-                                    : (vw.U(LY::B0 + 2, ri, rj) - vw.U(LY::B0 + 2, -ri, -rj)) * idxd / (T)2;
+                    const T dv = sel0<T>(z1, (vw.A(AX::V0 + 2, ri, rj) - vw.A(AX::V0 + 2, -ri, -rj)) * idxd / (T)2);
+                    const T db = sel0<T>(z1, (vw.U(LY::B0 + 2, ri, rj) - vw.U(LY::B0 + 2, -ri, -rj)) * idxd / (T)2);
                     const T Km = (T)(dcd * ph.chyp[LY::M0 + 2]);
                     const T sv = Km * (vw.Rp(LY::pv(2, d), 0, 0) + vw.Rp(LY::pv(2, d), -ri, -rj));
                     const T G = dv * sv / (T)4;
@@ -1307,14 +1319,6 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                 }
             }
         }
-        // The next plane's loads, in flight through the rest of this plane.
-        // The own column's come last and are issued on every path (the last
-        // iteration reloads its plane, a column off the grid reads cell 0),
-        // so that the compiler's waits for earlier loads count past them.
-        if (P + 1 <= k1 && bv[1])
-            hz_load<T>(cur[1], uin, aux, bg, R, nc, bc[1] + s2 * (ci_t)(P + 1), false);
-        hz_load<T>(cur[0], uin, aux, bg, R, nc, bv[0] ? bc[0] + s2 * (ci_t)min(P + 1, k1) : 0u,
-                   true);
         __syncthreads(); // the products X_{f,k}(P-1) are in LDS
         if (act && outO) {
             const int xs[2] = {i, j};
