@@ -106,6 +106,15 @@ int smaug_set_zstream(smaug_engine *e, int enable, int kz);
 int smaug_set_ratio_form(smaug_engine *e, int zstreamed, int kz);
 /* the form of the 3-D update pass in use (enable as above) and its depth */
 int smaug_get_zstream(smaug_engine *e, int *enable, int *kz);
+/* 3-D grids: when the uploaded background is the same on every k-plane (a
+ * stratified atmosphere whose vertical axis is i), smaug_set_state also
+ * keeps it as one k-plane, and the aux/cmax pass and the z-streamed
+ * source and update kernels read that plane, once per column, instead of
+ * the full background (enable = 1, the default; 0 always reads the full
+ * arrays).  Same arithmetic either way.  smaug_get_bg_plane reports whether
+ * the plane is in use. */
+int smaug_set_bg_plane(smaug_engine *e, int enable);
+int smaug_get_bg_plane(smaug_engine *e, int *active);
 
 /* conservation diagnostics: sums[f] = sum over the interior cells of evolved
  * field f of the current state, accumulated in FP64 with compensated

@@ -81,6 +81,9 @@ struct smaug_engine {
     void *HYP;    // 3-D: hyperdiffusive part of L(w) for the z-streamed update
     double *d_part; // per-block partial sums of the conservation diagnostics
     double *bg;
+    double *bgp;  // 3-D: the background as one k-plane (field stride n0*n1)
+    int bgp_ok;   // the uploaded background does not vary along k
+    int bgp_on;   // use bgp when bgp_ok (default on)
     double *d_dx[3], *d_idx[3], *d_prof[3];
     unsigned long long *d_cmax;
     double *d_tm; // [t, dt]
@@ -100,10 +103,26 @@ struct smaug_engine {
 // ---------------------------------------------------------------------------
 // launch helpers (templated on D and T)
 // ---------------------------------------------------------------------------
+// the background of a 3-D grid is read from its k-plane copy
+static bool bg_planar(const smaug_engine *e) { return e->D == 3 && e->bgp_ok && e->bgp_on; }
+
+constexpr int PREP_KC = 16; // planes per block of k_prep_col
+
 template <int D, typename T>
 static int launch_cmax(smaug_engine *e, const void *u)
 {
     hipLaunchKernelGGL(k_reset_cmax, dim3(1), dim3(64), 0, e->stream, e->d_cmax);
+    if constexpr (D == 3) {
+        if (bg_planar(e)) {
+            const unsigned nx = (unsigned)((e->geo.s2 + 255) / 256);
+            const unsigned ny = (unsigned)((e->n[2] + PREP_KC - 1) / PREP_KC);
+            hipLaunchKernelGGL((k_prep_col<T>), dim3(nx, ny), dim3(256), 0, e->stream, e->geo,
+                               e->phys, (const T *)u, (const double *)e->bgp, (T *)e->AUX,
+                               e->d_cmax, PREP_KC);
+            HIPCHK(hipGetLastError());
+            return 0;
+        }
+    }
     idx_t nb = (e->nc + 255) / 256;
     if (nb > 4096) nb = 4096;
     hipLaunchKernelGGL((k_prep<D, T>), dim3((unsigned)nb), dim3(256), 0, e->stream, e->geo,
@@ -226,15 +245,17 @@ static int launch_hyper_sources(smaug_engine *e, const void *uin)
     if (e->phys.hyperdif && e->hyper_lds == 2) {
         unsigned nb;
         ZMap m = make_hmap(e, &nb);
+        const bool bp = bg_planar(e);
+        const double *bg = bp ? e->bgp : e->bg;
         if constexpr (sizeof(T) == 4)
-            hipLaunchKernelGGL(k_hyper_zs_f, dim3(nb), dim3(256), 0, e->stream, e->geo, m, e->phys,
-                               (const T *)uin, (const T *)e->R, (const T *)e->AUX,
-                               (const double *)e->bg, (const unsigned long long *)e->d_cmax,
+            hipLaunchKernelGGL(bp ? k_hyper_zs_f<true> : k_hyper_zs_f<false>, dim3(nb), dim3(256),
+                               0, e->stream, e->geo, m, e->phys, (const T *)uin, (const T *)e->R,
+                               (const T *)e->AUX, bg, (const unsigned long long *)e->d_cmax,
                                (T *)e->HYP);
         else
-            hipLaunchKernelGGL(k_hyper_zs_d, dim3(nb), dim3(256), 0, e->stream, e->geo, m, e->phys,
-                               (const T *)uin, (const T *)e->R, (const T *)e->AUX,
-                               (const double *)e->bg, (const unsigned long long *)e->d_cmax,
+            hipLaunchKernelGGL(bp ? k_hyper_zs_d<true> : k_hyper_zs_d<false>, dim3(nb), dim3(256),
+                               0, e->stream, e->geo, m, e->phys, (const T *)uin, (const T *)e->R,
+                               (const T *)e->AUX, bg, (const unsigned long long *)e->d_cmax,
                                (T *)e->HYP);
         HIPCHK(hipGetLastError());
         return 0;
@@ -271,14 +292,16 @@ static int launch_zstream(smaug_engine *e, StageArgs sa, const void *uin, const 
 {
     unsigned nb;
     ZMap m = make_zmap(e, &nb);
+    const bool bp = bg_planar(e);
+    const double *bg = bp ? e->bgp : e->bg;
     if constexpr (sizeof(T) == 4)
-        hipLaunchKernelGGL(k_zstream_f, dim3(nb), dim3(ZS_NT), 0, e->stream, e->geo, e->phys, sa, m,
-                           (const T *)uin, (const T *)un, (T *)uout, (T *)e->ACC,
-                           (const T *)e->HYP, (const double *)e->bg);
+        hipLaunchKernelGGL(bp ? k_zstream_f<true> : k_zstream_f<false>, dim3(nb), dim3(ZS_NT), 0,
+                           e->stream, e->geo, e->phys, sa, m, (const T *)uin, (const T *)un,
+                           (T *)uout, (T *)e->ACC, (const T *)e->HYP, bg);
     else
-        hipLaunchKernelGGL(k_zstream_d, dim3(nb), dim3(ZS_NT), 0, e->stream, e->geo, e->phys, sa, m,
-                           (const T *)uin, (const T *)un, (T *)uout, (T *)e->ACC,
-                           (const T *)e->HYP, (const double *)e->bg);
+        hipLaunchKernelGGL(bp ? k_zstream_d<true> : k_zstream_d<false>, dim3(nb), dim3(ZS_NT), 0,
+                           e->stream, e->geo, e->phys, sa, m, (const T *)uin, (const T *)un,
+                           (T *)uout, (T *)e->ACC, (const T *)e->HYP, bg);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -421,6 +444,7 @@ static void free_engine(smaug_engine *e)
     if (e->HYP) hipFree(e->HYP);
     if (e->d_part) hipFree(e->d_part);
     if (e->bg) hipFree(e->bg);
+    if (e->bgp) hipFree(e->bgp);
     if (e->d_cmax) hipFree(e->d_cmax);
     if (e->d_tm) hipFree(e->d_tm);
     if (e->stream) hipStreamDestroy(e->stream);
@@ -472,6 +496,7 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
     TRYALLOC(e->AUX, e->tsz * (3 + D) * e->nc);
     if (D == 3 && cfg->hyperdif) TRYALLOC(e->HYP, e->tsz * nve * e->nc);
     TRYALLOC(e->bg, sizeof(double) * nbg * e->nc);
+    if (D == 3) TRYALLOC(e->bgp, sizeof(double) * nbg * (size_t)e->n[0] * e->n[1]);
     TRYALLOC(e->d_cmax, sizeof(unsigned long long) * 4);
     TRYALLOC(e->d_tm, sizeof(double) * 2);
     for (int d = 0; d < 3; d++) {
@@ -509,6 +534,7 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
     e->kzh = 0;
     e->ratio_zs = D == 3;
     e->kzr = 0;
+    e->bgp_on = 1;
     *out = e;
     return 0;
 }
@@ -539,10 +565,37 @@ static int bc_only(smaug_engine *e, void *u)
     return launch_bc<D, T>(e, u);
 }
 
+static void drop_graphs(smaug_engine *e);
+
+// 1 if every background field of a 3-D grid is the same on every k-plane
+// (bit for bit), as for a stratified atmosphere with k horizontal
+static int bg_k_invariant(const smaug_engine *e, const double *bg)
+{
+    if (e->D != 3) return 0;
+    const size_t s2 = (size_t)e->n[0] * e->n[1];
+    for (int f = 0; f < 2 + e->D; f++) {
+        const double *b = bg + (size_t)f * e->nc;
+        for (int k = 1; k < e->n[2]; k++)
+            if (memcmp(b, b + k * s2, s2 * sizeof(double))) return 0;
+    }
+    return 1;
+}
+
 extern "C" int smaug_set_state(smaug_engine *e, const double *w, const double *bg)
 {
     if (!e || !w || !bg) return fail(SMAUG_EINVAL, "null argument");
     int nve = 2 + 2 * e->D, nbg = 2 + e->D;
+    if (e->D == 3) {
+        const int ok = bg_k_invariant(e, bg);
+        if (ok) {
+            const size_t s2 = (size_t)e->n[0] * e->n[1];
+            for (int f = 0; f < nbg; f++)
+                HIPCHK(hipMemcpyAsync(e->bgp + f * s2, bg + (size_t)f * e->nc, s2 * sizeof(double),
+                                      hipMemcpyHostToDevice, e->stream));
+        }
+        if (ok != e->bgp_ok) drop_graphs(e); // the captured launches read the other copy
+        e->bgp_ok = ok;
+    }
     idx_t n = (idx_t)nve * e->nc;
     std::vector<char> buf;
     if (e->prec == SMAUG_FP64)
@@ -965,6 +1018,21 @@ extern "C" int smaug_set_ratio_form(smaug_engine *e, int zstreamed, int kz)
     e->ratio_zs = zstreamed ? 1 : 0;
     e->kzr = kz;
     drop_graphs(e);
+    return 0;
+}
+
+extern "C" int smaug_set_bg_plane(smaug_engine *e, int enable)
+{
+    if (!e) return fail(SMAUG_EINVAL, "null argument");
+    e->bgp_on = enable ? 1 : 0;
+    drop_graphs(e);
+    return 0;
+}
+
+extern "C" int smaug_get_bg_plane(smaug_engine *e, int *active)
+{
+    if (!e || !active) return fail(SMAUG_EINVAL, "null argument");
+    *active = bg_planar(e) ? 1 : 0;
     return 0;
 }
 

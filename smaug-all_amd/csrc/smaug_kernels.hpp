@@ -48,16 +48,13 @@ struct Aux {
     static constexpr int IRHO = 0, PT = 1, EINT = 2, V0 = 3, N = 3 + D;
 };
 
-// aux of cell c and the running cfast maxima
+// aux of cell c (state s loaded) and the running cfast maxima
 template <int D, typename T>
-__device__ __forceinline__ void prep_cell(const Geo &g, const Phys &ph, const T *__restrict__ u,
-                                          const double *__restrict__ bg, T *__restrict__ aux,
-                                          ci_t c, double *mx)
+__device__ __forceinline__ void prep_st(const Geo &g, const Phys &ph, const St<D, T> &s,
+                                        T *__restrict__ aux, ci_t c, double *mx)
 {
     typedef Aux<D> AX;
     const size_t nc = (size_t)g.nc;
-    St<D, T> s;
-    load_st32<D, T>(s, u, bg, nc, c);
     const double rho = rho_full(s);
     const double ir = 1.0 / rho;
     st(aux + AX::IRHO * nc, c, (T)ir);
@@ -70,6 +67,16 @@ __device__ __forceinline__ void prep_cell(const Geo &g, const Phys &ph, const T 
         double v = cfast<D, T>(s, d, ph.gamma);
         mx[d] = v > mx[d] ? v : mx[d];
     }
+}
+
+template <int D, typename T>
+__device__ __forceinline__ void prep_cell(const Geo &g, const Phys &ph, const T *__restrict__ u,
+                                          const double *__restrict__ bg, T *__restrict__ aux,
+                                          ci_t c, double *mx)
+{
+    St<D, T> s;
+    load_st32<D, T>(s, u, bg, (size_t)g.nc, c);
+    prep_st<D, T>(g, ph, s, aux, c, mx);
 }
 
 // block-wide max of mx[] (blockDim.x == 256) folded into cm[] with a 64-bit
@@ -106,6 +113,34 @@ __global__ __launch_bounds__(256) void k_prep(Geo g, Phys ph, const T *__restric
     for (ci_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
         prep_cell<D, T>(g, ph, u, bg, aux, c, mx);
     block_max_to<D>(mx, red, cm);
+}
+
+// The same pass for a 3-D background that does not vary along k, stored as
+// one plane (bgp, field stride s2).  Thread = one column (i, j) of the
+// plane, block row y = a chunk of kc planes: the column's background is
+// loaded once and only the evolved fields are streamed.
+template <typename T>
+__global__ __launch_bounds__(256) void k_prep_col(Geo g, Phys ph, const T *__restrict__ u,
+                                                  const double *__restrict__ bgp,
+                                                  T *__restrict__ aux, unsigned long long *cm,
+                                                  int kc)
+{
+    __shared__ double red[3][4];
+    double mx[3] = {0.0, 0.0, 0.0};
+    const ci_t s2 = (ci_t)g.s2;
+    const ci_t p = blockIdx.x * 256 + threadIdx.x;
+    const int ka = blockIdx.y * kc;
+    const int kb = min(ka + kc, g.n[2]);
+    if (p < s2) {
+        St<3, T> s;
+        load_bg32<3, T>(s, bgp, (size_t)s2, p);
+        for (int k = ka; k < kb; k++) {
+            const ci_t c = p + s2 * (ci_t)k;
+            load_pert32<3, T>(s, u, (size_t)g.nc, c);
+            prep_st<3, T>(g, ph, s, aux, c, mx);
+        }
+    }
+    block_max_to<3>(mx, red, cm);
 }
 
 // ---------------------------------------------------------------------------

@@ -108,6 +108,30 @@ __device__ __forceinline__ void load_st(St<D, T> &s, const T *__restrict__ u,
     for (int d = 0; d < D; d++) s.bb[d] = bg[(idx_t)(2 + d) * nc + c];
 }
 
+// the perturbation fields of cell c only (the background is left as it is)
+template <int D, typename T>
+__device__ __forceinline__ void load_pert32(St<D, T> &s, const T *__restrict__ u, size_t nc, ci_t c)
+{
+    s.r = ld(u, c);
+#pragma unroll
+    for (int d = 0; d < D; d++) s.m[d] = ld(u + (1 + d) * nc, c);
+    s.e = ld(u + (1 + D) * nc, c);
+#pragma unroll
+    for (int d = 0; d < D; d++) s.b[d] = ld(u + (2 + D + d) * nc, c);
+}
+
+// the background fields of a cell from an array of field stride nb (nc for
+// a full background, the plane size for a background stored as one k-plane)
+template <int D, typename T>
+__device__ __forceinline__ void load_bg32(St<D, T> &s, const double *__restrict__ bg, size_t nb,
+                                          ci_t c)
+{
+    s.eb = ld(bg, c);
+    s.rb = ld(bg + nb, c);
+#pragma unroll
+    for (int d = 0; d < D; d++) s.bb[d] = ld(bg + (2 + d) * nb, c);
+}
+
 template <int D, typename T>
 __device__ __forceinline__ void load_st32(St<D, T> &s, const T *__restrict__ u,
                                           const double *__restrict__ bg, size_t nc, ci_t c)

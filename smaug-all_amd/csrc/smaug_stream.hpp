@@ -80,12 +80,17 @@ struct ZCell {
 };
 
 // a cell's raw state as loaded (the loads of the next plane are issued
-// into it while the current plane is being processed)
-template <typename T>
+// into it while the current plane is being processed).  BGP: the background
+// does not vary along k and was loaded into s once for the column, so only
+// the evolved fields are loaded.
+template <bool BGP, typename T>
 __device__ __forceinline__ void zs_raw(St<3, T> &s, const T *__restrict__ u,
                                        const double *__restrict__ bg, size_t nc, ci_t c)
 {
-    load_st32<3, T>(s, u, bg, nc, c);
+    if constexpr (BGP)
+        load_pert32<3, T>(s, u, nc, c);
+    else
+        load_st32<3, T>(s, u, bg, nc, c);
 }
 
 template <typename T>
@@ -151,7 +156,9 @@ __device__ __forceinline__ int zs_field(int a, int q)
     return 5 + (q == 5 ? first : second);
 }
 
-template <typename T>
+// BGP: bg is a background that does not vary along k, stored as one plane
+// (field stride s2); its values are loaded once per column
+template <bool BGP, typename T>
 __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const StageArgs &sa,
                                              const ZMap &zm, const T *__restrict__ uin,
                                              const T *__restrict__ un, T *__restrict__ uout,
@@ -240,7 +247,12 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
     const bool need_acc = sa.mode == 0 && sa.stage != 0;
     // raw state of the own cell and the halo cells of the current plane
     St<3, T> ro, rh[ZS_NHQ];
-    zs_raw<T>(ro, uin, bg, nc, own_in ? colc + s2 * (ci_t)(k0 - 2) : 0u);
+    if constexpr (BGP) {
+        load_bg32<3, T>(ro, bg, (size_t)s2, own_in ? colc : 0u);
+#pragma unroll
+        for (int q = 0; q < ZS_NHQ; q++) load_bg32<3, T>(rh[q], bg, (size_t)s2, hv[q] ? (ci_t)hc[q] : 0u);
+    }
+    zs_raw<BGP, T>(ro, uin, bg, nc, own_in ? colc + s2 * (ci_t)(k0 - 2) : 0u);
     // The output completed in one iteration (stage result ov, accumulator
     // av) is stored at the head of the next one, behind the loads of the own
     // cell's plane, which the head then waits for without waiting for the
@@ -317,7 +329,10 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
                 fh[f] = ld(dUh + f * nc, chy);
             }
             fr = ld(uin, cd);
-            frb = ld(bg + nc, cd);
+            if constexpr (BGP)
+                frb = ro.rb;
+            else
+                frb = ld(bg + nc, cd);
         }
         // The own cell's loads come last and are issued on every path (the
         // last iteration reloads its plane, lanes off the grid read cell 0):
@@ -325,8 +340,8 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
         if (P + 1 >= k0 && P + 1 < k1)
 #pragma unroll
             for (int q = 0; q < ZS_NHQ; q++)
-                if (hv[q]) zs_raw<T>(rh[q], uin, bg, nc, (ci_t)hc[q] + s2 * (ci_t)(P + 1));
-        zs_raw<T>(ro, uin, bg, nc, own_in ? colc + s2 * (ci_t)min(P + 1, k1 + 1) : 0u);
+                if (hv[q]) zs_raw<BGP, T>(rh[q], uin, bg, nc, (ci_t)hc[q] + s2 * (ci_t)(P + 1));
+        zs_raw<BGP, T>(ro, uin, bg, nc, own_in ? colc + s2 * (ci_t)min(P + 1, k1 + 1) : 0u);
         if (outp) {
             __syncthreads();
             if (act) {
@@ -456,7 +471,8 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
 
 // FP32 state: up to 256 VGPRs (the prefetched plane and the output ring
 // stay in registers without spills), two 256-thread blocks of 37 KB of LDS
-// per CU
+// per CU.  BGP: bg is a background stored as one k-plane.
+template <bool BGP>
 __global__ __launch_bounds__(ZS_NT) __attribute__((amdgpu_waves_per_eu(2))) void
 k_zstream_f(Geo g, Phys ph, StageArgs sa, ZMap zm, const float *__restrict__ uin,
             const float *__restrict__ un, float *__restrict__ uout, float *__restrict__ acc,
@@ -466,10 +482,11 @@ k_zstream_f(Geo g, Phys ph, StageArgs sa, ZMap zm, const float *__restrict__ uin
     __shared__ float sV[3][ZS_NB];    // velocity
     __shared__ float sZ[ZS_KZMAX];    // 1/(12 dz) of the output planes
     __shared__ double sP[ZS_KZMAX];   // driver profile along k
-    zstream_body<float>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV, sZ, sP);
+    zstream_body<BGP, float>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV, sZ, sP);
 }
 
 // FP64 state: one wave per SIMD, so that nothing spills; 74 KB of LDS
+template <bool BGP>
 __global__ __launch_bounds__(ZS_NT) __attribute__((amdgpu_waves_per_eu(1))) void
 k_zstream_d(Geo g, Phys ph, StageArgs sa, ZMap zm, const double *__restrict__ uin,
             const double *__restrict__ un, double *__restrict__ uout, double *__restrict__ acc,
@@ -479,7 +496,7 @@ k_zstream_d(Geo g, Phys ph, StageArgs sa, ZMap zm, const double *__restrict__ ui
     __shared__ double sV[3][ZS_NB];
     __shared__ double sZ[ZS_KZMAX];
     __shared__ double sP[ZS_KZMAX];
-    zstream_body<double>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV, sZ, sP);
+    zstream_body<BGP, double>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV, sZ, sP);
 }
 
 // ---------------------------------------------------------------------------
@@ -843,8 +860,9 @@ struct HzCell {
 };
 
 // all 21 ratio pairs for the own column (full = true); a halo column needs
-// only the in-plane pairs and pv/pb(f, 2), f in-plane
-template <typename T>
+// only the in-plane pairs and pv/pb(f, 2), f in-plane.  BGP: the background
+// does not vary along k, and h.rb was set once for the column.
+template <bool BGP, typename T>
 __device__ __forceinline__ void hz_load(HzCell<T> &h, const T *__restrict__ u,
                                         const T *__restrict__ aux, const double *__restrict__ bg,
                                         const T *__restrict__ R, size_t nc, ci_t c, bool full)
@@ -859,7 +877,7 @@ __device__ __forceinline__ void hz_load(HzCell<T> &h, const T *__restrict__ u,
         h.b[d] = ld(u + (LY::B0 + d) * nc, c);
         h.v[d] = ld(aux + (AX::V0 + d) * nc, c);
     }
-    h.rb = ld(bg + nc, c);
+    if constexpr (!BGP) h.rb = ld(bg + nc, c);
 #pragma unroll
     for (int p = 0; p < LY::NPAIR; p++)
         h.R[p] = (full || hz_islot(p) >= 0 || p == LY::pv(0, 2) || p == LY::pv(1, 2) ||
@@ -883,7 +901,8 @@ struct HzView {
     __device__ __forceinline__ T Rp(int p, int di, int dj) const { return sR[hz_islot(p)][bx(di, dj)]; }
 };
 
-template <typename T>
+// BGP: bg is a background stored as one k-plane (field stride s2)
+template <bool BGP, typename T>
 __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, const Phys &ph,
                                               const T *__restrict__ uin,
                                               const T *__restrict__ R,
@@ -990,9 +1009,13 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
     const bool zper = g.per[2];
 
     HzCell<T> cur[2];
+    if constexpr (BGP) {
+#pragma unroll
+        for (int q = 0; q < 2; q++) cur[q].rb = ld(bg + (size_t)s2, bv[q] ? bc[q] : 0u);
+    }
 #pragma unroll
     for (int q = 0; q < 2; q++)
-        if (bv[q]) hz_load<T>(cur[q], uin, aux, bg, R, nc, bc[q] + s2 * (ci_t)(k0 - 1), q == 0);
+        if (bv[q]) hz_load<BGP, T>(cur[q], uin, aux, bg, R, nc, bc[q] + s2 * (ci_t)(k0 - 1), q == 0);
 
     // The output completed in one iteration is stored at the head of the
     // next one, behind the loads of the own column's plane, which the head
@@ -1175,8 +1198,8 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
         // iteration reloads its plane, a column off the grid reads cell 0),
         // so that the compiler's waits for earlier loads count past them.
         if (P + 1 <= k1 && bv[1])
-            hz_load<T>(cur[1], uin, aux, bg, R, nc, bc[1] + s2 * (ci_t)(P + 1), false);
-        hz_load<T>(cur[0], uin, aux, bg, R, nc, bv[0] ? bc[0] + s2 * (ci_t)min(P + 1, k1) : 0u,
+            hz_load<BGP, T>(cur[1], uin, aux, bg, R, nc, bc[1] + s2 * (ci_t)(P + 1), false);
+        hz_load<BGP, T>(cur[0], uin, aux, bg, R, nc, bv[0] ? bc[0] + s2 * (ci_t)min(P + 1, k1) : 0u,
                    true);
         __syncthreads();
         if (act) {
@@ -1353,6 +1376,7 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
     for (int f = 0; f < 8; f++) st(dUh + f * nc, hco, hs[f]);
 }
 
+template <bool BGP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void
 k_hyper_zs_f(Geo g, ZMap zm, Phys ph, const float *__restrict__ uin, const float *__restrict__ R,
              const float *__restrict__ aux, const double *__restrict__ bg,
@@ -1362,9 +1386,10 @@ k_hyper_zs_f(Geo g, ZMap zm, Phys ph, const float *__restrict__ uin, const float
     __shared__ double sRb[HB_NB];
     __shared__ double sDz[ZS_KZMAX]; // dz cmax_z of the output planes
     __shared__ float sIz[ZS_KZMAX];  // 1/dz of the output planes
-    hyper_zs_body<float>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRb, sR, sX, sDz, sIz);
+    hyper_zs_body<BGP, float>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRb, sR, sX, sDz, sIz);
 }
 
+template <bool BGP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void
 k_hyper_zs_d(Geo g, ZMap zm, Phys ph, const double *__restrict__ uin,
              const double *__restrict__ R, const double *__restrict__ aux,
@@ -1374,7 +1399,7 @@ k_hyper_zs_d(Geo g, ZMap zm, Phys ph, const double *__restrict__ uin,
     __shared__ double sU[7][HB_NB], sA[4][HB_NB], sR[HZ_NIP][HB_NB], sX[8][HB_NB];
     __shared__ double sRb[HB_NB];
     __shared__ double sDz[ZS_KZMAX], sIz[ZS_KZMAX];
-    hyper_zs_body<double>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRb, sR, sX, sDz, sIz);
+    hyper_zs_body<BGP, double>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRb, sR, sX, sDz, sIz);
 }
 
 // the hyperdiffusive sources of every interior cell into dUh (the stage

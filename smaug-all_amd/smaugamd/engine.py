@@ -26,6 +26,7 @@ EXPORTS = [
     "smaug_time_rk4", "smaug_ncell", "smaug_ninterior", "smaug_copy_probe",
     "smaug_set_panel", "smaug_get_panel", "smaug_time_pass", "smaug_autotune",
     "smaug_set_zstream", "smaug_get_zstream", "smaug_time_passes", "smaug_set_ratio_form",
+    "smaug_set_bg_plane", "smaug_get_bg_plane",
     "smaug_sums", "smaug_ens_sums", "smaug_ens_time_passes",
     "smaug_ens_create", "smaug_ens_destroy", "smaug_ens_nrun", "smaug_ens_ninterior",
     "smaug_ens_set_state", "smaug_ens_get_state", "smaug_ens_rk4", "smaug_ens_time_rk4",
@@ -89,6 +90,8 @@ def load_library(path: str = LIB_PATH):
         lib.smaug_set_zstream.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         lib.smaug_get_zstream.argtypes = [vp, ip, ip]
         lib.smaug_set_ratio_form.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        lib.smaug_set_bg_plane.argtypes = [vp, ctypes.c_int]
+        lib.smaug_get_bg_plane.argtypes = [vp, ip]
         lib.smaug_sums.argtypes = [vp, dp]
         lib.smaug_ens_sums.argtypes = [vp, ctypes.c_int, dp]
         lib.smaug_ens_time_passes.argtypes = [vp, dp, dp, dp]
@@ -239,6 +242,17 @@ class Engine:
         en, kz = ctypes.c_int(), ctypes.c_int()
         self._check(self.lib.smaug_get_zstream(self.h, ctypes.byref(en), ctypes.byref(kz)))
         return en.value, kz.value
+
+    def set_bg_plane(self, enable):
+        """3-D: read a background that is the same on every k-plane from its
+        one-plane copy (default) or always from the full arrays"""
+        self._check(self.lib.smaug_set_bg_plane(self.h, int(enable)))
+
+    def get_bg_plane(self):
+        """True if the kernels read the background from its k-plane copy"""
+        a = ctypes.c_int()
+        self._check(self.lib.smaug_get_bg_plane(self.h, ctypes.byref(a)))
+        return bool(a.value)
 
     def time_pass(self, pass_, reps=5):
         ms = ctypes.c_double()

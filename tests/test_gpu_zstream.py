@@ -1,6 +1,8 @@
 """The z-streamed 3-D stage update (k_zstream) against the per-cell stage
 update and against the FP64 oracle, on tiles and z-chunks that do not divide
 the grid."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -16,6 +18,8 @@ from oracle.oracle import Oracle  # noqa: E402
 # neighbour contributions
 TOL_RHS = {FP64: 1e-12, MIXED: 2e-5}
 TOL_ORACLE_FP64 = 1e-10
+# planar against full background after two RK4 steps (last-place roundings)
+TOL_PLANE = {FP64: 1e-14, MIXED: 1e-6}
 
 
 def strat(nz, nx, ny):
@@ -89,3 +93,55 @@ def test_zstreamed_ratios_match_per_cell(name, setup, kz, prec):
     got = e.rhs(t=1.0)
     e.close()
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("prec", [FP64, MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("name,setup,kz", CASES, ids=[c[0] for c in CASES])
+def test_bg_plane_matches_full_background(name, setup, kz, prec):
+    # a stratified background is the same on every k-plane: the kernels then
+    # read its one-plane copy, with the same arithmetic.  L(w) agrees bit for
+    # bit with the full-background path; after two RK4 steps a few cells may
+    # differ by a last-place rounding (the compiler may contract a
+    # multiply-add differently in the two kernel instances)
+    print(f"[case] bg plane {name} kz={kz} prec={prec}", flush=True)
+    o = Oracle(setup)
+    w0 = o.boundary(perturbed(setup, amp=1e-3, seed=9))
+    dt = 0.5 * configs.hyperviscous_dt(setup)
+    out = []
+    for plane in (True, False):
+        e = Engine(setup, prec)
+        e.set_zstream(True, kz)
+        e.set_bg_plane(plane)
+        e.set_state(w0)
+        assert e.get_bg_plane() == plane
+        r = e.rhs(t=0.5)
+        e.rk4(0.25, dt, 2)
+        out.append((r, e.get_state()))
+        e.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    err = rel_err(out[0][1], out[1][1])
+    assert np.all(err < TOL_PLANE[prec]), err
+
+
+def test_bg_plane_off_for_k_varying_background():
+    # a background that varies along k is read from the full arrays, and the
+    # result still matches the oracle
+    name, setup, kz = CASES[1]
+    print(f"[case] k-varying background {name}", flush=True)
+    bg = setup.bg.copy()
+    n0, n1, n2 = setup.n
+    kk = np.arange(n2).reshape(n2, 1, 1)
+    fac = (1.0 + 1e-3 * np.sin(kk)).repeat(n1, 1).repeat(n0, 2).reshape(-1)
+    bg[1] *= fac  # rho_b
+    s = dataclasses.replace(setup, bg=bg)
+    o = Oracle(s)
+    w0 = o.boundary(perturbed(s, amp=1e-3, seed=10))
+    e = Engine(s, FP64)
+    e.set_zstream(True, kz)
+    e.set_state(w0, bg)
+    assert not e.get_bg_plane()
+    got = e.rhs(t=0.5)
+    e.close()
+    ref = o.rhs(w0, 0.5)
+    err = rel_err(got, ref, interior_mask(s))
+    assert np.all(err < TOL_ORACLE_FP64), err
