@@ -118,7 +118,7 @@ static int launch_cmax(smaug_engine *e, const void *u)
             const unsigned ny = (unsigned)((e->n[2] + PREP_KC - 1) / PREP_KC);
             hipLaunchKernelGGL((k_prep_col<T>), dim3(nx, ny), dim3(256), 0, e->stream, e->geo,
                                e->phys, (const T *)u, (const double *)e->bgp, (T *)e->AUX,
-                               e->d_cmax, PREP_KC);
+                               e->d_cmax, PREP_KC, e->zstream ? 0 : 1);
             HIPCHK(hipGetLastError());
             return 0;
         }

@@ -48,17 +48,21 @@ struct Aux {
     static constexpr int IRHO = 0, PT = 1, EINT = 2, V0 = 3, N = 3 + D;
 };
 
-// aux of cell c (state s loaded) and the running cfast maxima
+// aux of cell c (state s loaded) and the running cfast maxima; full = 0
+// leaves out 1/rho and the pressure perturbation, which only the per-cell
+// update reads
 template <int D, typename T>
 __device__ __forceinline__ void prep_st(const Geo &g, const Phys &ph, const St<D, T> &s,
-                                        T *__restrict__ aux, ci_t c, double *mx)
+                                        T *__restrict__ aux, ci_t c, double *mx, int full = 1)
 {
     typedef Aux<D> AX;
     const size_t nc = (size_t)g.nc;
     const double rho = rho_full(s);
     const double ir = 1.0 / rho;
-    st(aux + AX::IRHO * nc, c, (T)ir);
-    st(aux + AX::PT * nc, c, (T)ptot_pert<D, T>(s, rho, ph.gamma));
+    if (full) {
+        st(aux + AX::IRHO * nc, c, (T)ir);
+        st(aux + AX::PT * nc, c, (T)ptot_pert<D, T>(s, rho, ph.gamma));
+    }
     st(aux + AX::EINT * nc, c, (T)e_int<D, T>(s, rho));
 #pragma unroll
     for (int d = 0; d < D; d++) st(aux + (AX::V0 + d) * nc, c, (T)((double)s.m[d] * ir));
@@ -118,12 +122,12 @@ __global__ __launch_bounds__(256) void k_prep(Geo g, Phys ph, const T *__restric
 // The same pass for a 3-D background that does not vary along k, stored as
 // one plane (bgp, field stride s2).  Thread = one column (i, j) of the
 // plane, block row y = a chunk of kc planes: the column's background is
-// loaded once and only the evolved fields are streamed.
+// loaded once and only the evolved fields are streamed.  full as in prep_st.
 template <typename T>
 __global__ __launch_bounds__(256) void k_prep_col(Geo g, Phys ph, const T *__restrict__ u,
                                                   const double *__restrict__ bgp,
                                                   T *__restrict__ aux, unsigned long long *cm,
-                                                  int kc)
+                                                  int kc, int full)
 {
     __shared__ double red[3][4];
     double mx[3] = {0.0, 0.0, 0.0};
@@ -137,7 +141,7 @@ __global__ __launch_bounds__(256) void k_prep_col(Geo g, Phys ph, const T *__res
         for (int k = ka; k < kb; k++) {
             const ci_t c = p + s2 * (ci_t)k;
             load_pert32<3, T>(s, u, (size_t)g.nc, c);
-            prep_st<3, T>(g, ph, s, aux, c, mx);
+            prep_st<3, T>(g, ph, s, aux, c, mx, full);
         }
     }
     block_max_to<3>(mx, red, cm);
