@@ -108,8 +108,10 @@ static bool bg_planar(const smaug_engine *e) { return e->D == 3 && e->bgp_ok && 
 
 constexpr int PREP_KC = 16; // planes per block of k_prep_col
 
+// aux quantities and cmax of u; fill = 1 (k-plane background only) also
+// fills the ghost cells of u, which the previous stage then left unfilled
 template <int D, typename T>
-static int launch_cmax(smaug_engine *e, const void *u)
+static int launch_cmax(smaug_engine *e, const void *u, int fill = 0)
 {
     hipLaunchKernelGGL(k_reset_cmax, dim3(1), dim3(64), 0, e->stream, e->d_cmax);
     if constexpr (D == 3) {
@@ -117,8 +119,8 @@ static int launch_cmax(smaug_engine *e, const void *u)
             const unsigned nx = (unsigned)((e->geo.s2 + 255) / 256);
             const unsigned ny = (unsigned)((e->n[2] + PREP_KC - 1) / PREP_KC);
             hipLaunchKernelGGL((k_prep_col<T>), dim3(nx, ny), dim3(256), 0, e->stream, e->geo,
-                               e->phys, (const T *)u, (const double *)e->bgp, (T *)e->AUX,
-                               e->d_cmax, PREP_KC, e->zstream ? 0 : 1);
+                               e->phys, (T *)u, (const double *)e->bgp, (T *)e->AUX,
+                               e->d_cmax, PREP_KC, e->zstream ? 0 : 1, fill);
             HIPCHK(hipGetLastError());
             return 0;
         }
@@ -346,14 +348,17 @@ __global__ void k_advance_time(double *tm)
     if (threadIdx.x == 0) tm[0] += tm[1];
 }
 
-// one stage: cmax(uin), ratios(uin), update -> uout, ghost fill of uout
+// one stage: cmax(uin), ratios(uin), update -> uout, ghost fill of uout.
+// With the k-plane background, the outputs of RK stages 0-2 are left
+// unfilled and the aux/cmax pass of the next stage fills their ghosts.
 template <int D, typename T>
 static int stage(smaug_engine *e, StageArgs sa, const void *uin, const void *un, void *uout,
                  hipEvent_t ev0, hipEvent_t ev1, const hipEvent_t *evk = nullptr)
 {
     int rc;
+    const bool defer = D == 3 && bg_planar(e) && sa.mode == 0;
     if (evk) HIPCHK(hipEventRecord(evk[0], e->stream));
-    if ((rc = launch_cmax<D, T>(e, uin))) return rc;
+    if ((rc = launch_cmax<D, T>(e, uin, defer && sa.stage > 0))) return rc;
     if (evk) HIPCHK(hipEventRecord(evk[1], e->stream));
     if (e->phys.hyperdif)
         if ((rc = launch_ratio<D, T>(e, uin))) return rc;
@@ -364,7 +369,7 @@ static int stage(smaug_engine *e, StageArgs sa, const void *uin, const void *un,
     if ((rc = launch_stage_update<D, T>(e, sa, uin, un, uout))) return rc;
     if (ev1) HIPCHK(hipEventRecord(ev1, e->stream));
     if (evk) HIPCHK(hipEventRecord(evk[4], e->stream));
-    if (sa.mode != 2)
+    if (sa.mode != 2 && !(defer && sa.stage < 3))
         if ((rc = launch_bc<D, T>(e, uout))) return rc;
     if (evk) HIPCHK(hipEventRecord(evk[5], e->stream));
     return 0;

@@ -119,34 +119,6 @@ __global__ __launch_bounds__(256) void k_prep(Geo g, Phys ph, const T *__restric
     block_max_to<D>(mx, red, cm);
 }
 
-// The same pass for a 3-D background that does not vary along k, stored as
-// one plane (bgp, field stride s2).  Thread = one column (i, j) of the
-// plane, block row y = a chunk of kc planes: the column's background is
-// loaded once and only the evolved fields are streamed.  full as in prep_st.
-template <typename T>
-__global__ __launch_bounds__(256) void k_prep_col(Geo g, Phys ph, const T *__restrict__ u,
-                                                  const double *__restrict__ bgp,
-                                                  T *__restrict__ aux, unsigned long long *cm,
-                                                  int kc, int full)
-{
-    __shared__ double red[3][4];
-    double mx[3] = {0.0, 0.0, 0.0};
-    const ci_t s2 = (ci_t)g.s2;
-    const ci_t p = blockIdx.x * 256 + threadIdx.x;
-    const int ka = blockIdx.y * kc;
-    const int kb = min(ka + kc, g.n[2]);
-    if (p < s2) {
-        St<3, T> s;
-        load_bg32<3, T>(s, bgp, (size_t)s2, p);
-        for (int k = ka; k < kb; k++) {
-            const ci_t c = p + s2 * (ci_t)k;
-            load_pert32<3, T>(s, u, (size_t)g.nc, c);
-            prep_st<3, T>(g, ph, s, aux, c, mx, full);
-        }
-    }
-    block_max_to<3>(mx, red, cm);
-}
-
 // ---------------------------------------------------------------------------
 // hyperviscosity ratio.  For pair (q, dim) at cell x (along dim) with the
 // pads q[-1] = q[5], q[n] = q[n-6]:
@@ -256,6 +228,55 @@ __device__ __forceinline__ int bc_map(int x, int n, int per)
     if (per) return (x < 2) ? n - 4 + x : 4 - n + x;
     if (x < 2) return 4 - x;
     return (x == n - 1) ? x - 4 : x - 2;
+}
+
+// The aux/cmax pass for a 3-D background that does not vary along k, stored
+// as one plane (bgp, field stride s2).  Thread = one column (i, j) of the
+// plane, block row y = a chunk of kc planes: the column's background is
+// loaded once and only the evolved fields are streamed.  full as in
+// prep_st.  fill = 1 also fills the ghost cells of u first: a ghost cell
+// takes the state of its image (bc_map in every dim, as k_bc does) and its
+// aux and cfast are formed with its own background.  Only ghost cells are
+// written and the images are interior cells, so no thread reads a cell that
+// another writes.
+template <typename T>
+__global__ __launch_bounds__(256) void k_prep_col(Geo g, Phys ph, T *__restrict__ u,
+                                                  const double *__restrict__ bgp,
+                                                  T *__restrict__ aux, unsigned long long *cm,
+                                                  int kc, int full, int fill)
+{
+    __shared__ double red[3][4];
+    double mx[3] = {0.0, 0.0, 0.0};
+    const ci_t s2 = (ci_t)g.s2, s1 = (ci_t)g.s1;
+    const size_t nc = (size_t)g.nc;
+    const ci_t p = blockIdx.x * 256 + threadIdx.x;
+    const int ka = blockIdx.y * kc;
+    const int kb = min(ka + kc, g.n[2]);
+    if (p < s2) {
+        St<3, T> s;
+        load_bg32<3, T>(s, bgp, (size_t)s2, p);
+        const int i = (int)(p % s1), j = (int)(p / s1);
+        const ci_t pimg = (ci_t)bc_map(i, g.n[0], g.per[0]) + s1 * (ci_t)bc_map(j, g.n[1], g.per[1]);
+        for (int k = ka; k < kb; k++) {
+            const ci_t c = p + s2 * (ci_t)k;
+            const int yk = bc_map(k, g.n[2], g.per[2]);
+            // one load site and one arithmetic path for ghost and interior
+            // cells, so that both get the same instruction sequence
+            const bool gh = fill && (pimg != p || yk != k);
+            load_pert32<3, T>(s, u, nc, gh ? pimg + s2 * (ci_t)yk : c);
+            if (gh) {
+                st(u, c, s.r);
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    st(u + (1 + d) * nc, c, s.m[d]);
+                    st(u + (5 + d) * nc, c, s.b[d]);
+                }
+                st(u + 4 * nc, c, s.e);
+            }
+            prep_st<3, T>(g, ph, s, aux, c, mx, full);
+        }
+    }
+    block_max_to<3>(mx, red, cm);
 }
 
 // number of ghost-slab elements of a grid (4 layers per slab, slabs may overlap)
