@@ -1342,7 +1342,9 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                 }
             }
         }
-        __syncthreads(); // the products X_{f,k}(P-1) are in LDS
+        // the products X_{f,k}(P-1) were written to LDS before the barrier
+        // that follows the staging, and are next rewritten after the first
+        // barrier of the next plane, so no barrier is needed here
         if (act && outO) {
             const int xs[2] = {i, j};
 #pragma unroll
