@@ -86,6 +86,8 @@ struct smaug_engine {
     int bgp_on;   // use bgp when bgp_ok (default on)
     double *d_dx[3], *d_idx[3], *d_prof[3];
     unsigned long long *d_cmax;
+    double *d_cpart; // per-block partial cfast maxima of the aux/cmax pass
+    int ncpart;      // its capacity in blocks
     double *d_tm; // [t, dt]
     Geo geo;
     Phys phys;
@@ -108,27 +110,34 @@ static bool bg_planar(const smaug_engine *e) { return e->D == 3 && e->bgp_ok && 
 
 constexpr int PREP_KC = 16; // planes per block of k_prep_col
 
+// blocks of the grid-stride aux/cmax pass
+static idx_t prep_blocks(idx_t nc)
+{
+    idx_t nb = (nc + 255) / 256;
+    return nb > 4096 ? 4096 : nb;
+}
+
 // aux quantities and cmax of u; fill = 1 (k-plane background only) also
 // fills the ghost cells of u, which the previous stage then left unfilled
 template <int D, typename T>
 static int launch_cmax(smaug_engine *e, const void *u, int fill = 0)
 {
-    hipLaunchKernelGGL(k_reset_cmax, dim3(1), dim3(64), 0, e->stream, e->d_cmax);
-    if constexpr (D == 3) {
-        if (bg_planar(e)) {
-            const unsigned nx = (unsigned)((e->geo.s2 + 255) / 256);
-            const unsigned ny = (unsigned)((e->n[2] + PREP_KC - 1) / PREP_KC);
-            hipLaunchKernelGGL((k_prep_col<T>), dim3(nx, ny), dim3(256), 0, e->stream, e->geo,
-                               e->phys, (T *)u, (const double *)e->bgp, (T *)e->AUX,
-                               e->d_cmax, PREP_KC, e->zstream ? 0 : 1, fill);
-            HIPCHK(hipGetLastError());
-            return 0;
-        }
+    unsigned nb;
+    if (D == 3 && bg_planar(e)) {
+        const unsigned nx = (unsigned)((e->geo.s2 + 255) / 256);
+        const unsigned ny = (unsigned)((e->n[2] + PREP_KC - 1) / PREP_KC);
+        nb = nx * ny;
+        hipLaunchKernelGGL((k_prep_col<T>), dim3(nx, ny), dim3(256), 0, e->stream, e->geo,
+                           e->phys, (T *)u, (const double *)e->bgp, (T *)e->AUX, e->d_cpart,
+                           PREP_KC, e->zstream ? 0 : 1, fill);
+    } else {
+        nb = (unsigned)prep_blocks(e->nc);
+        hipLaunchKernelGGL((k_prep<D, T>), dim3(nb), dim3(256), 0, e->stream, e->geo, e->phys,
+                           (const T *)u, (const double *)e->bg, (T *)e->AUX, e->d_cpart);
     }
-    idx_t nb = (e->nc + 255) / 256;
-    if (nb > 4096) nb = 4096;
-    hipLaunchKernelGGL((k_prep<D, T>), dim3((unsigned)nb), dim3(256), 0, e->stream, e->geo,
-                       e->phys, (const T *)u, (const double *)e->bg, (T *)e->AUX, e->d_cmax);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_cmax_reduce<D>, dim3(1), dim3(1024), 0, e->stream,
+                       (const double *)e->d_cpart, (int)nb, e->d_cmax);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -451,6 +460,7 @@ static void free_engine(smaug_engine *e)
     if (e->bg) hipFree(e->bg);
     if (e->bgp) hipFree(e->bgp);
     if (e->d_cmax) hipFree(e->d_cmax);
+    if (e->d_cpart) hipFree(e->d_cpart);
     if (e->d_tm) hipFree(e->d_tm);
     if (e->stream) hipStreamDestroy(e->stream);
     delete e;
@@ -503,6 +513,16 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
     TRYALLOC(e->bg, sizeof(double) * nbg * e->nc);
     if (D == 3) TRYALLOC(e->bgp, sizeof(double) * nbg * (size_t)e->n[0] * e->n[1]);
     TRYALLOC(e->d_cmax, sizeof(unsigned long long) * 4);
+    {
+        idx_t nb = prep_blocks(e->nc);
+        if (D == 3) {
+            const idx_t ncol = ((idx_t)e->n[0] * e->n[1] + 255) / 256 *
+                               ((e->n[2] + PREP_KC - 1) / PREP_KC);
+            if (ncol > nb) nb = ncol;
+        }
+        e->ncpart = (int)nb;
+        TRYALLOC(e->d_cpart, sizeof(double) * 4 * nb);
+    }
     TRYALLOC(e->d_tm, sizeof(double) * 2);
     for (int d = 0; d < 3; d++) {
         int nd = e->n[d];

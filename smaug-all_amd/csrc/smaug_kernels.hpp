@@ -103,11 +103,66 @@ __device__ __forceinline__ void block_max_to(double *mx, double (*red)[4], unsig
     }
 }
 
-// blockDim.x == 256; grid-stride over every cell (ghosts included)
+// block-wide max of mx[] (blockDim.x == 256) written to part[0..D-1]: the
+// single-grid passes leave one partial maximum per block, and k_cmax_reduce
+// folds them.  (One 64-bit atomic max per block on the same few addresses
+// serialises: thousands of short blocks then spend more time in the
+// atomics than in their cells.)
+template <int D>
+__device__ __forceinline__ void block_max_part(double *mx, double (*red)[4], double *part)
+{
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        double v = wave_max(mx[d]);
+        if (lane == 0) red[d][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < D) {
+        int d = threadIdx.x;
+        double v = red[d][0];
+        for (int k = 1; k < 4; k++) v = red[d][k] > v ? red[d][k] : v;
+        part[d] = v;
+    }
+}
+
+// cm[d] = max over the nb partial maxima part[4 b + d] (one block of 1024
+// threads), as the bits of a non-negative double
+template <int D>
+__global__ __launch_bounds__(1024) void k_cmax_reduce(const double *__restrict__ part, int nb,
+                                                      unsigned long long *cm)
+{
+    __shared__ double red[D][16];
+    double mx[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) mx[d] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 1024)
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            const double v = part[4 * b + d];
+            mx[d] = v > mx[d] ? v : mx[d];
+        }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        const double v = wave_max(mx[d]);
+        if (lane == 0) red[d][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < D) {
+        const int d = threadIdx.x;
+        double v = red[d][0];
+        for (int k = 1; k < 16; k++) v = red[d][k] > v ? red[d][k] : v;
+        cm[d] = (unsigned long long)__double_as_longlong(v);
+    }
+}
+
+// blockDim.x == 256; grid-stride over every cell (ghosts included); the
+// block's cfast maxima go to part[4 blockIdx.x + d]
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_prep(Geo g, Phys ph, const T *__restrict__ u,
                                                const double *__restrict__ bg,
-                                               T *__restrict__ aux, unsigned long long *cm)
+                                               T *__restrict__ aux, double *part)
 {
     __shared__ double red[D][4];
     double mx[D];
@@ -116,7 +171,7 @@ __global__ __launch_bounds__(256) void k_prep(Geo g, Phys ph, const T *__restric
     const ci_t nc = (ci_t)g.nc;
     for (ci_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
         prep_cell<D, T>(g, ph, u, bg, aux, c, mx);
-    block_max_to<D>(mx, red, cm);
+    block_max_part<D>(mx, red, part + 4 * blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -238,11 +293,12 @@ __device__ __forceinline__ int bc_map(int x, int n, int per)
 // takes the state of its image (bc_map in every dim, as k_bc does) and its
 // aux and cfast are formed with its own background.  Only ghost cells are
 // written and the images are interior cells, so no thread reads a cell that
-// another writes.
+// another writes.  The block's cfast maxima go to part[4 b + d], b = the
+// linear block index.
 template <typename T>
 __global__ __launch_bounds__(256) void k_prep_col(Geo g, Phys ph, T *__restrict__ u,
                                                   const double *__restrict__ bgp,
-                                                  T *__restrict__ aux, unsigned long long *cm,
+                                                  T *__restrict__ aux, double *part,
                                                   int kc, int full, int fill)
 {
     __shared__ double red[3][4];
@@ -276,7 +332,7 @@ __global__ __launch_bounds__(256) void k_prep_col(Geo g, Phys ph, T *__restrict_
             prep_st<3, T>(g, ph, s, aux, c, mx, full);
         }
     }
-    block_max_to<3>(mx, red, cm);
+    block_max_part<3>(mx, red, part + 4 * (blockIdx.x + gridDim.x * blockIdx.y));
 }
 
 // number of ghost-slab elements of a grid (4 layers per slab, slabs may overlap)
