@@ -27,11 +27,42 @@ using namespace smaug;
 
 namespace {
 
-constexpr int PREP_CELLS = 1024; // cells per prep item (4 per thread)
-constexpr int BOX_CELLS = 512;   // ratio / update cells per item (2 per thread)
-constexpr int BC_CELLS = 1024;   // ghost-slab elements per item (4 per thread)
+// Work-item sizes.  An item costs a queue atomic, a block-wide pull and the
+// run's descriptor loads, so items are several cells per thread; the update
+// items stay smaller, as the heaviest pass needs the finer balance.
+constexpr int PREP_CELLS = 4096;   // cells per prep item (16 per thread)
+constexpr int RATIO_CELLS = 1024;  // ratio-box cells per ratio item (4 per thread)
+constexpr int UPDATE_CELLS = 512;  // interior cells per update item (2 per thread)
+constexpr int BC_CELLS = 1024;     // ghost-slab elements per item (4 per thread)
+
+// t / d for t < 2^31 as (t * m) >> s with s = 31 + ceil(log2 d) and
+// m = floor(2^s / d) + 1: the error of the reciprocal, m d - 2^s <= d <=
+// 2^(s-31), keeps t (m d - 2^s) < 2^s, so the quotient is exact
+struct FastDiv {
+    unsigned long long m;
+    int s;
+};
+
+static FastDiv make_fastdiv(unsigned d)
+{
+    int c = 0;
+    while ((1ull << c) < d) c++;
+    FastDiv f;
+    f.s = 31 + c;
+    f.m = (unsigned long long)((((unsigned __int128)1) << f.s) / d) + 1;
+    return f;
+}
+
+__device__ __forceinline__ unsigned fdiv(unsigned t, const FastDiv &f)
+{
+    return (unsigned)(((unsigned long long)t * f.m) >> f.s);
+}
 constexpr int NPASS = 4;         // prep, ratio, update, ghost fill
 constexpr int NQ = 8;            // queue slices, one per XCD
+// slice counters lie 4 KiB apart: atomics on one cache line (or one memory
+// channel) are served one after another, and every pull of a pass, as well
+// as every block's probes of drained slices at its end, is such an atomic
+constexpr int CSTRIDE = 1024;
 
 template <typename T>
 struct RunDesc {
@@ -42,13 +73,14 @@ struct RunDesc {
     unsigned long long *cm; // [4 stages][4] cmax bit patterns
     double *tm;             // [t, dt]
     int ext[2][3];          // extents of the ratio box [1, n-3] (0) and the interior (1)
+    FastDiv dv[2][2];       // division by ext[b][0] and ext[b][1]
 };
 
 struct Queue {
     const int *base;   // nrun + 1 prefix sums of the items of each run
     const int *run;    // the run of every item
     int nrun, nitems;
-    unsigned *ctr;     // NQ slice counters
+    unsigned *ctr;     // NQ slice counters, CSTRIDE apart
 };
 
 // next item for the calling block (thread 0 only), -1 when every slice is
@@ -59,7 +91,7 @@ __device__ __forceinline__ int pull(const Queue &q, int xcd, int &v)
         const int s = (xcd + v) & (NQ - 1);
         const long long lo = ((long long)q.nitems * s) / NQ;
         const long long hi = ((long long)q.nitems * (s + 1)) / NQ;
-        const unsigned k = atomicAdd(&q.ctr[s], 1u);
+        const unsigned k = atomicAdd(&q.ctr[(size_t)s * CSTRIDE], 1u);
         if (lo + (long long)k < hi) return (int)(lo + k);
         v++;
     }
@@ -94,14 +126,14 @@ __device__ __forceinline__ int block_pull(const Queue &q, int &v, int *s_it, int
 }
 
 // cell t (linear, i fastest) of a box of extents ext starting at lo; a run
-// has fewer than 2^31 cells, so 32-bit division suffices
-__device__ __forceinline__ void box_cell(unsigned t, const int *ext, int lo, int &x0, int &x1,
-                                         int &x2)
+// has fewer than 2^31 cells, so the quotients fit the fast division
+__device__ __forceinline__ void box_cell(unsigned t, const int *ext, const FastDiv *dv, int lo,
+                                         int &x0, int &x1, int &x2)
 {
     const unsigned e0 = (unsigned)ext[0], e1 = (unsigned)ext[1];
-    const unsigned r = t / e0;
+    const unsigned r = fdiv(t, dv[0]);
     x0 = lo + (int)(t - r * e0);
-    const unsigned k = r / e1;
+    const unsigned k = fdiv(r, dv[1]);
     x1 = lo + (int)(r - k * e1);
     x2 = lo + (int)k;
 }
@@ -111,7 +143,7 @@ __global__ void ke_reset(unsigned *ctr, int nctr, unsigned long long *cm, int nc
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nctr + ncm;
          i += gridDim.x * blockDim.x) {
         if (i < nctr)
-            ctr[i] = 0u;
+            ctr[(size_t)i * CSTRIDE] = 0u;
         else
             cm[i - nctr] = 0ull;
     }
@@ -131,20 +163,30 @@ __global__ __launch_bounds__(256) void ke_prep(const RunDesc<T> *__restrict__ ru
     __shared__ double red[3][4];
     int v = 0;
     if (threadIdx.x == 0) s_it[1] = -2;
+    // the block's cfast maxima are kept over consecutive items of one run
+    // and folded into the run's cmax (one atomic per dim) when the run
+    // changes and at the end
+    int prev = -1;
+    double mx[3] = {0.0, 0.0, 0.0};
     for (;;) {
         int r;
         const int it = block_pull(q, v, s_it, r);
         if (it < 0) break;
+        if (r != prev) {
+            if (prev >= 0) block_max_to<D>(mx, red, runs[prev].cm + 4 * stage);
+#pragma unroll
+            for (int d = 0; d < 3; d++) mx[d] = 0.0;
+            prev = r;
+        }
         const RunDesc<T> &rd = runs[r];
         const idx_t c0 = (idx_t)(it - q.base[r]) * PREP_CELLS;
-        double mx[3] = {0.0, 0.0, 0.0};
-#pragma unroll
+#pragma unroll 4
         for (int k = 0; k < PREP_CELLS / 256; k++) {
             const idx_t c = c0 + k * 256 + threadIdx.x;
             if (c < rd.geo.nc) prep_cell<D, T>(rd.geo, ph, rd.U[src], rd.bg, rd.AUX, c, mx);
         }
-        block_max_to<D>(mx, red, rd.cm + 4 * stage);
     }
+    if (prev >= 0) block_max_to<D>(mx, red, runs[prev].cm + 4 * stage);
 }
 
 template <int D, typename T>
@@ -160,12 +202,12 @@ __global__ __launch_bounds__(256) void ke_ratio(const RunDesc<T> *__restrict__ r
         if (it < 0) break;
         const RunDesc<T> &rd = runs[r];
         const unsigned nb = (unsigned)rd.ext[0][0] * rd.ext[0][1] * rd.ext[0][2];
-#pragma unroll
-        for (int k = 0; k < BOX_CELLS / 256; k++) {
-            const unsigned t = (unsigned)(it - q.base[r]) * BOX_CELLS + k * 256 + threadIdx.x;
+#pragma unroll 2
+        for (int k = 0; k < RATIO_CELLS / 256; k++) {
+            const unsigned t = (unsigned)(it - q.base[r]) * RATIO_CELLS + k * 256 + threadIdx.x;
             if (t >= nb) break;
             int x0, x1, x2;
-            box_cell(t, rd.ext[0], 1, x0, x1, x2);
+            box_cell(t, rd.ext[0], rd.dv[0], 1, x0, x1, x2);
             ratio_cell<D, T>(rd.geo, rd.U[src], rd.AUX, rd.R, x0, x1, D == 3 ? x2 : 0);
         }
     }
@@ -185,11 +227,11 @@ __global__ __launch_bounds__(256) void ke_update(const RunDesc<T> *__restrict__ 
         const RunDesc<T> &rd = runs[r];
         const unsigned nb = (unsigned)rd.ext[1][0] * rd.ext[1][1] * rd.ext[1][2];
 #pragma unroll
-        for (int k = 0; k < BOX_CELLS / 256; k++) {
-            const unsigned t = (unsigned)(it - q.base[r]) * BOX_CELLS + k * 256 + threadIdx.x;
+        for (int k = 0; k < UPDATE_CELLS / 256; k++) {
+            const unsigned t = (unsigned)(it - q.base[r]) * UPDATE_CELLS + k * 256 + threadIdx.x;
             if (t >= nb) break;
             int x0, x1, x2;
-            box_cell(t, rd.ext[1], 2, x0, x1, x2);
+            box_cell(t, rd.ext[1], rd.dv[1], 2, x0, x1, x2);
             if (D != 3) x2 = 0;
             StageArgs sa;
             sa.mode = 0;
@@ -214,7 +256,7 @@ __global__ __launch_bounds__(256) void ke_bc(const RunDesc<T> *__restrict__ runs
         if (it < 0) break;
         const RunDesc<T> &rd = runs[r];
         const idx_t nbc = bc_count<D>(rd.geo.n[0], rd.geo.n[1], rd.geo.n[2]);
-#pragma unroll
+#pragma unroll 4
         for (int k = 0; k < BC_CELLS / 256; k++) {
             const idx_t t = (idx_t)(it - q.base[r]) * BC_CELLS + k * 256 + threadIdx.x;
             if (t < nbc) bc_elem<D, T>(rd.geo, rd.U[dst], t);
@@ -319,13 +361,15 @@ static int setup_typed(smaug_ensemble *e)
             rd.ext[0][d] = d < D ? g.n[d] - 3 : 1;
             rd.ext[1][d] = d < D ? g.n[d] - 4 : 1;
         }
+        for (int b = 0; b < 2; b++)
+            for (int d = 0; d < 2; d++) rd.dv[b][d] = make_fastdiv((unsigned)rd.ext[b][d]);
         const long long nbox[2] = {(long long)rd.ext[0][0] * rd.ext[0][1] * rd.ext[0][2],
                                    (long long)rd.ext[1][0] * rd.ext[1][1] * rd.ext[1][2]};
         if (g.nc >= (1ll << 31)) return fail(SMAUG_EINVAL, "run %d: too many cells for a member", r);
         long long items[NPASS] = {
             (long long)((g.nc + PREP_CELLS - 1) / PREP_CELLS),
-            e->phys.hyperdif ? (nbox[0] + BOX_CELLS - 1) / BOX_CELLS : 0,
-            (nbox[1] + BOX_CELLS - 1) / BOX_CELLS,
+            e->phys.hyperdif ? (nbox[0] + RATIO_CELLS - 1) / RATIO_CELLS : 0,
+            (nbox[1] + UPDATE_CELLS - 1) / UPDATE_CELLS,
             (long long)((bc_count<D>(g.n[0], g.n[1], n2) + BC_CELLS - 1) / BC_CELLS)};
         for (int k = 0; k < NPASS; k++) {
             long long nb = (long long)base[k][r] + items[k];
@@ -432,7 +476,7 @@ extern "C" int smaug_ens_create(const smaug_cfg *cfgs, int nrun, smaug_ensemble 
     ECHK(hipMalloc((void **)&e->axes, sizeof(double) * sx));
     ECHK(hipMalloc((void **)&e->tm, sizeof(double) * 2 * nrun));
     ECHK(hipMalloc((void **)&e->cm, sizeof(unsigned long long) * 16 * nrun));
-    ECHK(hipMalloc((void **)&e->ctr, sizeof(unsigned) * 4 * NPASS * NQ));
+    ECHK(hipMalloc((void **)&e->ctr, sizeof(unsigned) * 4 * NPASS * NQ * CSTRIDE));
     ECHK(hipMemset(e->state, 0, su));
     ECHK(hipMemset(e->tm, 0, sizeof(double) * 2 * nrun));
 
@@ -558,7 +602,7 @@ static Queue queue(smaug_ensemble *e, int pass, int launch)
     q.run = e->irun[pass];
     q.nrun = e->nrun;
     q.nitems = e->nitems[pass];
-    q.ctr = e->ctr + (size_t)launch * NQ;
+    q.ctr = e->ctr + (size_t)launch * NQ * CSTRIDE;
     return q;
 }
 
