@@ -85,6 +85,8 @@ static Geo make_geo(const struct params *p)
     g.s1 = g_ctx.n[0];
     g.s2 = (idx_t)g_ctx.n[0] * g_ctx.n[1];
     g.nc = g_ctx.dimp;
+    g.bgf = g.nc;
+    g.bgm = ~0u;
     return g;
 }
 

@@ -277,6 +277,7 @@ struct smaug_ensemble {
     char *state;                   // U0, U1, U2, ACC of every run
     char *ratio, *aux;
     double *bg, *axes, *tm;
+    double *bgu;                   // nrun x 8: a run's background values when uniform
     unsigned long long *cm;        // nrun x 16
     unsigned *ctr;                 // 16 launches x NQ
     void *descs;                   // RunDesc<T>[nrun]
@@ -295,7 +296,7 @@ static void free_ens(smaug_ensemble *e)
     if (!e) return;
     if (e->gvalid) hipGraphExecDestroy(e->gexec);
     void *ptrs[] = {e->state, e->ratio, e->aux, e->bg, e->axes, e->tm, e->cm, e->ctr, e->descs,
-                    e->d_part};
+                    e->d_part, e->bgu};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (int k = 0; k < NPASS; k++) {
@@ -431,7 +432,7 @@ extern "C" int smaug_ens_create(const smaug_cfg *cfgs, int nrun, smaug_ensemble 
     e->tsz = e->prec == SMAUG_FP64 ? sizeof(double) : sizeof(float);
     e->cfg.assign(cfgs, cfgs + nrun);
     e->state = e->ratio = e->aux = nullptr;
-    e->bg = e->axes = e->tm = nullptr;
+    e->bg = e->axes = e->tm = e->bgu = nullptr;
     e->cm = nullptr;
     e->ctr = nullptr;
     e->descs = nullptr;
@@ -477,6 +478,7 @@ extern "C" int smaug_ens_create(const smaug_cfg *cfgs, int nrun, smaug_ensemble 
     ECHK(hipMalloc((void **)&e->tm, sizeof(double) * 2 * nrun));
     ECHK(hipMalloc((void **)&e->cm, sizeof(unsigned long long) * 16 * nrun));
     ECHK(hipMalloc((void **)&e->ctr, sizeof(unsigned) * 4 * NPASS * NQ * CSTRIDE));
+    ECHK(hipMalloc((void **)&e->bgu, sizeof(double) * 8 * nrun));
     ECHK(hipMemset(e->state, 0, su));
     ECHK(hipMemset(e->tm, 0, sizeof(double) * 2 * nrun));
 
@@ -504,6 +506,8 @@ extern "C" int smaug_ens_create(const smaug_cfg *cfgs, int nrun, smaug_ensemble 
         g.s1 = g.n[0];
         g.s2 = (idx_t)g.n[0] * g.n[1];
         g.nc = e->nc[r];
+        g.bgf = g.nc;
+        g.bgm = ~0u;
         e->geo.push_back(g);
     }
     ECHK(hipMemcpy(e->axes, host.data(), sizeof(double) * pos, hipMemcpyHostToDevice));
@@ -556,6 +560,35 @@ static int ens_set_state(smaug_ensemble *e, int r, const double *w, const double
     HIPCHK(hipMemcpyAsync(u0, b.data(), sizeof(T) * b.size(), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync((char *)e->bg + e->off_bg[r], bg, sizeof(double) * nbg * nc,
                           hipMemcpyHostToDevice, e->stream));
+    // a background that is the same in every cell is read by the per-cell
+    // kernels as its nbg values (Geo::bgf = 1, bgm = 0), else in full; the
+    // run's descriptor is rewritten (the captured step reads it at run time)
+    {
+        int uni = 1;
+        for (int f = 0; f < nbg && uni; f++)
+            for (idx_t c = 1; c < nc; c++)
+                if (memcmp(&bg[(size_t)f * nc + c], &bg[(size_t)f * nc], sizeof(double))) {
+                    uni = 0;
+                    break;
+                }
+        RunDesc<T> rd;
+        RunDesc<T> *dd = (RunDesc<T> *)e->descs + r;
+        HIPCHK(hipStreamSynchronize(e->stream));
+        HIPCHK(hipMemcpy(&rd, dd, sizeof(rd), hipMemcpyDeviceToHost));
+        if (uni) {
+            double v[8];
+            for (int f = 0; f < nbg; f++) v[f] = bg[(size_t)f * nc];
+            HIPCHK(hipMemcpy(e->bgu + 8 * r, v, sizeof(double) * nbg, hipMemcpyHostToDevice));
+            rd.bg = e->bgu + 8 * r;
+            rd.geo.bgf = 1;
+            rd.geo.bgm = 0u;
+        } else {
+            rd.bg = (const double *)((char *)e->bg + e->off_bg[r]);
+            rd.geo.bgf = rd.geo.nc;
+            rd.geo.bgm = ~0u;
+        }
+        HIPCHK(hipMemcpy(dd, &rd, sizeof(rd), hipMemcpyHostToDevice));
+    }
     const Geo &g = e->geo[r];
     const idx_t tot = bc_count<D>(g.n[0], g.n[1], g.n[2]);
     hipLaunchKernelGGL((k_bc<D, T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, e->stream,

@@ -79,7 +79,8 @@ __device__ __forceinline__ void prep_cell(const Geo &g, const Phys &ph, const T 
                                           ci_t c, double *mx)
 {
     St<D, T> s;
-    load_st32<D, T>(s, u, bg, (size_t)g.nc, c);
+    load_pert32<D, T>(s, u, (size_t)g.nc, c);
+    load_bg32<D, T>(s, bg, (size_t)g.bgf, c & g.bgm);
     prep_st<D, T>(g, ph, s, aux, c, mx);
 }
 
@@ -480,6 +481,8 @@ struct HyGlobal {
     const double *__restrict__ bg;
     const T *__restrict__ R;
     size_t nc;
+    size_t bgf; // background field stride and cell mask (Geo::bgf, bgm)
+    ci_t bgm;
     ci_t c, s1, s2;
     __device__ __forceinline__ ci_t at(int di, int dj, int dk) const
     {
@@ -495,7 +498,7 @@ struct HyGlobal {
     }
     __device__ __forceinline__ double rhob(int di, int dj, int dk) const
     {
-        return ld(bg + nc, at(di, dj, dk));
+        return ld(bg + bgf, at(di, dj, dk) & bgm);
     }
     __device__ __forceinline__ T Rp(int p, int di, int dj, int dk) const
     {
@@ -653,6 +656,8 @@ __device__ __forceinline__ void hyper_cell(const Geo &g, const Phys &ph, const T
     vw.bg = bg;
     vw.R = R;
     vw.nc = (size_t)g.nc;
+    vw.bgf = (size_t)g.bgf;
+    vw.bgm = g.bgm;
     vw.s1 = (ci_t)g.s1;
     vw.s2 = (ci_t)g.s2;
     vw.c = (ci_t)x0 + vw.s1 * (ci_t)x1 + vw.s2 * (ci_t)x2;
@@ -758,7 +763,7 @@ __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const 
     // field and aux accessors
     auto U = [&](int f, ci_t cc) -> T { return ld(uin + f * nc, cc); };
     auto A = [&](int a, ci_t cc) -> T { return ld(aux + a * nc, cc); };
-    auto BG = [&](int f, ci_t cc) -> double { return ld(bg + f * nc, cc); };
+    auto BG = [&](int f, ci_t cc) -> double { return ld(bg + f * (size_t)g.bgf, cc & g.bgm); };
 
     T dU[LY::NVE];
 #pragma unroll

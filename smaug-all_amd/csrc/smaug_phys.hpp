@@ -20,6 +20,12 @@ struct Geo {
     int n[3];         // cells incl. ghosts per dim (n[2] == 1 in 2-D)
     int per[3];       // 1 if periodic along d (boundtype 0)
     idx_t s1, s2, nc; // strides of dims 1 and 2, cell count
+    // background as the per-cell kernels read it: field f of cell c at
+    // bg[f * bgf + (c & bgm)]; a full array has bgf = nc, bgm = ~0, a
+    // background that is the same in every cell is passed as its nb
+    // values with bgf = 1, bgm = 0
+    idx_t bgf;
+    unsigned bgm;
     const double *dx[3];
     const double *idx[3];  // 1/dx per cell
     const double *prof[3]; // driver profile factors per dim

@@ -773,6 +773,8 @@ __device__ __forceinline__ void hyper_lds_body(const Geo &g, const TileMap &tm, 
     vw.gm.bg = bg;
     vw.gm.R = R;
     vw.gm.nc = nc;
+    vw.gm.bgf = nc; // this kernel reads the full background
+    vw.gm.bgm = ~0u;
     vw.gm.s1 = s1;
     vw.gm.s2 = s2;
     vw.gm.c = (ci_t)i + s1 * (ci_t)j + pc;
