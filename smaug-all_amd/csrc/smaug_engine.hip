@@ -188,7 +188,7 @@ static TileMap make_map(int e0, int e1, int e2, int tpx, int tpy, unsigned *nblo
 static ZMap make_rmap(const smaug_engine *e, unsigned *nblocks)
 {
     ZMap m;
-    m.ntx = (e->n[0] - 3 + RZ_TI - 1) / RZ_TI;
+    m.ntx = (e->n[0] + RZ_TI - 1) / RZ_TI;
     m.nty = (e->n[1] - 3 + RZ_TJ - 1) / RZ_TJ;
     const int nk = e->n[2] - 3;
     m.kz = e->kzr > 0 ? (e->kzr < nk ? e->kzr : nk) : (nk < 32 ? nk : 32);

@@ -501,7 +501,9 @@ k_zstream_d(Geo g, Phys ph, StageArgs sa, ZMap zm, const double *__restrict__ ui
 
 // ---------------------------------------------------------------------------
 // Hyperviscosity ratios, z-streamed.  A block owns 64 x 8 columns of the box
-// [1, n-3] and walks a chunk of its planes.  Each plane's window quantities
+// [1, n-3] and walks a chunk of its planes.  Along i the blocks cover whole
+// rows [0, n-1]: the ratios at i = 0, n-2, n-1 are never read, but writing
+// them keeps the 21 output rows free of partly written cache lines.  Each plane's window quantities
 // (rho', e_int, v, b') over the tile plus the window halo (2 cells below, 3
 // above, pads mirrored as in ratio_cell) are staged in LDS once; the ratios
 // along i and j read their 6-point windows there.  Along k every thread
@@ -531,9 +533,9 @@ __device__ __forceinline__ void ratio_zs_body(const Geo &g, const ZMap &zm,
     const size_t nc = (size_t)g.nc;
     const ci_t s1 = (ci_t)g.s1, s2 = (ci_t)g.s2;
     const int ti = threadIdx.x % RZ_TI, tj = threadIdx.x / RZ_TI;
-    const int i0 = 1 + tx * RZ_TI, j0 = 1 + ty * RZ_TJ;
+    const int i0 = tx * RZ_TI, j0 = 1 + ty * RZ_TJ;
     const int i = i0 + ti, j = j0 + tj;
-    const bool act = i <= n0 - 3 && j <= n1 - 3;
+    const bool act = i <= n0 - 1 && j <= n1 - 3;
     const int k0 = 1 + kc * zm.kz;
     const int k1 = min(k0 + zm.kz, n2 - 2);
     const int own = (tj + 2) * RB_BI + ti + 2;
