@@ -65,6 +65,8 @@ def parse():
                          "autotuner (FORM as in smaug_set_zstream: 0 per-cell, 1 z-streamed, "
                          "3 z-streamed with LDS-plane sources), e.g. for profiling runs")
     ap.add_argument("--ens-steps", type=int, default=200)
+    ap.add_argument("--ot-steps", type=int, default=500,
+                    help="RK4 steps of the Orszag-Tang leg (500 is the workload's count)")
     return ap.parse_args()
 
 
@@ -357,7 +359,7 @@ def main():
         print(f"[bench] ensemble: {ens['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     ot = None
     if not a.no_ot:
-        ot = orszag_tang_leg()
+        ot = orszag_tang_leg(a.ot_steps)
         print(f"[bench] orszag-tang: {ot['mixed']['ms_per_step']:.3f} ms/step", file=sys.stderr,
               flush=True)
     cpu = None if a.no_cpu else cpu_baseline(n)
