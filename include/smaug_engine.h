@@ -37,12 +37,14 @@ typedef struct {
     int n[3];             /* cells per dim incl. ghosts, each >= 8; n[2] ignored for ndim 2 */
     int bc[3];            /* boundtype per dim: 0 periodic, 4 contcd4 */
     int hyperdif;         /* p->hyperdifmom */
-    int driver;           /* 0 none, 1 vertical Gaussian velocity driver */
+    int driver;           /* 0 none, 1 vertical Gaussian velocity driver,
+                             2 the torsional driver of usersource.cu (3-D) */
     int precision;        /* SMAUG_FP64 or SMAUG_MIXED */
     double gamma;
     double g[3];
     double chyp[13];      /* indexed by the reference field enum */
-    double drv[8];        /* A, period, sigma_h, z0, sigma_z, xc, yc */
+    double drv[8];        /* A, period, sigma_h, z0, sigma_z, xc, yc; for driver 2
+                             also L, the normalising length of the lever arm */
     const double *dx[3];  /* cell metric per dim, length n[d] (copied) */
     const double *pos[3]; /* cell centres per dim, length n[d] (copied) */
 } smaug_cfg;

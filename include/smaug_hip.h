@@ -175,13 +175,19 @@ int cusync(struct params **p);
  * slot 0 with the fused stage kernels, replayed from a hipGraph.  precision
  * 0 = FP64, 1 = FP32 perturbations / FP64 background, pressure and cmax.
  * Slot 0 and slot 1 hold the result; p->qt advances by nsteps*p->dt.
- * The user source is the vertical velocity driver configured by
- * cusetdriver (off by default).
+ * The user source is the driver configured by cusetdriver (off by
+ * default).
  */
 int cuadvancerk4(struct params **p, struct params **d_p, real **d_wmod, int nsteps,
                  int precision);
 /* driver parameters: amplitude, period, horizontal width, height, vertical
- * width, horizontal centre (dims 1 and 2); enable = 0 switches it off */
+ * width, horizontal centre (dims 1 and 2); enable = 0 switches it off,
+ * 1 selects the vertical Gaussian driver, 2 (3-D only) the reference's
+ * torsional driver (usersource.cu:4-86: vx = (A yp/L) G sin(2 pi qt/P)
+ * along dim 1, vy = -(A xp/L) G sin(2 pi qt/P) along dim 2, L = 4 Mm).
+ * With enable = 2 and drv = NULL the reference's constants are used
+ * (A = 500, P = 30, widths 0.1 Mm, height 0.1 Mm, vertical width
+ * 0.05 Mm, centre 2 Mm, 2 Mm). */
 int cusetdriver(int enable, const real drv[7]);
 /* message of the last failure in this thread */
 const char *cuerrorstring(void);

@@ -720,6 +720,36 @@ void sm_rhs(const sm_grid *g, const double *w, const double *bg, double t,
             rhs[F_EN(D) * nc + c] += (v * v) * r / 2.0;
         }
     }
+    /* user source: the torsional driver of the reference
+     * (dev/smaug/src/usersource.cu:4-86): horizontal velocities
+     * vx = (A yp/L) G tdep along dim 1 and vy = -(A xp/L) G tdep along dim 2,
+     * G = exp(-xp^2/sh^2) exp(-yp^2/sh^2) exp(-zp^2/sz^2), with xp, yp, zp the
+     * offsets from (xc, yc, z0) along dims 1, 2, 0; m_1 += dt vx rho,
+     * m_2 += dt vy rho, e += dt (vx^2 + vy^2) rho / 2 */
+    if (g->driver == 2 && D == 3) {
+        const double PI2 = 6.283185307179586;
+        double A = g->drv[0], P = g->drv[1], sh = g->drv[2], z0 = g->drv[3], sz = g->drv[4],
+               xc = g->drv[5], yc = g->drv[6], L = g->drv[7];
+        double tdep = sin(t * PI2 / P);
+        FORALL(G)
+        {
+            if (!interior(&G, i_, j_, k_)) continue;
+            long c = CIDX(G);
+            double xp = g->pos[1][j_] - xc;
+            double zp = g->pos[0][i_] - z0;
+            double yp = g->pos[2][k_] - yc;
+            double ez = exp(-zp * zp / (sz * sz));
+            double ex = exp(-xp * xp / (sh * sh));
+            double ey = exp(-yp * yp / (sh * sh));
+            double exyz = ex * ey * ez;
+            double vx = (A * yp / L) * exyz * tdep;
+            double vy = -(A * xp / L) * exyz * tdep;
+            double r = rp[c] + rbp[c];
+            rhs[F_MOM(D, 1) * nc + c] += vx * r;
+            rhs[F_MOM(D, 2) * nc + c] += vy * r;
+            rhs[F_EN(D) * nc + c] += (vx * vx + vy * vy) * r / 2.0;
+        }
+    }
 
     free(rho);
     free(pt);

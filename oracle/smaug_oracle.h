@@ -24,11 +24,13 @@ typedef struct {
     int n[3];            /* cells per dim incl. ghosts; n[2] == 1 when ndim == 2 */
     int bc[3];           /* per dim: 0 periodic, 4 contcd4 mirror (iosmaugparams boundtype) */
     int hyperdif;        /* p->hyperdifmom */
-    int driver;          /* 0 none, 1 vertical Gaussian velocity driver */
+    int driver;          /* 0 none, 1 vertical Gaussian velocity driver,
+                            2 torsional driver of usersource.cu (3-D) */
     double gamma;
     double g[3];
     double chyp[13];     /* indexed by the reference field enum (CEV) */
-    double drv[8];       /* A, period, sigma_h, z0, sigma_z, xc (dim 1), yc (dim 2) */
+    double drv[8];       /* A, period, sigma_h, z0, sigma_z, xc (dim 1), yc (dim 2),
+                            and for driver 2 the normalising length L */
     const double *dx[3]; /* cell metric per dim, length n[d]  (wd[delx1..3]) */
     const double *pos[3];/* cell centre per dim, length n[d]  (wd[pos1..3])  */
 } sm_grid;

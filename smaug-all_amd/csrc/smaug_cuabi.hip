@@ -64,6 +64,7 @@ struct Ctx {
     double *d_dx[3] = {nullptr, nullptr, nullptr};
     double *d_idx[3] = {nullptr, nullptr, nullptr};
     double *d_prof[3] = {nullptr, nullptr, nullptr};
+    double *d_profq[3] = {nullptr, nullptr, nullptr};
     unsigned long long *d_red = nullptr;
     smaug_engine *eng[2] = {nullptr, nullptr};
     int driver = 0;
@@ -81,6 +82,7 @@ static Geo make_geo(const struct params *p)
         g.dx[d] = g_ctx.d_dx[d];
         g.idx[d] = g_ctx.d_idx[d];
         g.prof[d] = g_ctx.d_prof[d];
+        g.profq[d] = g_ctx.d_profq[d];
     }
     g.s1 = g_ctx.n[0];
     g.s2 = (idx_t)g_ctx.n[0] * g_ctx.n[1];
@@ -346,16 +348,30 @@ __global__ void kc_grav(Geo g, Phys ph, const double *__restrict__ win,
         wout[(idx_t)LY::EN * nc + c] += pdt * r * (ph.g[d] * win[(idx_t)(LY::M0 + d) * nc + c]) / (r + rb);
 }
 
-// vertical velocity driver at time qt (source_parallel, centdiff2.cu:785-867)
+// user driver at time qt (source_parallel, centdiff2.cu:785-867): the
+// vertical Gaussian driver (1) or the torsional one of usersource.cu:4-86
+// (2: vx = (A yp/L) G tdep along dim 1, vy = -(A xp/L) G tdep along dim 2)
 __global__ void kc_source(Geo g, Phys ph, const double *__restrict__ win,
                           double *__restrict__ wout, double pdt, double qt)
 {
     CELL_INT();
     const idx_t nc = g.nc;
-    double pr = ph.drv[0] * sin(qt * 6.283185307179586 / ph.drv[1]) * g.prof[0][x0] * g.prof[1][x1];
-    if (D == 3) pr *= g.prof[2][x2];
     (void)xs;
     const double rho = win[c] + win[(idx_t)(LY::NVE + 1) * nc + c];
+    const double tdep = sin(qt * 6.283185307179586 / ph.drv[1]);
+    if (ph.driver == 2) {
+        if (D == 3) {
+            const double a = ph.drv[0] * tdep / ph.drv[7] * g.prof[0][x0];
+            const double vx = a * g.prof[1][x1] * g.profq[2][x2];
+            const double vy = -(a * g.profq[1][x1] * g.prof[2][x2]);
+            wout[(idx_t)(LY::M0 + 1) * nc + c] += pdt * vx * rho;
+            wout[(idx_t)(LY::M0 + D - 1) * nc + c] += pdt * vy * rho;
+            wout[(idx_t)LY::EN * nc + c] += pdt * (vx * vx + vy * vy) * rho / 2.0;
+        }
+        return;
+    }
+    double pr = ph.drv[0] * tdep * g.prof[0][x0] * g.prof[1][x1];
+    if (D == 3) pr *= g.prof[2][x2];
     wout[(idx_t)LY::M0 * nc + c] += pdt * pr * rho;
     wout[(idx_t)LY::EN * nc + c] += pdt * (pr * pr) * rho / 2.0;
 }
@@ -526,7 +542,7 @@ static int upload_axes(const struct params *p)
 {
     for (int d = 0; d < 3; d++) {
         int nd = g_ctx.n[d];
-        std::vector<double> dx(nd, 1.0), idx(nd, 1.0), prof(nd, 1.0);
+        std::vector<double> dx(nd, 1.0), idx(nd, 1.0), prof(nd, 1.0), profq(nd, 0.0);
         if (d < D) {
             for (int i = 0; i < nd; i++) {
                 dx[i] = g_ctx.dx[d][i];
@@ -536,6 +552,7 @@ static int upload_axes(const struct params *p)
                     double c0 = d == 0 ? g_ctx.drv[3] : (d == 1 ? g_ctx.drv[5] : g_ctx.drv[6]);
                     double s = d == 0 ? g_ctx.drv[4] : g_ctx.drv[2];
                     prof[i] = exp(-(pp - c0) * (pp - c0) / (s * s));
+                    if (d > 0) profq[i] = (pp - c0) * prof[i];
                 }
             }
         }
@@ -543,10 +560,12 @@ static int upload_axes(const struct params *p)
             CHK(hipMalloc((void **)&g_ctx.d_dx[d], sizeof(double) * nd));
             CHK(hipMalloc((void **)&g_ctx.d_idx[d], sizeof(double) * nd));
             CHK(hipMalloc((void **)&g_ctx.d_prof[d], sizeof(double) * nd));
+            CHK(hipMalloc((void **)&g_ctx.d_profq[d], sizeof(double) * nd));
         }
         CHK(hipMemcpy(g_ctx.d_dx[d], dx.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
         CHK(hipMemcpy(g_ctx.d_idx[d], idx.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
         CHK(hipMemcpy(g_ctx.d_prof[d], prof.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
+        CHK(hipMemcpy(g_ctx.d_profq[d], profq.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
     }
     (void)p;
     return 0;
@@ -878,9 +897,19 @@ extern "C" int cusource(struct params **p, struct params **d_p, struct state **d
 
 extern "C" int cusetdriver(int enable, const real drv[7])
 {
-    g_ctx.driver = enable ? 1 : 0;
+    if (enable < 0 || enable > 2) return cfail(SMAUG_EINVAL, "cusetdriver: enable must be 0, 1 or 2");
+    if (enable == 2 && D != 3)
+        return cfail(SMAUG_EINVAL, "cusetdriver: the torsional driver drives m_y and m_z (3-D only)");
+    g_ctx.driver = enable;
     if (enable && drv)
         for (int k = 0; k < 7; k++) g_ctx.drv[k] = drv[k];
+    if (enable == 2) {
+        // the reference's constants (usersource.cu:26-57)
+        static const double ref[8] = {500.0, 30.0, 0.1e6, 0.1e6, 0.05e6, 2.0e6, 2.0e6, 4.0e6};
+        if (!drv)
+            for (int k = 0; k < 7; k++) g_ctx.drv[k] = ref[k];
+        g_ctx.drv[7] = ref[7];
+    }
     if (g_ctx.ready) {
         struct params dummy;
         memset(&dummy, 0, sizeof(dummy));
@@ -1314,6 +1343,7 @@ extern "C" int cufinish(struct params **p, real **w, real **wnew, struct state *
         if (g_ctx.d_dx[d]) (void)hipFree(g_ctx.d_dx[d]);
         if (g_ctx.d_idx[d]) (void)hipFree(g_ctx.d_idx[d]);
         if (g_ctx.d_prof[d]) (void)hipFree(g_ctx.d_prof[d]);
+        if (g_ctx.d_profq[d]) (void)hipFree(g_ctx.d_profq[d]);
     }
     (void)hipFree(g_ctx.dp);
     (void)hipFree(g_ctx.ds);

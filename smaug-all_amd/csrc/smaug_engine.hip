@@ -35,6 +35,12 @@ int smaug::check_cfg(const smaug_cfg *cfg)
         if (!cfg->dx[d]) return fail(SMAUG_EINVAL, "dx[%d] missing", d);
         if (cfg->driver && !cfg->pos[d]) return fail(SMAUG_EINVAL, "pos[%d] missing", d);
     }
+    if (cfg->driver < 0 || cfg->driver > 2)
+        return fail(SMAUG_EINVAL, "driver must be 0 (none), 1 (vertical) or 2 (torsional)");
+    if (cfg->driver == 2) {
+        if (D != 3) return fail(SMAUG_EINVAL, "the torsional driver drives m_y and m_z: 3-D only");
+        if (!(cfg->drv[7] != 0.0)) return fail(SMAUG_EINVAL, "torsional driver: drv[7] (L) is 0");
+    }
     if (cfg->precision != SMAUG_FP64 && cfg->precision != SMAUG_MIXED)
         return fail(SMAUG_EINVAL, "precision must be SMAUG_FP64 or SMAUG_MIXED");
     long long nc = (long long)cfg->n[0] * cfg->n[1] * (D == 3 ? cfg->n[2] : 1);
@@ -44,22 +50,25 @@ int smaug::check_cfg(const smaug_cfg *cfg)
 }
 
 void smaug::axis_arrays(const smaug_cfg *cfg, int d, int nd, std::vector<double> &dx,
-                        std::vector<double> &idx, std::vector<double> &prof)
+                        std::vector<double> &idx, std::vector<double> &prof,
+                        std::vector<double> &profq)
 {
     dx.assign(nd, 1.0);
     idx.assign(nd, 1.0);
     prof.assign(nd, 1.0);
+    profq.assign(nd, 0.0);
     if (d >= cfg->ndim) return;
     for (int i = 0; i < nd; i++) {
         dx[i] = cfg->dx[d][i];
         idx[i] = 1.0 / dx[i];
     }
-    if (cfg->driver == 1) {
+    if (cfg->driver == 1 || cfg->driver == 2) {
         for (int i = 0; i < nd; i++) {
             double p = cfg->pos[d][i];
             double c0 = d == 0 ? cfg->drv[3] : (d == 1 ? cfg->drv[5] : cfg->drv[6]);
             double s = d == 0 ? cfg->drv[4] : cfg->drv[2];
             prof[i] = exp(-(p - c0) * (p - c0) / (s * s));
+            if (d > 0) profq[i] = (p - c0) * prof[i];
         }
     }
 }
@@ -86,7 +95,7 @@ struct smaug_engine {
     int bgp_on;   // use bgp when bgp_ok (default on)
     double *bgu;  // the background's nb values when it is the same in every cell
     int bgu_ok;   // the per-cell kernels then read bgu (Geo::bgf = 1, bgm = 0)
-    double *d_dx[3], *d_idx[3], *d_prof[3];
+    double *d_dx[3], *d_idx[3], *d_prof[3], *d_profq[3];
     unsigned long long *d_cmax;
     double *d_cpart; // per-block partial cfast maxima of the aux/cmax pass
     int ncpart;      // its capacity in blocks
@@ -473,6 +482,7 @@ static void free_engine(smaug_engine *e)
         if (e->d_dx[k]) hipFree(e->d_dx[k]);
         if (e->d_idx[k]) hipFree(e->d_idx[k]);
         if (e->d_prof[k]) hipFree(e->d_prof[k]);
+        if (e->d_profq[k]) hipFree(e->d_profq[k]);
     }
     if (e->ACC) hipFree(e->ACC);
     if (e->R) hipFree(e->R);
@@ -550,11 +560,13 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
     TRYALLOC(e->d_tm, sizeof(double) * 2);
     for (int d = 0; d < 3; d++) {
         int nd = e->n[d];
-        std::vector<double> dx, idx, prof;
-        axis_arrays(cfg, d, nd, dx, idx, prof);
+        std::vector<double> dx, idx, prof, profq;
+        axis_arrays(cfg, d, nd, dx, idx, prof, profq);
         TRYALLOC(e->d_dx[d], sizeof(double) * nd);
         TRYALLOC(e->d_idx[d], sizeof(double) * nd);
         TRYALLOC(e->d_prof[d], sizeof(double) * nd);
+        TRYALLOC(e->d_profq[d], sizeof(double) * nd);
+        hipMemcpy(e->d_profq[d], profq.data(), sizeof(double) * nd, hipMemcpyHostToDevice);
         hipMemcpy(e->d_dx[d], dx.data(), sizeof(double) * nd, hipMemcpyHostToDevice);
         hipMemcpy(e->d_idx[d], idx.data(), sizeof(double) * nd, hipMemcpyHostToDevice);
         hipMemcpy(e->d_prof[d], prof.data(), sizeof(double) * nd, hipMemcpyHostToDevice);
@@ -566,6 +578,7 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
         g.dx[d] = e->d_dx[d];
         g.idx[d] = e->d_idx[d];
         g.prof[d] = e->d_prof[d];
+        g.profq[d] = e->d_profq[d];
     }
     g.s1 = e->n[0];
     g.s2 = (idx_t)e->n[0] * e->n[1];

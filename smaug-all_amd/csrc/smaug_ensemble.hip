@@ -468,7 +468,7 @@ extern "C" int smaug_ens_create(const smaug_cfg *cfgs, int nrun, smaug_ensemble 
         if (c.hyperdif) sr += align256(e->tsz * npair * nc);
         sa += align256(e->tsz * (3 + D) * nc);
         sb += align256(sizeof(double) * nbg * nc);
-        sx += 3 * (size_t)(c.n[0] + c.n[1] + n2);
+        sx += 4 * (size_t)(c.n[0] + c.n[1] + n2);
     }
     ECHK(hipMalloc((void **)&e->state, su));
     if (sr) ECHK(hipMalloc((void **)&e->ratio, sr));
@@ -482,7 +482,7 @@ extern "C" int smaug_ens_create(const smaug_cfg *cfgs, int nrun, smaug_ensemble 
     ECHK(hipMemset(e->state, 0, su));
     ECHK(hipMemset(e->tm, 0, sizeof(double) * 2 * nrun));
 
-    // per-run metric arrays, packed [dx | 1/dx | profile] per dim
+    // per-run metric arrays, packed [dx | 1/dx | profile | lever profile] per dim
     std::vector<double> host(sx);
     size_t pos = 0;
     for (int r = 0; r < nrun; r++) {
@@ -491,17 +491,19 @@ extern "C" int smaug_ens_create(const smaug_cfg *cfgs, int nrun, smaug_ensemble 
         memset(&g, 0, sizeof(g));
         for (int d = 0; d < 3; d++) {
             const int nd = d < D ? c.n[d] : 1;
-            std::vector<double> dx, idx, prof;
-            axis_arrays(&c, d, nd, dx, idx, prof);
+            std::vector<double> dx, idx, prof, profq;
+            axis_arrays(&c, d, nd, dx, idx, prof, profq);
             memcpy(&host[pos], dx.data(), sizeof(double) * nd);
             memcpy(&host[pos + nd], idx.data(), sizeof(double) * nd);
             memcpy(&host[pos + 2 * nd], prof.data(), sizeof(double) * nd);
+            memcpy(&host[pos + 3 * nd], profq.data(), sizeof(double) * nd);
             g.n[d] = nd;
             g.per[d] = d < D ? (c.bc[d] == 0) : 1;
             g.dx[d] = e->axes + pos;
             g.idx[d] = e->axes + pos + nd;
             g.prof[d] = e->axes + pos + 2 * nd;
-            pos += 3 * (size_t)nd;
+            g.profq[d] = e->axes + pos + 3 * nd;
+            pos += 4 * (size_t)nd;
         }
         g.s1 = g.n[0];
         g.s2 = (idx_t)g.n[0] * g.n[1];

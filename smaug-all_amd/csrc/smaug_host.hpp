@@ -21,9 +21,12 @@ int check_cfg(const smaug_cfg *cfg);
 // arrays along the unused third dim of a 2-D grid).  The driver profile is
 // exp(-(z-z0)^2/sz^2) along dim 0 and exp(-(x-xc)^2/sh^2) along the
 // horizontal dims, so that the product over dims is the driver's
-// Gaussian footprint (usersource.cu:4-92).
+// Gaussian footprint (usersource.cu:4-92); profq is (x - xc) times the
+// profile along the horizontal dims (the torsional driver's lever arms),
+// 0 along dim 0.
 void axis_arrays(const smaug_cfg *cfg, int d, int nd, std::vector<double> &dx,
-                 std::vector<double> &idx, std::vector<double> &prof);
+                 std::vector<double> &idx, std::vector<double> &prof,
+                 std::vector<double> &profq);
 
 // compensated FP64 sums of the evolved fields of u over the interior cells
 // of grid g (k_sums plus a compensated host pass), nve values into sums;

@@ -685,8 +685,26 @@ __device__ __forceinline__ void driver_add(double pr, const T rc, const double r
     dU[LY::EN] += v * v * rfull / (T)2;
 }
 
+// the torsional driver's sources (horizontal velocities vx along dim 1,
+// vy along dim 2: m_1 += rho vx, m_2 += rho vy, e += rho (vx^2 + vy^2)/2)
+// of a cell with total density rc + rbc, added to dU
+template <typename T>
+__device__ __forceinline__ void driver_add_tor(double vx_, double vy_, const T rc,
+                                               const double rbc, T *dU)
+{
+    typedef Lay<3> LY;
+    const T vx = (T)vx_, vy = (T)vy_;
+    const T rfull = (T)((double)rc + rbc);
+    dU[LY::M0 + 1] += vx * rfull;
+    dU[LY::M0 + 2] += vy * rfull;
+    dU[LY::EN] += (vx * vx + vy * vy) * rfull / (T)2;
+}
+
 // the user driver at interior cell (x0, x1, x2) added to dU; its spatial
-// factor is the product of the per-axis profiles (usersource.cu:4-92)
+// factor is the product of the per-axis profiles (usersource.cu:4-92).
+// Driver 2 is the reference's torsional one: vx = (A/L) sin(..) P0 P1 Q2,
+// vy = -(A/L) sin(..) P0 Q1 P2 with P the Gaussian profiles and Q the lever
+// profiles (x - xc) P (usersource.cu:26-73).
 template <int D, typename T>
 __device__ __forceinline__ void driver_cell(const Geo &g, const Phys &ph, const StageArgs &sa,
                                             const T rc, const double rbc, const int x0,
@@ -696,6 +714,12 @@ __device__ __forceinline__ void driver_cell(const Geo &g, const Phys &ph, const 
         double pr = driver_amp(ph, sa, sa.tm[0], sa.tm[1]) * g.prof[0][x0] * g.prof[1][x1];
         if (D == 3) pr *= g.prof[2][x2];
         driver_add<D, T>(pr, rc, rbc, dU);
+    } else if constexpr (D == 3) {
+        if (ph.driver == 2) {
+            const double a = driver_amp(ph, sa, sa.tm[0], sa.tm[1]) / ph.drv[7] * g.prof[0][x0];
+            driver_add_tor<T>(a * g.prof[1][x1] * g.profq[2][x2],
+                              -(a * g.profq[1][x1] * g.prof[2][x2]), rc, rbc, dU);
+        }
     }
 }
 

@@ -29,6 +29,7 @@ struct Geo {
     const double *dx[3];
     const double *idx[3];  // 1/dx per cell
     const double *prof[3]; // driver profile factors per dim
+    const double *profq[3]; // torsional driver: (x_d - c_d) prof[d] along dims 1, 2
 };
 
 struct Phys {

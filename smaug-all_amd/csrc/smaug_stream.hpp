@@ -165,7 +165,7 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
                                              T *__restrict__ acc, const T *__restrict__ dUh,
                                              const double *__restrict__ bg,
                                              T (*sF)[7][ZS_NB], T (*sV)[ZS_NB], T *sZ,
-                                             double *sP)
+                                             double *sP, double *sQ)
 {
     typedef Lay<3> LY;
 
@@ -212,16 +212,24 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
     const T rk_b = (T)(rs == 0 ? dt / 6 : dt / 3);
     const bool rk_old_acc = sa.mode == 0 && rs == 3;
     T i12x[2] = {0, 0};
-    double pxy = 0;
+    // driver 1: v = pxy P2(k); driver 2 (torsional): vx = pxy Q2(k),
+    // vy = pxq P2(k), as driver_cell forms them
+    double pxy = 0, pxq = 0;
     if (act) {
         i12x[0] = (T)(1.0 / (12.0 * g.dx[0][i]));
         i12x[1] = (T)(1.0 / (12.0 * g.dx[1][j]));
-        if (ph.driver == 1)
+        if (ph.driver == 1) {
             pxy = driver_amp(ph, sa, sa.tm[0], dt) * g.prof[0][i] * g.prof[1][j];
+        } else if (ph.driver == 2) {
+            const double a = driver_amp(ph, sa, sa.tm[0], dt) / ph.drv[7] * g.prof[0][i];
+            pxy = a * g.prof[1][j];
+            pxq = -(a * g.profq[1][j]);
+        }
     }
     for (int t = threadIdx.x; t < k1 - k0; t += ZS_NT) {
         sZ[t] = (T)(1.0 / (12.0 * g.dx[2][k0 + t]));
         sP[t] = g.prof[2][k0 + t];
+        sQ[t] = g.profq[2][k0 + t];
     }
     __syncthreads();
 
@@ -321,7 +329,7 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
         double frb;
         {
             const ci_t cu = fin && need_un ? co : 0u, cc = fin && need_acc ? co : 0u;
-            const ci_t chy = fin && ph.hyperdif ? co : 0u, cd = fin && ph.driver == 1 ? co : 0u;
+            const ci_t chy = fin && ph.hyperdif ? co : 0u, cd = fin && ph.driver ? co : 0u;
 #pragma unroll
             for (int f = 0; f < 8; f++) {
                 fu[f] = ld(un + f * nc, cu);
@@ -439,7 +447,10 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
 #pragma unroll
                     for (int f = 0; f < 8; f++) dU[f] += fh[f];
                 }
-                if (ph.driver == 1) driver_add<3, T>(pxy * sP[O - k0], fr, frb, dU);
+                if (ph.driver == 1)
+                    driver_add<3, T>(pxy * sP[O - k0], fr, frb, dU);
+                else if (ph.driver == 2)
+                    driver_add_tor<T>(pxy * sQ[O - k0], pxq * sP[O - k0], fr, frb, dU);
                 // the stage result and accumulator as rk_write forms them
 #pragma unroll
                 for (int f = 0; f < 8; f++) {
@@ -482,7 +493,8 @@ k_zstream_f(Geo g, Phys ph, StageArgs sa, ZMap zm, const float *__restrict__ uin
     __shared__ float sV[3][ZS_NB];    // velocity
     __shared__ float sZ[ZS_KZMAX];    // 1/(12 dz) of the output planes
     __shared__ double sP[ZS_KZMAX];   // driver profile along k
-    zstream_body<BGP, float>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV, sZ, sP);
+    __shared__ double sQ[ZS_KZMAX];   // torsional driver's lever profile along k
+    zstream_body<BGP, float>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV, sZ, sP, sQ);
 }
 
 // FP64 state: one wave per SIMD, so that nothing spills; 74 KB of LDS
@@ -496,7 +508,8 @@ k_zstream_d(Geo g, Phys ph, StageArgs sa, ZMap zm, const double *__restrict__ ui
     __shared__ double sV[3][ZS_NB];
     __shared__ double sZ[ZS_KZMAX];
     __shared__ double sP[ZS_KZMAX];
-    zstream_body<BGP, double>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV, sZ, sP);
+    __shared__ double sQ[ZS_KZMAX];
+    zstream_body<BGP, double>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV, sZ, sP, sQ);
 }
 
 // ---------------------------------------------------------------------------

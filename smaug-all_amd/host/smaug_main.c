@@ -14,7 +14,10 @@
  *
  *   smaug_main -i in.ini [-o out.ini] [-n steps] [-dt dt] [-rk4] [-mixed]
  *              [-bc b0,b1[,b2]] [-hyperdif 0|1] [-chyp rho,e,mom,b]
- *              [-courant c] [-moddton] [-driver A,P,sh,z0,sz,xc,yc] [-t0 t]
+ *              [-courant c] [-moddton] [-driver A,P,sh,z0,sz,xc,yc] [-torsional]
+ *              [-t0 t]
+ * -torsional selects the reference's torsional driver (usersource.cu), with
+ * the -driver values if given and the reference's constants otherwise
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -50,7 +53,7 @@ static int parse_list(const char *s, double *v, int maxn)
 int main(int argc, char **argv)
 {
     const char *in = NULL, *out = NULL;
-    int nt = 1, rk4 = 0, mixed = 0, hyperdif = 1, moddton = 0, bcset = 0, driver = 0;
+    int nt = 1, rk4 = 0, mixed = 0, hyperdif = 1, moddton = 0, bcset = 0, driver = 0, have_drv = 0;
     double dtarg = 0, courant = 0.3, t0 = 0;
     double bcv[3] = {4, 4, 4}, chv[4] = {0.02, 0.02, 0.4, 0.02}, drv[7] = {0};
     for (int a = 1; a < argc; a++) {
@@ -66,7 +69,11 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[a], "-hyperdif") && a + 1 < argc) hyperdif = atoi(argv[++a]);
         else if (!strcmp(argv[a], "-bc") && a + 1 < argc) bcset = parse_list(argv[++a], bcv, 3);
         else if (!strcmp(argv[a], "-chyp") && a + 1 < argc) parse_list(argv[++a], chv, 4);
-        else if (!strcmp(argv[a], "-driver") && a + 1 < argc) driver = parse_list(argv[++a], drv, 7) == 7;
+        else if (!strcmp(argv[a], "-driver") && a + 1 < argc) {
+            have_drv = parse_list(argv[++a], drv, 7) == 7;
+            if (driver != 2) driver = have_drv;
+        } else if (!strcmp(argv[a], "-torsional"))
+            driver = 2;
         else {
             fprintf(stderr, "unknown or incomplete option %s\n", argv[a]);
             return 2;
@@ -146,7 +153,7 @@ int main(int argc, char **argv)
     struct bparams *d_bp;
     struct state *d_state;
     real *d_wnew, *d_wmod, *d_dwn1, *d_wd, *d_wtemp, *d_wtemp1, *d_wtemp2, *wnew = NULL;
-    CALL(cusetdriver(driver, drv));
+    CALL(cusetdriver(driver, (driver == 2 && !have_drv) ? NULL : drv));
     CALL(cuinit(&p, &bp, &wmod, &wnew, &wd, &st, &d_p, &d_bp, &d_wnew, &d_wmod, &d_dwn1, &d_wd,
                 &d_state, &d_wtemp, &d_wtemp1, &d_wtemp2));
     CALL(initgrid(&p, &st, &wd, &d_p, &d_dwn1, &d_wd, &d_state, &d_wtemp, &d_wtemp1, &d_wtemp2));

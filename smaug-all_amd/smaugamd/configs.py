@@ -225,6 +225,28 @@ def stratified(nz=256, nx=256, ny=256, lz=6.0e6, lx=4.0e6, ly=4.0e6, bz_tesla=0.
     return s
 
 
+def torsional_driver(s: Setup, amp=500.0, period=30.0, sigma_h=0.1e6, z0=0.1e6, sigma_z=0.05e6,
+                     xc=None, yc=None, length=4.0e6) -> Setup:
+    """the set-up with the reference's own user driver in place of the
+    vertical one: the torsional velocities of dev/smaug/src/usersource.cu:4-86,
+    vx = (A yp/L) G sin(2 pi t/P) along dim 1, vy = -(A xp/L) G sin(2 pi t/P)
+    along dim 2, G = exp(-(xp^2 + yp^2)/sigma_h^2) exp(-zp^2/sigma_z^2), with
+    the reference's constants (A = 500 m/s, P = 30 s, sigma_h = 0.1 Mm,
+    z0 = 0.1 Mm, sigma_z = 0.05 Mm, L = 4 Mm) and the centre (xc, yc) in the
+    middle of the horizontal extent unless given (the reference's 2 Mm, 2 Mm
+    on its 4 x 4 Mm box)"""
+    if s.ndim != 3:
+        raise ValueError("the torsional driver drives m_y and m_z: 3-D grids only")
+    p1, p2 = s.pos[1], s.pos[2]
+    if xc is None:
+        xc = 0.5 * (p1[2] + p1[-3])
+    if yc is None:
+        yc = 0.5 * (p2[2] + p2[-3])
+    out = dataclasses.replace(s, drv=np.array([amp, period, sigma_h, z0, sigma_z, xc, yc, length]))
+    out.driver = 2
+    return out
+
+
 def hyperviscous_dt(s: Setup, rmax: float = 6.0, dtdiffpar: float = 0.25) -> float:
     """0.25 dx^2/nu_max with nu_max = dx cmax_d max(chyp) rmax, over dims,
     for the set-up's initial state (cmax_d from the same fast speed as

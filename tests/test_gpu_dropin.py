@@ -96,3 +96,26 @@ def test_driver_rk4_fused_stratified(tmp_path):
     err = rel_err(data[: s.nve], ref)
     print("driver rk4 stratified rel err", err)
     assert np.all(err < 1e-11), err
+
+
+def test_driver_torsional_onestep_and_rk4(tmp_path):
+    # the C driver with the reference's torsional user source (-torsional):
+    # per-entry-point onestep (cusource with p->qt) and fused RK4
+    s = configs.torsional_driver(_strat_small())
+    o = Oracle(s)
+    s.w = o.boundary(perturbed(s, amp=1e-3, seed=10))
+    inp = roundtrip_state(s, tmp_path, "strat_tor")
+    n, t0 = 3, 5.0
+    for rk4 in (False, True):
+        out = str(tmp_path / f"strat_tor_out{int(rk4)}.ini")
+        args = ["-i", inp, "-o", out, "-n", str(n), "-dt", repr(s.dt), "-t0", str(t0),
+                "-bc", "4,4,4", "-driver", _drv_arg(s), "-torsional"] + (["-rk4"] if rk4 else [])
+        run_driver(3, args, tmp_path)
+        _, _, data = vacio.read_ascii(out)
+        if rk4:
+            ref, _ = Oracle(s).run(s.w, t0, s.dt, n, rk4=True)
+        else:  # the driver advances p->qt before each step (iosmaug.cpp:918)
+            ref, _ = Oracle(s).run(s.w, t0 + s.dt, s.dt, n, rk4=False)
+        err = rel_err(data[: s.nve], ref)
+        print("driver torsional rk4" if rk4 else "driver torsional onestep", "rel err", err)
+        assert np.all(err < 1e-11), err

@@ -100,3 +100,39 @@ def test_euler_steps_fp64():
     err = rel_err(got, ref)
     print("euler 10 steps rel err", err)
     assert np.all(err < 1e-11), err
+
+
+@pytest.mark.parametrize("form", ["zstream", "per-cell"])
+def test_torsional_driver_fp64(form):
+    # the reference's own user driver (usersource.cu), both 3-D update forms
+    s = configs.torsional_driver(configs.stratified(nz=20, nx=12, ny=16, lz=6e6 * 20 / 256 * 4,
+                                                    lx=4e6 * 12 / 256, ly=4e6 * 16 / 256))
+    print(f"[case] torsional driver fp64 form={form} n={s.n}", flush=True)
+    o = Oracle(s)
+    w = o.boundary(perturbed(s, amp=1e-3, seed=13))
+    t = 6.0  # sin(2 pi t / 30) != 0
+    ref_r = o.rhs(w, t=t)
+    ref, _ = o.run(w, t, s.dt, 3, rk4=True)
+    e = Engine(s, FP64)
+    if form == "per-cell":
+        e.set_zstream(False)
+    e.set_state(w)
+    got_r = e.rhs(t=t)
+    e.rk4(t, s.dt, 3)
+    got = e.get_state()
+    e.close()
+    m = interior_mask(s)
+    # the driver moves the horizontal momenta
+    assert np.abs(ref_r[2][m]).max() > 0 and np.abs(ref_r[3][m]).max() > 0
+    err = rel_err(got_r, ref_r, m)
+    assert np.all(err < TOL_RHS_FP64), err
+    err = rel_err(got, ref, m)
+    assert np.all(err < TOL_RHS_FP64), err
+
+
+def test_torsional_driver_rejected_in_2d():
+    s = configs.orszag_tang(nx=16)
+    s.driver = 2
+    s.drv = np.array([500.0, 30.0, 0.1, 0.0, 0.1, 0.5, 0.5, 4.0])
+    with pytest.raises(Exception, match="3-D only"):
+        Engine(s, FP64)

@@ -309,6 +309,44 @@ def test_gravity_and_driver_closed_forms():
     np.testing.assert_allclose(r[1][m], (-274.0 * w[0])[m], rtol=1e-12)
 
 
+def test_torsional_driver_closed_forms():
+    # the reference's own driver (usersource.cu:4-86) on a state at rest
+    # with no perturbation: L(w) is the driver alone
+    s = configs.torsional_driver(configs.stratified(nz=12, nx=12, ny=12, lz=1.2e6, lx=1e6,
+                                                    ly=1e6), sigma_h=0.2e6, sigma_z=0.3e6,
+                                 z0=0.5e6)
+    s.hyperdif = 0
+    o = Oracle(s)
+    t = 4.0
+    r = o.rhs(o.boundary(np.zeros((s.nve, s.ncell))), t=t)
+    m = interior_mask(s)
+    i, j, k = cell_coords(s)
+    A, P, sh, z0, sz, xc, yc, L = s.drv
+    xp, yp, zp = s.pos[1][j] - xc, s.pos[2][k] - yc, s.pos[0][i] - z0
+    rho = s.bg[1]
+    vx, vy = r[2] / rho, r[3] / rho
+    scale = np.abs(vx[m]).max()
+    assert scale > 0
+    # purely horizontal and torsional: no vertical push, no radial component
+    assert np.all(r[1][m] == 0)
+    np.testing.assert_allclose((vx * xp + vy * yp)[m], 0, atol=1e-12 * scale * 1e6)
+    # energy = rho |v|^2 / 2 of the same velocities
+    np.testing.assert_allclose(r[4][m], (0.5 * rho * (vx * vx + vy * vy))[m], rtol=1e-12)
+    # the grid is symmetric about the centre: no net momentum is injected
+    assert abs(r[2][m].sum()) < 1e-12 * np.abs(r[2][m]).sum()
+    assert abs(r[3][m].sum()) < 1e-12 * np.abs(r[3][m]).sum()
+    # the speed |v| = A r/L exp(-r^2/sh^2) exp(-zp^2/sz^2) |sin(2 pi t/P)|
+    rad = np.sqrt(xp * xp + yp * yp)
+    speed = A * rad / L * np.exp(-rad ** 2 / sh ** 2) * np.exp(-zp ** 2 / sz ** 2) * \
+        abs(math.sin(2 * math.pi * t / P))
+    np.testing.assert_allclose(np.hypot(vx, vy)[m], speed[m], rtol=1e-12, atol=1e-300)
+    # the rotation sense: vx follows +yp and vy follows -xp (times sin(2 pi t/P))
+    td = math.sin(2 * math.pi * t / P)
+    live = m & (np.hypot(vx, vy) > 1e-6 * scale)
+    assert np.all(np.sign(vx[live] * td) == np.sign(yp[live]))
+    assert np.all(np.sign(vy[live] * td) == -np.sign(xp[live]))
+
+
 def test_rk4_is_fourth_order_in_time():
     s = configs.orszag_tang(nx=16)
     s.hyperdif = 0
