@@ -145,3 +145,26 @@ def test_bg_plane_off_for_k_varying_background():
     ref = o.rhs(w0, 0.5)
     err = rel_err(got, ref, interior_mask(s))
     assert np.all(err < TOL_ORACLE_FP64), err
+
+
+@pytest.mark.parametrize("bc", [(4, 0, 0), (4, 4, 0), (4, 0, 4)], ids=["cpp", "ccp", "cpc"])
+def test_folded_ghost_fill_periodic_vs_oracle(bc):
+    # the k-plane background path fills the ghosts of RK stage outputs in the
+    # next aux pass; with periodic horizontal faces the images wrap around
+    # (the vertical axis stays contcd4: a periodic vertical axis would join
+    # the dense bottom of the atmosphere to its top)
+    s = dataclasses.replace(strat(20, 12, 16), bc=bc)
+    print(f"[case] folded ghost fill bc={bc} n={s.n}", flush=True)
+    o = Oracle(s)
+    w0 = o.boundary(perturbed(s, amp=1e-3, seed=14))
+    dt = 0.5 * configs.hyperviscous_dt(s)
+    ref, _ = o.run(w0, 0.25, dt, 3, rk4=True)
+    e = Engine(s, FP64)
+    e.set_state(w0)
+    assert e.get_bg_plane()
+    e.rk4(0.25, dt, 3)
+    got = e.get_state()
+    e.close()
+    # every cell, ghosts included
+    err = rel_err(got, ref)
+    assert np.all(err < TOL_ORACLE_FP64), err
