@@ -770,22 +770,32 @@ extern "C" int smaug_cmax(smaug_engine *e, double cmax[3])
     return DISPATCH(e, do_cmax, e, cmax);
 }
 
+template <int D, typename T>
+static int do_courant(smaug_engine *e, double *out)
+{
+    const unsigned nb = (unsigned)prep_blocks(e->nc);
+    const double *bg;
+    const Geo g = pc_geo(e, &bg);
+    hipLaunchKernelGGL((k_courant<D, T>), dim3(nb), dim3(256), 0, e->stream, g, e->phys,
+                       (const T *)e->U[0], bg, e->d_cpart);
+    HIPCHK(hipGetLastError());
+    // one value, reduced into the spare fourth cmax slot
+    hipLaunchKernelGGL(k_cmax_reduce<1>, dim3(1), dim3(1024), 0, e->stream,
+                       (const double *)e->d_cpart, (int)nb, e->d_cmax + 3);
+    HIPCHK(hipGetLastError());
+    unsigned long long b;
+    HIPCHK(hipMemcpyAsync(&b, e->d_cmax + 3, sizeof(b), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    memcpy(out, &b, sizeof(double));
+    return 0;
+}
+
+// max over cells and dims of cfast_d / dx_d at the cell (the Courant limit
+// of cucomputemaxcourant over every dim), exact on non-uniform grids
 extern "C" int smaug_maxcourant(smaug_engine *e, double *out)
 {
     if (!e || !out) return fail(SMAUG_EINVAL, "null argument");
-    double cm[3];
-    int rc = smaug_cmax(e, cm);
-    if (rc) return rc;
-    // cfast_d/dx_d: dx is uniform along each dim for the supported grids,
-    // so the max of the ratio is cmax_d / min dx_d
-    double m = 0.0;
-    for (int d = 0; d < e->D; d++) {
-        double mn = 1e300;
-        for (int i = 0; i < e->n[d]; i++) mn = fmin(mn, e->cfg.dx[d][i]);
-        m = fmax(m, cm[d] / mn);
-    }
-    *out = m;
-    return 0;
+    return DISPATCH(e, do_courant, e, out);
 }
 
 template <int D, typename T>

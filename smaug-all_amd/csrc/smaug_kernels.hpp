@@ -158,6 +158,30 @@ __global__ __launch_bounds__(1024) void k_cmax_reduce(const double *__restrict__
     }
 }
 
+// Courant limit: max over every cell and dim of cfast_d / dx_d at the cell
+// (cucomputemaxcourant, computedervfields.cu:2118-2296), the block's
+// maximum into part[4 blockIdx.x]; blockDim.x == 256, grid-stride
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_courant(Geo g, Phys ph, const T *__restrict__ u,
+                                                  const double *__restrict__ bg, double *part)
+{
+    __shared__ double red[1][4];
+    double mx[1] = {0.0};
+    const ci_t nc = (ci_t)g.nc, s1 = (ci_t)g.s1, s2 = (ci_t)g.s2;
+    for (ci_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        St<D, T> s;
+        load_pert32<D, T>(s, u, (size_t)g.nc, c);
+        load_bg32<D, T>(s, bg, (size_t)g.bgf, c & g.bgm);
+        const int x[3] = {(int)(c % s1), (int)((c / s1) % (ci_t)g.n[1]), (int)(c / s2)};
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            const double v = cfast<D, T>(s, d, ph.gamma) * g.idx[d][x[d]];
+            mx[0] = v > mx[0] ? v : mx[0];
+        }
+    }
+    block_max_part<1>(mx, red, part + 4 * blockIdx.x);
+}
+
 // blockDim.x == 256; grid-stride over every cell (ghosts included); the
 // block's cfast maxima go to part[4 blockIdx.x + d]
 template <int D, typename T>

@@ -191,6 +191,12 @@ class Engine:
         self._check(self.lib.smaug_cmax(self.h, _dptr(c)))
         return c
 
+    def maxcourant(self):
+        """max over cells and dims of cfast_d / dx_d of the current state"""
+        v = np.zeros(1)
+        self._check(self.lib.smaug_maxcourant(self.h, _dptr(v)))
+        return float(v[0])
+
     def rk4(self, t0, dt, nsteps):
         self._check(self.lib.smaug_rk4(self.h, t0, dt, nsteps))
 

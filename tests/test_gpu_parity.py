@@ -136,3 +136,29 @@ def test_torsional_driver_rejected_in_2d():
     s.drv = np.array([500.0, 30.0, 0.1, 0.0, 0.1, 0.5, 0.5, 4.0])
     with pytest.raises(Exception, match="3-D only"):
         Engine(s, FP64)
+
+
+@pytest.mark.parametrize("name", ["ot32-stretched", "strat-stretched"])
+def test_maxcourant_fp64_nonuniform(name):
+    # max over cells of cfast_d / dx_d at the cell: on a stretched grid this
+    # differs from max cfast_d / min dx_d
+    if name.startswith("ot"):
+        s = configs.orszag_tang(nx=32)
+    else:
+        s = configs.stratified(nz=20, nx=12, ny=16, lz=6e6 * 20 / 256 * 4, lx=4e6 * 12 / 256,
+                               ly=4e6 * 16 / 256)
+    d = s.ndim - 1
+    x = np.arange(s.n[d])
+    s.dx[d] = s.dx[d] * (1.0 + 0.3 * np.sin(2 * np.pi * x / s.n[d]))
+    print(f"[case] maxcourant stretched {name} n={s.n}", flush=True)
+    o = Oracle(s)
+    w = o.boundary(perturbed(s, amp=1e-2, seed=15))
+    ref = o.maxcourant(w)
+    e = Engine(s, FP64)
+    e.set_state(w)
+    got = e.maxcourant()
+    cm = e.cmax()
+    e.close()
+    uniform_bound = max(cm[k] / np.min(s.dx[k]) for k in range(s.ndim))
+    assert got == pytest.approx(ref, rel=1e-12)
+    assert got < uniform_bound * (1 - 1e-6)  # the stretched axis matters
