@@ -150,7 +150,7 @@ def cpu_baseline(n):
     o = Oracle(s)
     w0 = o.boundary(s.w)
     o.run(w0, 0.0, s.dt, 1)  # warm caches and page in
-    steps = 3
+    steps = 4
     t = time.perf_counter()
     o.run(w0, 0.0, s.dt, steps)
     el = time.perf_counter() - t
