@@ -328,23 +328,54 @@ static int launch_hyper_sources(smaug_engine *e, const void *uin)
     return 0;
 }
 
+// fold (k-plane background only): the update also forms the aux
+// quantities and cfast partials of its output, k_ghost_prep fills the
+// output's ghost cells and forms theirs, and the partials are reduced into
+// cmax -- the aux/cmax pass of the next stage, done without reading the
+// interior of the output again
 template <typename T>
 static int launch_zstream(smaug_engine *e, StageArgs sa, const void *uin, const void *un,
-                          void *uout)
+                          void *uout, bool fold = false)
 {
     unsigned nb;
     ZMap m = make_zmap(e, &nb);
     const bool bp = bg_planar(e);
     const double *bg = bp ? e->bgp : e->bg;
-    if constexpr (sizeof(T) == 4)
-        hipLaunchKernelGGL(bp ? k_zstream_f<true> : k_zstream_f<false>, dim3(nb), dim3(ZS_NT), 0,
-                           e->stream, e->geo, e->phys, sa, m, (const T *)uin, (const T *)un,
-                           (T *)uout, (T *)e->ACC, (const T *)e->HYP, bg);
-    else
-        hipLaunchKernelGGL(bp ? k_zstream_d<true> : k_zstream_d<false>, dim3(nb), dim3(ZS_NT), 0,
-                           e->stream, e->geo, e->phys, sa, m, (const T *)uin, (const T *)un,
-                           (T *)uout, (T *)e->ACC, (const T *)e->HYP, bg);
+    if (fold && !bp) return fail(SMAUG_EINVAL, "folded aux pass needs the k-plane background");
+    if constexpr (sizeof(T) == 4) {
+        if (fold)
+            hipLaunchKernelGGL((k_zstream_f<true, true>), dim3(nb), dim3(ZS_NT), 0, e->stream,
+                               e->geo, e->phys, sa, m, (const T *)uin, (const T *)un, (T *)uout,
+                               (T *)e->ACC, (const T *)e->HYP, bg, (T *)e->AUX, e->d_cpart);
+        else
+            hipLaunchKernelGGL(bp ? k_zstream_f<true> : k_zstream_f<false>, dim3(nb), dim3(ZS_NT),
+                               0, e->stream, e->geo, e->phys, sa, m, (const T *)uin,
+                               (const T *)un, (T *)uout, (T *)e->ACC, (const T *)e->HYP, bg,
+                               nullptr, nullptr);
+    } else {
+        if (fold)
+            hipLaunchKernelGGL((k_zstream_d<true, true>), dim3(nb), dim3(ZS_NT), 0, e->stream,
+                               e->geo, e->phys, sa, m, (const T *)uin, (const T *)un, (T *)uout,
+                               (T *)e->ACC, (const T *)e->HYP, bg, (T *)e->AUX, e->d_cpart);
+        else
+            hipLaunchKernelGGL(bp ? k_zstream_d<true> : k_zstream_d<false>, dim3(nb), dim3(ZS_NT),
+                               0, e->stream, e->geo, e->phys, sa, m, (const T *)uin,
+                               (const T *)un, (T *)uout, (T *)e->ACC, (const T *)e->HYP, bg,
+                               nullptr, nullptr);
+    }
     HIPCHK(hipGetLastError());
+    if (fold) {
+        const idx_t ng = ghost_count(e->n[0], e->n[1], e->n[2]);
+        const unsigned gb = (unsigned)((ng + 255) / 256);
+        if ((idx_t)nb + gb > (idx_t)e->ncpart) return fail(SMAUG_EINVAL, "cfast partials overflow");
+        hipLaunchKernelGGL((k_ghost_prep<T>), dim3(gb), dim3(256), 0, e->stream, e->geo, e->phys,
+                           (T *)uout, (const double *)e->bgp, (T *)e->AUX,
+                           e->d_cpart + (size_t)4 * nb, 0);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_cmax_reduce<3>, dim3(1), dim3(1024), 0, e->stream,
+                           (const double *)e->d_cpart, (int)(nb + gb), e->d_cmax);
+        HIPCHK(hipGetLastError());
+    }
     return 0;
 }
 
@@ -365,10 +396,10 @@ static int launch_pre_update(smaug_engine *e, const void *uin)
 
 template <int D, typename T>
 static int launch_stage_update(smaug_engine *e, StageArgs sa, const void *uin, const void *un,
-                               void *uout)
+                               void *uout, bool fold = false)
 {
     if constexpr (D == 3)
-        if (e->zstream) return launch_zstream<T>(e, sa, uin, un, uout);
+        if (e->zstream) return launch_zstream<T>(e, sa, uin, un, uout, fold);
     return launch_update<D, T>(e, sa, uin, un, uout);
 }
 
@@ -390,15 +421,19 @@ __global__ void k_advance_time(double *tm)
 
 // one stage: cmax(uin), ratios(uin), update -> uout, ghost fill of uout.
 // With the k-plane background, the outputs of RK stages 0-2 are left
-// unfilled and the aux/cmax pass of the next stage fills their ghosts.
+// unfilled and the aux/cmax pass of the next stage fills their ghosts; with
+// the z-streamed update as well, that pass is folded into the update of
+// RK stages 0-2 (launch_zstream) and the next stage starts at the ratios.
 template <int D, typename T>
 static int stage(smaug_engine *e, StageArgs sa, const void *uin, const void *un, void *uout,
                  hipEvent_t ev0, hipEvent_t ev1, const hipEvent_t *evk = nullptr)
 {
     int rc;
     const bool defer = D == 3 && bg_planar(e) && sa.mode == 0;
+    const bool fold = defer && e->zstream;
     if (evk) HIPCHK(hipEventRecord(evk[0], e->stream));
-    if ((rc = launch_cmax<D, T>(e, uin, defer && sa.stage > 0))) return rc;
+    if (!(fold && sa.stage > 0))
+        if ((rc = launch_cmax<D, T>(e, uin, defer && sa.stage > 0))) return rc;
     if (evk) HIPCHK(hipEventRecord(evk[1], e->stream));
     if (e->phys.hyperdif)
         if ((rc = launch_ratio<D, T>(e, uin))) return rc;
@@ -406,7 +441,7 @@ static int stage(smaug_engine *e, StageArgs sa, const void *uin, const void *un,
     if ((rc = launch_pre_update<D, T>(e, uin))) return rc;
     if (ev0) HIPCHK(hipEventRecord(ev0, e->stream));
     if (evk) HIPCHK(hipEventRecord(evk[3], e->stream));
-    if ((rc = launch_stage_update<D, T>(e, sa, uin, un, uout))) return rc;
+    if ((rc = launch_stage_update<D, T>(e, sa, uin, un, uout, fold && sa.stage < 3))) return rc;
     if (ev1) HIPCHK(hipEventRecord(ev1, e->stream));
     if (evk) HIPCHK(hipEventRecord(evk[4], e->stream));
     if (sa.mode != 2 && !(defer && sa.stage < 3))
@@ -553,6 +588,12 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
             const idx_t ncol = ((idx_t)e->n[0] * e->n[1] + 255) / 256 *
                                ((e->n[2] + PREP_KC - 1) / PREP_KC);
             if (ncol > nb) nb = ncol;
+            // the folded aux pass: z-stream blocks at depth 1 and the ghost cells
+            const idx_t tiles = ((idx_t)(e->n[0] - 4 + ZS_TI - 1) / ZS_TI) *
+                                ((e->n[1] - 4 + ZS_TJ - 1) / ZS_TJ);
+            const idx_t nzs = 8 * ((tiles + 7) / 8) * (e->n[2] - 4) +
+                              (ghost_count(e->n[0], e->n[1], e->n[2]) + 255) / 256;
+            if (nzs > nb) nb = nzs;
         }
         e->ncpart = (int)nb;
         TRYALLOC(e->d_cpart, sizeof(double) * 4 * nb);

@@ -360,6 +360,80 @@ __global__ __launch_bounds__(256) void k_prep_col(Geo g, Phys ph, T *__restrict_
     block_max_part<3>(mx, red, part + 4 * (blockIdx.x + gridDim.x * blockIdx.y));
 }
 
+// Ghost cells of a 3-D grid, each once: the four k-ghost planes in full,
+// then per interior plane its ring (rows j = 0, 1, n1-2, n1-1 in full, then
+// the cells i = 0, 1, n0-2, n0-1 of the rows between)
+__host__ __device__ __forceinline__ idx_t ghost_count(int n0, int n1, int n2)
+{
+    return (idx_t)4 * n0 * n1 + (idx_t)(n2 - 4) * (4 * n0 + 4 * (n1 - 4));
+}
+
+__device__ __forceinline__ void ghost_cell(idx_t t, int n0, int n1, int n2, int &i, int &j, int &k)
+{
+    const idx_t pl = (idx_t)n0 * n1;
+    if (t < 4 * pl) {
+        const int q = (int)(t / pl);
+        const int p = (int)(t - q * pl);
+        k = q < 2 ? q : n2 - 4 + q;
+        j = p / n0;
+        i = p - j * n0;
+        return;
+    }
+    t -= 4 * pl;
+    const int ring = 4 * n0 + 4 * (n1 - 4);
+    const int kk = (int)(t / ring);
+    int r = (int)(t - (idx_t)kk * ring);
+    k = 2 + kk;
+    if (r < 4 * n0) {
+        const int q = r / n0;
+        j = q < 2 ? q : n1 - 4 + q;
+        i = r - q * n0;
+    } else {
+        r -= 4 * n0;
+        j = 2 + (r >> 2);
+        const int q = r & 3;
+        i = q < 2 ? q : n0 - 4 + q;
+    }
+}
+
+// The ghost-cell share of k_prep_col with fill = 1, for a state whose
+// interior aux and cfast maxima the stage update that wrote it has formed
+// already: every ghost cell takes the state of its image, and its aux and
+// cfast are formed with its own background (the k-plane bgp) by the same
+// load site and arithmetic.  The block's maxima go to part[4 b + d].
+template <typename T>
+__global__ __launch_bounds__(256) void k_ghost_prep(Geo g, Phys ph, T *__restrict__ u,
+                                                    const double *__restrict__ bgp,
+                                                    T *__restrict__ aux, double *part, int full)
+{
+    __shared__ double red[3][4];
+    double mx[3] = {0.0, 0.0, 0.0};
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    const idx_t t = (idx_t)blockIdx.x * 256 + threadIdx.x;
+    if (t < ghost_count(n0, n1, n2)) {
+        const ci_t s2 = (ci_t)g.s2, s1 = (ci_t)g.s1;
+        const size_t nc = (size_t)g.nc;
+        int i, j, k;
+        ghost_cell(t, n0, n1, n2, i, j, k);
+        const ci_t p = (ci_t)i + s1 * (ci_t)j;
+        const ci_t c = p + s2 * (ci_t)k;
+        const ci_t img = (ci_t)bc_map(i, n0, g.per[0]) + s1 * (ci_t)bc_map(j, n1, g.per[1]) +
+                         s2 * (ci_t)bc_map(k, n2, g.per[2]);
+        St<3, T> s;
+        load_bg32<3, T>(s, bgp, (size_t)s2, p);
+        load_pert32<3, T>(s, u, nc, img);
+        st(u, c, s.r);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            st(u + (1 + d) * nc, c, s.m[d]);
+            st(u + (5 + d) * nc, c, s.b[d]);
+        }
+        st(u + 4 * nc, c, s.e);
+        prep_st<3, T>(g, ph, s, aux, c, mx, full);
+    }
+    block_max_part<3>(mx, red, part + 4 * blockIdx.x);
+}
+
 // number of ghost-slab elements of a grid (4 layers per slab, slabs may overlap)
 template <int D>
 __host__ __device__ __forceinline__ idx_t bc_count(int n0, int n1, int n2)

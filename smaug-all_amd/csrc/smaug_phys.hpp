@@ -251,11 +251,18 @@ __device__ __forceinline__ double e_int(const St<D, T> &s, double rho)
     return (double)s.e - 0.5 * (bb + mm / rho);
 }
 
-// fast magnetosonic speed along d plus |v_d|, computec3 (dervfields.cu:418-486), in double
+// fast magnetosonic speed along d plus |v_d|, computec3 (dervfields.cu:418-486), in double.
+// The five divisions by rho of that formula are products with 1/rho here
+// (one division per cell instead of nine once the d-independent terms are
+// shared): the result differs from the divided form by rounding only.
+// Contraction into FMAs is off, so that every kernel that inlines this gets
+// the same bits whatever else the products feed.
 template <int D, typename T>
 __device__ __forceinline__ double cfast(const St<D, T> &s, int d, double gam)
 {
-    double rho = rho_full(s);
+#pragma clang fp contract(off)
+    const double rho = rho_full(s);
+    const double ir = 1.0 / rho;
     double mm = 0, bb = 0, bgb = 0, bbb = 0;
 #pragma unroll
     for (int t = 0; t < D; t++) {
@@ -265,9 +272,9 @@ __device__ __forceinline__ double cfast(const St<D, T> &s, int d, double gam)
         bgb += B * b;
         bbb += B * B;
     }
-    double pk = (gam - 1) * ((double)s.e - 0.5 * (mm / rho) - 0.5 * bb - bgb);
-    double cs2 = (gam * (pk + (gam - 1) * (s.eb - 0.5 * bbb))) / rho;
-    double cf2 = ((bb + bbb + 2.0 * bgb) / rho) + cs2;
+    double pk = (gam - 1) * ((double)s.e - 0.5 * (mm * ir) - 0.5 * bb - bgb);
+    double cs2 = (gam * (pk + (gam - 1) * (s.eb - 0.5 * bbb))) * ir;
+    double cf2 = ((bb + bbb + 2.0 * bgb) * ir) + cs2;
     double btd = 0, md = 0;
 #pragma unroll
     for (int t = 0; t < D; t++)
@@ -275,8 +282,8 @@ __device__ __forceinline__ double cfast(const St<D, T> &s, int d, double gam)
             btd = s.bb[t] + (double)s.b[t];
             md = s.m[t];
         }
-    double cfi = 0.5 * (cf2 + sqrt(cf2 * cf2 - 4.0 * cs2 * (btd * btd) / rho));
-    return sqrt(cfi) + fabs(md / rho);
+    double cfi = 0.5 * (cf2 + sqrt(cf2 * cf2 - 4.0 * cs2 * (btd * btd) * ir));
+    return sqrt(cfi) + fabs(md * ir);
 }
 
 // edge rules of the reference stencils along a non-periodic direction:

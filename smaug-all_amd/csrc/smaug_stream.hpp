@@ -157,20 +157,28 @@ __device__ __forceinline__ int zs_field(int a, int q)
 }
 
 // BGP: bg is a background that does not vary along k, stored as one plane
-// (field stride s2); its values are loaded once per column
-template <bool BGP, typename T>
+// (field stride s2); its values are loaded once per column.  FOLD (with
+// BGP): the aux quantities (e_int, v) of every output cell are written to
+// aux_out and the block's cfast maxima of the output to part[4 b + d], so
+// that the next stage needs no aux/cmax pass over the interior
+template <bool BGP, bool FOLD, typename T>
 __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const StageArgs &sa,
                                              const ZMap &zm, const T *__restrict__ uin,
                                              const T *__restrict__ un, T *__restrict__ uout,
                                              T *__restrict__ acc, const T *__restrict__ dUh,
                                              const double *__restrict__ bg,
                                              T (*sF)[7][ZS_NB], T (*sV)[ZS_NB], T *sZ,
-                                             double *sP, double *sQ)
+                                             double *sP, double *sQ, T *__restrict__ aux_out,
+                                             double *part, double (*red)[4])
 {
     typedef Lay<3> LY;
 
     int tx, ty, kc;
-    if (!zmap_block(zm, tx, ty, kc)) return;
+    if (!zmap_block(zm, tx, ty, kc)) {
+        if (FOLD && threadIdx.x < 3) part[4 * blockIdx.x + threadIdx.x] = 0.0;
+        return;
+    }
+    double mxf[3] = {0.0, 0.0, 0.0};
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     const size_t nc = (size_t)g.nc;
     const ci_t s1 = (ci_t)g.s1, s2 = (ci_t)g.s2;
@@ -457,6 +465,18 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
                     ov[f] = sa.mode == 2 ? dU[f] : (rk_old_acc ? fa[f] : fu[f]) + rk_a * dU[f];
                     av[f] = wacc ? (sa.stage == 0 ? fu[f] : fa[f]) + rk_b * dU[f] : (T)0;
                 }
+                if constexpr (FOLD) {
+                    // the output's aux and cfast, as the aux pass forms them
+                    St<3, T> so = ro; // the column's background
+                    so.r = ov[0];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        so.m[d] = ov[1 + d];
+                        so.b[d] = ov[5 + d];
+                    }
+                    so.e = ov[4];
+                    prep_st<3, T>(g, ph, so, aux_out, co, mxf, 0);
+                }
             }
             // advance the ring
 #pragma unroll
@@ -478,38 +498,45 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
         }
     }
     store_out();
+    if constexpr (FOLD) block_max_part<3>(mxf, red, part + 4 * blockIdx.x);
 }
 
 // FP32 state: up to 256 VGPRs (the prefetched plane and the output ring
 // stay in registers without spills), two 256-thread blocks of 37 KB of LDS
 // per CU.  BGP: bg is a background stored as one k-plane.
-template <bool BGP>
+template <bool BGP, bool FOLD = false>
 __global__ __launch_bounds__(ZS_NT) __attribute__((amdgpu_waves_per_eu(2))) void
 k_zstream_f(Geo g, Phys ph, StageArgs sa, ZMap zm, const float *__restrict__ uin,
             const float *__restrict__ un, float *__restrict__ uout, float *__restrict__ acc,
-            const float *__restrict__ dUh, const double *__restrict__ bg)
+            const float *__restrict__ dUh, const double *__restrict__ bg,
+            float *__restrict__ aux_out = nullptr, double *part = nullptr)
 {
+    __shared__ double red[3][4];
     __shared__ float sF[2][7][ZS_NB]; // fluxes along i (0) and j (1)
     __shared__ float sV[3][ZS_NB];    // velocity
     __shared__ float sZ[ZS_KZMAX];    // 1/(12 dz) of the output planes
     __shared__ double sP[ZS_KZMAX];   // driver profile along k
     __shared__ double sQ[ZS_KZMAX];   // torsional driver's lever profile along k
-    zstream_body<BGP, float>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV, sZ, sP, sQ);
+    zstream_body<BGP, FOLD, float>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV, sZ, sP, sQ,
+                                   aux_out, part, red);
 }
 
 // FP64 state: one wave per SIMD, so that nothing spills; 74 KB of LDS
-template <bool BGP>
+template <bool BGP, bool FOLD = false>
 __global__ __launch_bounds__(ZS_NT) __attribute__((amdgpu_waves_per_eu(1))) void
 k_zstream_d(Geo g, Phys ph, StageArgs sa, ZMap zm, const double *__restrict__ uin,
             const double *__restrict__ un, double *__restrict__ uout, double *__restrict__ acc,
-            const double *__restrict__ dUh, const double *__restrict__ bg)
+            const double *__restrict__ dUh, const double *__restrict__ bg,
+            double *__restrict__ aux_out = nullptr, double *part = nullptr)
 {
+    __shared__ double red[3][4];
     __shared__ double sF[2][7][ZS_NB];
     __shared__ double sV[3][ZS_NB];
     __shared__ double sZ[ZS_KZMAX];
     __shared__ double sP[ZS_KZMAX];
     __shared__ double sQ[ZS_KZMAX];
-    zstream_body<BGP, double>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV, sZ, sP, sQ);
+    zstream_body<BGP, FOLD, double>(g, ph, sa, zm, uin, un, uout, acc, dUh, bg, sF, sV, sZ, sP, sQ,
+                                    aux_out, part, red);
 }
 
 // ---------------------------------------------------------------------------
