@@ -210,20 +210,24 @@ __device__ __forceinline__ double rho_full(const St<D, T> &s)
 }
 
 // total-pressure perturbation, computept3 (dervfields.cu:284-330):
-// (g-1)(e' - m^2/2rho) - (g-2)(b'.b_b + b'^2/2), in double
+// (g-1)(e' - m^2/2rho) - (g-2)(b'.b_b + b'^2/2), in double.  m^2/rho is
+// taken as a product with 1/rho, the reciprocal the callers form anyway (the
+// compiler shares it), and every fused multiply-add is written out with
+// contraction off, so that every kernel that inlines this gets the same bits.
 template <int D, typename T>
 __device__ __forceinline__ double ptot_pert(const St<D, T> &s, double rho, double gam)
 {
+#pragma clang fp contract(off)
     double mm = 0, bgb = 0, bb = 0;
 #pragma unroll
     for (int d = 0; d < D; d++) {
-        double m = s.m[d], b = s.b[d];
-        mm += m * m;
-        bgb += b * s.bb[d];
-        bb += b * b;
+        const double m = s.m[d], b = s.b[d];
+        mm = fma(m, m, mm);
+        bgb = fma(b, (double)s.bb[d], bgb);
+        bb = fma(b, b, bb);
     }
-    double ptmp = (gam - 2.0) * (bgb + 0.5 * bb);
-    return (gam - 1.0) * ((double)s.e - 0.5 * mm / rho) - ptmp;
+    const double ptmp = (gam - 2.0) * fma(0.5, bb, bgb);
+    return (gam - 1.0) * fma(-0.5 * mm, 1.0 / rho, (double)s.e) - ptmp;
 }
 
 // background total pressure, computepbg3 (dervfields.cu:332-368)
@@ -237,18 +241,20 @@ __device__ __forceinline__ double ptot_bg(const St<D, T> &s, double gam)
 }
 
 // internal-energy perturbation used by the energy hyperdiffusion,
-// e' - (b'^2 + m^2/rho)/2 (hyperdifesource1.cu:300-330), in double
+// e' - (b'^2 + m^2/rho)/2 (hyperdifesource1.cu:300-330), in double; 1/rho
+// and contraction as in ptot_pert
 template <int D, typename T>
 __device__ __forceinline__ double e_int(const St<D, T> &s, double rho)
 {
+#pragma clang fp contract(off)
     double bb = 0, mm = 0;
 #pragma unroll
     for (int d = 0; d < D; d++) {
-        double b = s.b[d], m = s.m[d];
-        bb += b * b;
-        mm += m * m;
+        const double b = s.b[d], m = s.m[d];
+        bb = fma(b, b, bb);
+        mm = fma(m, m, mm);
     }
-    return (double)s.e - 0.5 * (bb + mm / rho);
+    return fma(-0.5, fma(mm, 1.0 / rho, bb), (double)s.e);
 }
 
 // fast magnetosonic speed along d plus |v_d|, computec3 (dervfields.cu:418-486), in double.
