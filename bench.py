@@ -105,19 +105,31 @@ def run_path(setup, precision, warmup, steps, dt_scale=1.0, tune=True, shape=Non
 
 
 def pass_kernels(form, bg_plane, fp32=True):
-    """kernel name (as rocprofv3 prints it, namespace and arguments dropped)
-    of each pass of a 3-D stage for the update form and background layout in
-    use"""
+    """the kernels of each pass of a 3-D RK4 step for the update form and
+    background layout in use: (name as rocprofv3 prints it, namespace and
+    arguments dropped; launches per step).  With the k-plane background and
+    the z-streamed update, the updates of stages 0-2 also form the next
+    stage's aux quantities and cfast maxima (k_ghost_prep does the ghost
+    cells), so the aux pass runs once per step, for stage 0."""
     t = "float" if fp32 else "double"
     s = "f" if fp32 else "d"
     bp = "true" if bg_plane else "false"
+    fold = bool(form) and bg_plane
     src = {1: f"k_hyper_zs_{s}<{bp}>", 2: f"k_hyper<3, {t}>", 3: f"k_hyper_lds_{s}"}
+    if fold:
+        upd = [(f"k_zstream_{s}<true, true>", 3), (f"k_ghost_prep<{t}>", 3),
+               (f"k_zstream_{s}<true, false>", 1)]
+    elif form:
+        upd = [(f"k_zstream_{s}<{bp}, false>", 4)]
+    else:
+        upd = [(f"k_update<3, {t}>", 4)]
     return {
-        "aux+cmax": f"k_prep_col<{t}>" if bg_plane else f"k_prep<3, {t}>",
-        "ratio": f"k_ratio_zs_{s}",
-        "hyperdiffusive sources": src.get(form, ""),
-        "stage update": f"k_zstream_{s}<{bp}>" if form else f"k_update<3, {t}>",
-        "ghost fill": f"k_bc<3, {t}>",
+        "aux+cmax": [(f"k_prep_col<{t}>", 1 if fold else 4)] if bg_plane
+        else [(f"k_prep<3, {t}>", 4)],
+        "ratio": [(f"k_ratio_zs_{s}", 4)],
+        "hyperdiffusive sources": [(src.get(form, ""), 4)],
+        "stage update": upd,
+        "ghost fill": [(f"k_bc<3, {t}>", 1 if bg_plane else 4)],
     }
 
 
@@ -137,6 +149,18 @@ def profiled_traffic(kernel, cells):
     if not k:
         return None
     return k["fetch_bytes"] * t["fetch_scale"] + k["write_bytes"] * t["write_scale"]
+
+
+def pass_traffic(kernels, cells):
+    """profiled HBM bytes of a pass per stage (its launches in one RK4 step /
+    4), or None when a kernel of it is not in the summary"""
+    tot = 0.0
+    for name, n in kernels:
+        b = profiled_traffic(name, cells)
+        if b is None:
+            return None
+        tot += b * n
+    return tot / STAGES_PER_STEP
 
 
 def cpu_baseline(n):
@@ -331,7 +355,8 @@ def main():
     bytes_launch = BYTES_PER_CELL_STEP["mixed"] / STAGES_PER_STEP * cells
     pms = info["passes_ms"]
     dom = max(pms, key=pms.get)
-    kname = pass_kernels(info["form"], info["bg_plane"])[dom]
+    pk = pass_kernels(info["form"], info["bg_plane"])
+    kname = " + ".join(k for k, _ in pk[dom])
     launch_ms = stage_ms if dom == "stage update" else pms[dom]
     achieved = bytes_launch / (launch_ms / 1e3) / 1e9
     stage_total_ms = sum(pms.values())
@@ -344,10 +369,9 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": profiled_traffic(kname, cells),
+        "traffic": pass_traffic(pk[dom], cells),
         "passes_ms": pms,
-        "passes_traffic_bytes": {k: profiled_traffic(v, cells)
-                                 for k, v in pass_kernels(info["form"], info["bg_plane"]).items()},
+        "passes_traffic_bytes": {k: pass_traffic(v, cells) for k, v in pk.items()},
         "stage_frac": bytes_launch / (stage_total_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
         "step_level_frac": (BYTES_PER_CELL_STEP["mixed"] * cells / (ms_step / 1e3) / 1e9)
         / HBM_PEAK_GBS,
