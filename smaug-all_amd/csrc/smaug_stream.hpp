@@ -935,13 +935,13 @@ template <typename T>
 struct HzView {
     const T (*sU)[HB_NB]; // rho', m_0..2, b'_0..2
     const T (*sA)[HB_NB]; // e_int, v_0..2
-    const double *sRb;
+    const T *sRf; // full density rho' + rho_b (formed in double, stored in T)
     const T (*sR)[HB_NB];
     int own;
     __device__ __forceinline__ int bx(int di, int dj) const { return own + di + HB_BI * dj; }
     __device__ __forceinline__ T U(int f, int di, int dj) const { return sU[hb_uslot(f)][bx(di, dj)]; }
     __device__ __forceinline__ T A(int a, int di, int dj) const { return sA[a - Aux<3>::EINT][bx(di, dj)]; }
-    __device__ __forceinline__ double rhob(int di, int dj) const { return sRb[bx(di, dj)]; }
+    __device__ __forceinline__ T rhof(int di, int dj) const { return sRf[bx(di, dj)]; }
     __device__ __forceinline__ T Rp(int p, int di, int dj) const { return sR[hz_islot(p)][bx(di, dj)]; }
 };
 
@@ -954,7 +954,7 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                                               const double *__restrict__ bg,
                                               const unsigned long long *__restrict__ cmb,
                                               T *__restrict__ dUh, T (*sU)[HB_NB],
-                                              T (*sA)[HB_NB], double *sRb, T (*sR)[HB_NB],
+                                              T (*sA)[HB_NB], T *sRf, T (*sR)[HB_NB],
                                               T (*sX)[HB_NB], double *sDz, T *sIz)
 {
     typedef Lay<3> LY;
@@ -980,11 +980,16 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
     // retire in order).  nu = (dx cmax) chyp R as in hyper_terms.
     double dc[2] = {0, 0};
     T id[2] = {0, 0};
+    // 1/dx, or 0 where the gradients of a cell along that axis are zeroed
+    // (zero1): the gradients below take their zeroing from this factor
+    T gx[2] = {0, 0};
     if (act) {
         dc[0] = g.dx[0][i] * cmax[0];
         dc[1] = g.dx[1][j] * cmax[1];
         id[0] = (T)g.idx[0][i];
         id[1] = (T)g.idx[1][j];
+        gx[0] = zero1(g.per[0], i, n0) ? (T)0 : id[0];
+        gx[1] = zero1(g.per[1], j, n1) ? (T)0 : id[1];
     }
     for (int t = threadIdx.x; t < k1 - k0; t += 256) {
         sDz[t] = g.dx[2][k0 + t] * cmax[2];
@@ -1018,7 +1023,7 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
     HzView<T> vw;
     vw.sU = sU;
     vw.sA = sA;
-    vw.sRb = sRb;
+    vw.sRf = sRf;
     vw.sR = sR;
     vw.own = own;
 
@@ -1028,7 +1033,7 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
     // own column, planes P-1 and P-2: the diagonal k-term inputs not already
     // in hv/hr (rho', e_int, v_2, m_0..2; prho(2), pe(2), pv(2,2); rho_b)
     T ow[2][6], oz[2][3];
-    double orb[2];
+    T of2 = 0; // full density of the own column, plane P-2
     // pending outputs P-1, P, P+1
     T rA[8], rB[8], rC[8];
 #pragma unroll
@@ -1048,7 +1053,6 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
         for (int f = 0; f < 6; f++) ow[t][f] = 0;
 #pragma unroll
         for (int f = 0; f < 3; f++) oz[t][f] = 0;
-        orb[t] = 0;
     }
     const bool zper = g.per[2];
 
@@ -1086,7 +1090,7 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                     sA[1 + d][b] = cur[q].v[d];
                 }
                 sA[0][b] = cur[q].e;
-                sRb[b] = cur[q].rb;
+                sRf[b] = (T)((double)cur[q].r + cur[q].rb);
 #pragma unroll
                 for (int p = 0; p < LY::NPAIR; p++)
                     if (hz_islot(p) >= 0) sR[hz_islot(p)][b] = cur[q].R[p];
@@ -1097,8 +1101,8 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
         const bool outO = O >= k0 && O < k1;
         if (outO) {
             const double dcz = sDz[O - k0];
-            const T idz = sIz[O - k0];
-            const bool z1 = zero1(zper, O, n2);
+            // idz / 2, or 0 where the k-gradients of plane O are zeroed
+            const T hz = zero1(zper, O, n2) ? (T)0 : sIz[O - k0] / (T)2;
 #pragma unroll
             for (int q = 0; q < 2; q++) {
                 if (!bv[q]) continue;
@@ -1108,8 +1112,8 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                     T Xr = 0, Xm = 0, Hb = 0, He = 0;
                     if (xq[f] >= 2 && xq[f] <= g.n[f] - 3) {
                         // ca = p - e_k (plane P-2), cb = p + e_k (plane P)
-                        const T dv = sel0<T>(z1, (cur[q].v[f] - hv[q][1][f]) * idz / (T)2);
-                        const T db = sel0<T>(z1, (cur[q].b[f] - hv[q][1][2 + f]) * idz / (T)2);
+                        const T dv = (cur[q].v[f] - hv[q][1][f]) * hz;
+                        const T db = (cur[q].b[f] - hv[q][1][2 + f]) * hz;
                         const T Km = (T)(dcz * ph.chyp[LY::M0 + f]);
                         const T sv = Km * (hr[q][0][f] + hr[q][1][f]);
                         const T G = dv * sv / (T)4;
@@ -1156,10 +1160,9 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                 if (k == 4) return oz[t][2];
                 return hr[0][t][2 + k - 5];
             };
-            const int xd = O;
             const double dcz = sDz[O - k0];
             const T idxd = sIz[O - k0];
-            const bool z1 = zero1(zper, xd, n2);
+            const T gz = zero1(zper, O, n2) ? (T)0 : idxd;
             auto nuRL = [&](int t, int field, T &nR, T &nL) {
                 const T K = (T)(dcz * ph.chyp[field]);
                 nR = K * Z(0, t);
@@ -1167,23 +1170,24 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
             };
             T nR, nL, gR, gL;
             const T rc = W(0, 0);
-            const double rhoc = (double)rc + orb[0];
             nuRL(0, LY::RHO, nR, nL);
-            gR = sel0<T>(z1, (W(2, 0) - rc) * idxd);
-            gL = sel0<T>(z1, (rc - W(1, 0)) * idxd);
+            gR = (W(2, 0) - rc) * gz;
+            gL = (rc - W(1, 0)) * gz;
             rA[LY::RHO] += (nR * gR - nL * gL) * idxd;
             nuRL(1, LY::EN, nR, nL);
-            gR = sel0<T>(z1, (W(2, 1) - W(0, 1)) * idxd);
-            gL = sel0<T>(z1, (W(0, 1) - W(1, 1)) * idxd);
+            gR = (W(2, 1) - W(0, 1)) * gz;
+            gL = (W(0, 1) - W(1, 1)) * gz;
             rA[LY::EN] += (nR * gR - nL * gL) * idxd;
-            const T rR = (T)((rhoc + (double)W(2, 0) + cur[0].rb) / 2);
-            const T rL = (T)((rhoc + (double)W(1, 0) + orb[1]) / 2);
+            // mean full density over the faces (planes P-1, P and P-1, P-2)
+            const T fc = hn[0][0];
+            const T rR = (fc + (T)((double)cur[0].r + cur[0].rb)) / (T)2;
+            const T rL = (fc + of2) / (T)2;
 #pragma unroll
             for (int f = 0; f < 3; f++) {
                 nuRL(2 + f, LY::M0 + f, nR, nL);
                 const T vc = W(0, 2 + f);
-                gR = sel0<T>(z1, (W(2, 2 + f) - vc) * idxd);
-                gL = sel0<T>(z1, (vc - W(1, 2 + f)) * idxd);
+                gR = (W(2, 2 + f) - vc) * gz;
+                gL = (vc - W(1, 2 + f)) * gz;
                 const T mc = W(0, 5 + f);
                 const T mR = (mc + W(2, 5 + f)) / (T)2, mL = (mc + W(1, 5 + f)) / (T)2;
                 const T k2 = (f == 2) ? (T)1 : (T)0.5;
@@ -1194,8 +1198,8 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
             for (int f = 0; f < 2; f++) {
                 nuRL(5 + f, LY::B0 + f, nR, nL);
                 const T bc_ = W(0, 8 + f), br = W(2, 8 + f), bl = W(1, 8 + f);
-                gR = sel0<T>(z1, (br - bc_) * idxd);
-                gL = sel0<T>(z1, (bc_ - bl) * idxd);
+                gR = (br - bc_) * gz;
+                gL = (bc_ - bl) * gz;
                 const T bR = (bc_ + br) / (T)2, bL = (bc_ + bl) / (T)2;
                 rA[LY::B0 + f] += (gR * nR - gL * nL) * idxd;
                 rA[LY::EN] += (bR * gR * nR - bL * gL * nL) * idxd;
@@ -1203,6 +1207,7 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
         }
         __builtin_amdgcn_sched_barrier(0);
         // the histories move up one plane
+        of2 = hn[0][0];
 #pragma unroll
         for (int q = 0; q < 2; q++)
             if (bv[q]) {
@@ -1226,7 +1231,6 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
             for (int f = 0; f < 6; f++) ow[1][f] = ow[0][f];
 #pragma unroll
             for (int f = 0; f < 3; f++) oz[1][f] = oz[0][f];
-            orb[1] = orb[0];
             ow[0][0] = cur[0].r;
             ow[0][1] = cur[0].e;
             ow[0][2] = cur[0].v[2];
@@ -1235,7 +1239,6 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
             oz[0][0] = cur[0].R[LY::prho(2)];
             oz[0][1] = cur[0].R[LY::pe(2)];
             oz[0][2] = cur[0].R[LY::pv(2, 2)];
-            orb[0] = cur[0].rb;
         }
         // The next plane's loads, in flight through the rest of this plane.
         // The own column's come last and are issued on every path (the last
@@ -1251,15 +1254,13 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
             // ---- in-plane terms of output P
             if (P >= k0 && P < k1) {
                 const T rc = vw.U(LY::RHO, 0, 0);
-                const double rhoc = (double)rc + vw.rhob(0, 0);
+                const T fc = vw.rhof(0, 0);
                 const T eic = vw.A(AX::EINT, 0, 0);
 #pragma unroll
                 for (int d = 0; d < 2; d++) {
                     const int ri = d == 0, rj = d == 1;
-                    const int xd = xs[d], nd = g.n[d];
                     const double dcd = dc[d];
-                    const T idxd = id[d];
-                    const bool z1 = zero1(g.per[d], xd, nd);
+                    const T idxd = id[d], gxd = gx[d], hxd = gxd / (T)2;
                     auto nuRL = [&](int pair, int field, T &nR, T &nL) {
                         const T K = (T)(dcd * ph.chyp[field]);
                         nR = K * vw.Rp(pair, 0, 0);
@@ -1267,21 +1268,21 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                     };
                     T nR, nL, gR, gL;
                     nuRL(LY::prho(d), LY::RHO, nR, nL);
-                    gR = sel0<T>(z1, (vw.U(LY::RHO, ri, rj) - rc) * idxd);
-                    gL = sel0<T>(z1, (rc - vw.U(LY::RHO, -ri, -rj)) * idxd);
+                    gR = (vw.U(LY::RHO, ri, rj) - rc) * gxd;
+                    gL = (rc - vw.U(LY::RHO, -ri, -rj)) * gxd;
                     rB[LY::RHO] += (nR * gR - nL * gL) * idxd;
                     nuRL(LY::pe(d), LY::EN, nR, nL);
-                    gR = sel0<T>(z1, (vw.A(AX::EINT, ri, rj) - eic) * idxd);
-                    gL = sel0<T>(z1, (eic - vw.A(AX::EINT, -ri, -rj)) * idxd);
+                    gR = (vw.A(AX::EINT, ri, rj) - eic) * gxd;
+                    gL = (eic - vw.A(AX::EINT, -ri, -rj)) * gxd;
                     rB[LY::EN] += (nR * gR - nL * gL) * idxd;
-                    const T rR = (T)((rhoc + (double)vw.U(LY::RHO, ri, rj) + vw.rhob(ri, rj)) / 2);
-                    const T rL = (T)((rhoc + (double)vw.U(LY::RHO, -ri, -rj) + vw.rhob(-ri, -rj)) / 2);
+                    const T rR = (fc + vw.rhof(ri, rj)) / (T)2;
+                    const T rL = (fc + vw.rhof(-ri, -rj)) / (T)2;
 #pragma unroll
                     for (int f = 0; f < 3; f++) {
                         nuRL(LY::pv(f, d), LY::M0 + f, nR, nL);
                         const T vc = vw.A(AX::V0 + f, 0, 0);
-                        gR = sel0<T>(z1, (vw.A(AX::V0 + f, ri, rj) - vc) * idxd);
-                        gL = sel0<T>(z1, (vc - vw.A(AX::V0 + f, -ri, -rj)) * idxd);
+                        gR = (vw.A(AX::V0 + f, ri, rj) - vc) * gxd;
+                        gL = (vc - vw.A(AX::V0 + f, -ri, -rj)) * gxd;
                         const T mc = vw.U(LY::M0 + f, 0, 0);
                         const T mR = (mc + vw.U(LY::M0 + f, ri, rj)) / (T)2;
                         const T mL = (mc + vw.U(LY::M0 + f, -ri, -rj)) / (T)2;
@@ -1296,8 +1297,8 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                         nuRL(LY::pb(f, d), LY::B0 + f, nR, nL);
                         const T bc_ = vw.U(LY::B0 + f, 0, 0);
                         const T br = vw.U(LY::B0 + f, ri, rj), bl = vw.U(LY::B0 + f, -ri, -rj);
-                        gR = sel0<T>(z1, (br - bc_) * idxd);
-                        gL = sel0<T>(z1, (bc_ - bl) * idxd);
+                        gR = (br - bc_) * gxd;
+                        gL = (bc_ - bl) * gxd;
                         const T bR = (bc_ + br) / (T)2, bL = (bc_ + bl) / (T)2;
                         rB[LY::B0 + f] += (gR * nR - gL * nL) * idxd;
                         rB[LY::EN] += (bR * gR * nR - bL * gL * nL) * idxd;
@@ -1316,15 +1317,13 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                             const int xfn = xf + sg;
                             if (xfn < 2 || xfn > nf - 3) continue;
                             const int ai = ni - ri, aj = nj - rj, bi = ni + ri, bj = nj + rj;
-                            const T dv = sel0<T>(z1, (vw.A(AX::V0 + f, bi, bj) - vw.A(AX::V0 + f, ai, aj)) *
-                                                  idxd / (T)2);
-                            const T db = sel0<T>(z1, (vw.U(LY::B0 + f, bi, bj) - vw.U(LY::B0 + f, ai, aj)) *
-                                                  idxd / (T)2);
+                            const T dv = (vw.A(AX::V0 + f, bi, bj) - vw.A(AX::V0 + f, ai, aj)) * hxd;
+                            const T db = (vw.U(LY::B0 + f, bi, bj) - vw.U(LY::B0 + f, ai, aj)) * hxd;
                             {
                                 const T K = (T)(dcd * ph.chyp[LY::M0 + f]);
                                 const T s2_ = K * (vw.Rp(LY::pv(f, d), ni, nj) + vw.Rp(LY::pv(f, d), ai, aj));
                                 const T G = dv * s2_ / (T)4;
-                                const T rhon = (T)((double)vw.U(LY::RHO, ni, nj) + vw.rhob(ni, nj));
+                                const T rhon = vw.rhof(ni, nj);
                                 Xr[side] = rhon * G;
                                 Xm[side] = G * vw.U(LY::M0 + d, ni, nj);
                             }
@@ -1351,17 +1350,14 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
 #pragma unroll
                 for (int d = 0; d < 2; d++) {
                     const int ri = d == 0, rj = d == 1;
-                    const int xd = xs[d], nd = g.n[d];
                     const double dcd = dc[d];
-                    const T idxd = id[d];
-                    const bool z1 = zero1(g.per[d], xd, nd);
-                    const T dv = sel0<T>(z1, (vw.A(AX::V0 + 2, ri, rj) - vw.A(AX::V0 + 2, -ri, -rj)) * idxd / (T)2);
-                    const T db = sel0<T>(z1, (vw.U(LY::B0 + 2, ri, rj) - vw.U(LY::B0 + 2, -ri, -rj)) * idxd / (T)2);
+                    const T hxd = gx[d] / (T)2;
+                    const T dv = (vw.A(AX::V0 + 2, ri, rj) - vw.A(AX::V0 + 2, -ri, -rj)) * hxd;
+                    const T db = (vw.U(LY::B0 + 2, ri, rj) - vw.U(LY::B0 + 2, -ri, -rj)) * hxd;
                     const T Km = (T)(dcd * ph.chyp[LY::M0 + 2]);
                     const T sv = Km * (vw.Rp(LY::pv(2, d), 0, 0) + vw.Rp(LY::pv(2, d), -ri, -rj));
                     const T G = dv * sv / (T)4;
-                    const T rhon = (T)((double)vw.U(LY::RHO, 0, 0) + vw.rhob(0, 0));
-                    const T Xr = rhon * G, Xm = G * vw.U(LY::M0 + d, 0, 0);
+                    const T Xr = vw.rhof(0, 0) * G, Xm = G * vw.U(LY::M0 + d, 0, 0);
                     const T Kb = (T)(dcd * ph.chyp[LY::B0 + 2]);
                     const T sb = Kb * (vw.Rp(LY::pb(2, d), 0, 0) + vw.Rp(LY::pb(2, d), -ri, -rj));
                     const T H = db * sb / (T)2;
@@ -1429,10 +1425,10 @@ k_hyper_zs_f(Geo g, ZMap zm, Phys ph, const float *__restrict__ uin, const float
              const unsigned long long *__restrict__ cmb, float *__restrict__ dUh)
 {
     __shared__ float sU[7][HB_NB], sA[4][HB_NB], sR[HZ_NIP][HB_NB], sX[8][HB_NB];
-    __shared__ double sRb[HB_NB];
+    __shared__ float sRf[HB_NB];
     __shared__ double sDz[ZS_KZMAX]; // dz cmax_z of the output planes
     __shared__ float sIz[ZS_KZMAX];  // 1/dz of the output planes
-    hyper_zs_body<BGP, float>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRb, sR, sX, sDz, sIz);
+    hyper_zs_body<BGP, float>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRf, sR, sX, sDz, sIz);
 }
 
 template <bool BGP>
@@ -1443,9 +1439,9 @@ k_hyper_zs_d(Geo g, ZMap zm, Phys ph, const double *__restrict__ uin,
              double *__restrict__ dUh)
 {
     __shared__ double sU[7][HB_NB], sA[4][HB_NB], sR[HZ_NIP][HB_NB], sX[8][HB_NB];
-    __shared__ double sRb[HB_NB];
+    __shared__ double sRf[HB_NB];
     __shared__ double sDz[ZS_KZMAX], sIz[ZS_KZMAX];
-    hyper_zs_body<BGP, double>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRb, sR, sX, sDz, sIz);
+    hyper_zs_body<BGP, double>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRf, sR, sX, sDz, sIz);
 }
 
 // the hyperdiffusive sources of every interior cell into dUh (the stage
