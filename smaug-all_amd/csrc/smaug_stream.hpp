@@ -955,7 +955,8 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                                               const unsigned long long *__restrict__ cmb,
                                               T *__restrict__ dUh, T (*sU)[HB_NB],
                                               T (*sA)[HB_NB], T *sRf, T (*sR)[HB_NB],
-                                              T (*sX)[HB_NB], double *sDz, T *sIz)
+                                              T (*sX)[HB_NB], double *sDz, T *sIz,
+                                              T (*sCx)[HB_TI], T (*sCy)[HB_TJ], T (*sCz)[ZS_KZMAX])
 {
     typedef Lay<3> LY;
     typedef Aux<3> AX;
@@ -991,9 +992,38 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
         gx[0] = zero1(g.per[0], i, n0) ? (T)0 : id[0];
         gx[1] = zero1(g.per[1], j, n1) ? (T)0 : id[1];
     }
+    // Per field slot q (rho, e, m_0..2, b_0..2) the factor of the diagonal
+    // terms of a cell along an axis: (dx cmax chyp_q), rounded to T, times
+    // 1/dx of the gradients (0 where zero1) and 1/dx of the divergence --
+    // sCx per column along i, sCy per row along j, sCz per output plane
+    // along k (slots up to b_1).
+    constexpr int CF[8] = {LY::RHO, LY::EN, LY::M0, LY::M0 + 1, LY::M0 + 2, LY::B0, LY::B0 + 1, LY::B0 + 2};
     for (int t = threadIdx.x; t < k1 - k0; t += 256) {
-        sDz[t] = g.dx[2][k0 + t] * cmax[2];
-        sIz[t] = (T)g.idx[2][k0 + t];
+        const int kk = k0 + t;
+        const double dz = g.dx[2][kk] * cmax[2];
+        const T izt = (T)g.idx[2][kk];
+        const T s = (zero1(g.per[2], kk, n2) ? (T)0 : izt) * izt;
+        sDz[t] = dz;
+        sIz[t] = izt;
+#pragma unroll
+        for (int q = 0; q < 7; q++) sCz[q][t] = (T)(dz * ph.chyp[CF[q]]) * s;
+    }
+    if (threadIdx.x < HB_TI + HB_TJ) {
+        const bool alongi = threadIdx.x < HB_TI;
+        const int a = alongi ? 0 : 1;
+        const int x = alongi ? i0 + threadIdx.x : j0 + (threadIdx.x - HB_TI);
+        const bool in = x <= g.n[a] - 3;
+        const double dd = in ? g.dx[a][x] * cmax[a] : 0.0;
+        const T ix = in ? (T)g.idx[a][x] : (T)0;
+        const T s = (in && !zero1(g.per[a], x, g.n[a]) ? ix : (T)0) * ix;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const T c = (T)(dd * ph.chyp[CF[q]]) * s;
+            if (alongi)
+                sCx[q][threadIdx.x] = c;
+            else
+                sCy[q][threadIdx.x - HB_TI] = c;
+        }
     }
     __syncthreads();
     // box cells of this thread: 0 = its own column, 1 = a halo column (if any)
@@ -1160,49 +1190,41 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                 if (k == 4) return oz[t][2];
                 return hr[0][t][2 + k - 5];
             };
-            const double dcz = sDz[O - k0];
-            const T idxd = sIz[O - k0];
-            const T gz = zero1(zper, O, n2) ? (T)0 : idxd;
-            auto nuRL = [&](int t, int field, T &nR, T &nL) {
-                const T K = (T)(dcz * ph.chyp[field]);
-                nR = K * Z(0, t);
-                nL = K * Z(1, t);
-            };
-            T nR, nL, gR, gL;
+            // the factors of the plane (as sCx along i)
+            const T *cz = &sCz[0][O - k0];
+            auto C = [&](int q) -> T { return cz[q * ZS_KZMAX]; };
+            T pR, pL;
             const T rc = W(0, 0);
-            nuRL(0, LY::RHO, nR, nL);
-            gR = (W(2, 0) - rc) * gz;
-            gL = (rc - W(1, 0)) * gz;
-            rA[LY::RHO] += (nR * gR - nL * gL) * idxd;
-            nuRL(1, LY::EN, nR, nL);
-            gR = (W(2, 1) - W(0, 1)) * gz;
-            gL = (W(0, 1) - W(1, 1)) * gz;
-            rA[LY::EN] += (nR * gR - nL * gL) * idxd;
-            // mean full density over the faces (planes P-1, P and P-1, P-2)
+            pR = Z(0, 0) * (W(2, 0) - rc);
+            pL = Z(1, 0) * (rc - W(1, 0));
+            rA[LY::RHO] += (pR - pL) * C(0);
+            pR = Z(0, 1) * (W(2, 1) - W(0, 1));
+            pL = Z(1, 1) * (W(0, 1) - W(1, 1));
+            rA[LY::EN] += (pR - pL) * C(1);
+            // twice the mean full density over the faces (planes P-1, P and
+            // P-1, P-2)
             const T fc = hn[0][0];
-            const T rR = (fc + (T)((double)cur[0].r + cur[0].rb)) / (T)2;
-            const T rL = (fc + of2) / (T)2;
+            const T rR = fc + (T)((double)cur[0].r + cur[0].rb);
+            const T rL = fc + of2;
 #pragma unroll
             for (int f = 0; f < 3; f++) {
-                nuRL(2 + f, LY::M0 + f, nR, nL);
                 const T vc = W(0, 2 + f);
-                gR = (W(2, 2 + f) - vc) * gz;
-                gL = (vc - W(1, 2 + f)) * gz;
+                pR = Z(0, 2 + f) * (W(2, 2 + f) - vc);
+                pL = Z(1, 2 + f) * (vc - W(1, 2 + f));
                 const T mc = W(0, 5 + f);
-                const T mR = (mc + W(2, 5 + f)) / (T)2, mL = (mc + W(1, 5 + f)) / (T)2;
-                const T k2 = (f == 2) ? (T)1 : (T)0.5;
-                rA[LY::M0 + f] += k2 * (rR * nR * gR - rL * nL * gL) * idxd;
-                rA[LY::EN] += k2 * (mR * nR * gR - mL * nL * gL) * idxd;
+                const T mR = mc + W(2, 5 + f), mL = mc + W(1, 5 + f);
+                const T ck = C(2 + f) * ((f == 2) ? (T)0.5 : (T)0.25);
+                rA[LY::M0 + f] += (rR * pR - rL * pL) * ck;
+                rA[LY::EN] += (mR * pR - mL * pL) * ck;
             }
 #pragma unroll
             for (int f = 0; f < 2; f++) {
-                nuRL(5 + f, LY::B0 + f, nR, nL);
                 const T bc_ = W(0, 8 + f), br = W(2, 8 + f), bl = W(1, 8 + f);
-                gR = (br - bc_) * gz;
-                gL = (bc_ - bl) * gz;
-                const T bR = (bc_ + br) / (T)2, bL = (bc_ + bl) / (T)2;
-                rA[LY::B0 + f] += (gR * nR - gL * nL) * idxd;
-                rA[LY::EN] += (bR * gR * nR - bL * gL * nL) * idxd;
+                pR = Z(0, 5 + f) * (br - bc_);
+                pL = Z(1, 5 + f) * (bc_ - bl);
+                const T cb = C(5 + f);
+                rA[LY::B0 + f] += (pR - pL) * cb;
+                rA[LY::EN] += ((bc_ + br) * pR - (bc_ + bl) * pL) * (cb * (T)0.5);
             }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -1260,48 +1282,44 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                 for (int d = 0; d < 2; d++) {
                     const int ri = d == 0, rj = d == 1;
                     const double dcd = dc[d];
-                    const T idxd = id[d], gxd = gx[d], hxd = gxd / (T)2;
-                    auto nuRL = [&](int pair, int field, T &nR, T &nL) {
-                        const T K = (T)(dcd * ph.chyp[field]);
-                        nR = K * vw.Rp(pair, 0, 0);
-                        nL = K * vw.Rp(pair, -ri, -rj);
-                    };
-                    T nR, nL, gR, gL;
-                    nuRL(LY::prho(d), LY::RHO, nR, nL);
-                    gR = (vw.U(LY::RHO, ri, rj) - rc) * gxd;
-                    gL = (rc - vw.U(LY::RHO, -ri, -rj)) * gxd;
-                    rB[LY::RHO] += (nR * gR - nL * gL) * idxd;
-                    nuRL(LY::pe(d), LY::EN, nR, nL);
-                    gR = (vw.A(AX::EINT, ri, rj) - eic) * gxd;
-                    gL = (eic - vw.A(AX::EINT, -ri, -rj)) * gxd;
-                    rB[LY::EN] += (nR * gR - nL * gL) * idxd;
-                    const T rR = (fc + vw.rhof(ri, rj)) / (T)2;
-                    const T rL = (fc + vw.rhof(-ri, -rj)) / (T)2;
+                    const T hxd = gx[d] / (T)2;
+                    // nu/R of field slot q times the gradient and divergence
+                    // factors of this cell along d (0 where zero1)
+                    auto C = [&](int q) -> T { return d == 0 ? sCx[q][ti] : sCy[q][tj]; };
+                    T pR, pL;
+                    pR = vw.Rp(LY::prho(d), 0, 0) * (vw.U(LY::RHO, ri, rj) - rc);
+                    pL = vw.Rp(LY::prho(d), -ri, -rj) * (rc - vw.U(LY::RHO, -ri, -rj));
+                    rB[LY::RHO] += (pR - pL) * C(0);
+                    pR = vw.Rp(LY::pe(d), 0, 0) * (vw.A(AX::EINT, ri, rj) - eic);
+                    pL = vw.Rp(LY::pe(d), -ri, -rj) * (eic - vw.A(AX::EINT, -ri, -rj));
+                    rB[LY::EN] += (pR - pL) * C(1);
+                    // twice the mean full density over the two faces
+                    const T rR = fc + vw.rhof(ri, rj);
+                    const T rL = fc + vw.rhof(-ri, -rj);
 #pragma unroll
                     for (int f = 0; f < 3; f++) {
-                        nuRL(LY::pv(f, d), LY::M0 + f, nR, nL);
                         const T vc = vw.A(AX::V0 + f, 0, 0);
-                        gR = (vw.A(AX::V0 + f, ri, rj) - vc) * gxd;
-                        gL = (vc - vw.A(AX::V0 + f, -ri, -rj)) * gxd;
+                        pR = vw.Rp(LY::pv(f, d), 0, 0) * (vw.A(AX::V0 + f, ri, rj) - vc);
+                        pL = vw.Rp(LY::pv(f, d), -ri, -rj) * (vc - vw.A(AX::V0 + f, -ri, -rj));
                         const T mc = vw.U(LY::M0 + f, 0, 0);
-                        const T mR = (mc + vw.U(LY::M0 + f, ri, rj)) / (T)2;
-                        const T mL = (mc + vw.U(LY::M0 + f, -ri, -rj)) / (T)2;
-                        const T k2 = (f == d) ? (T)1 : (T)0.5;
-                        rB[LY::M0 + f] += k2 * (rR * nR * gR - rL * nL * gL) * idxd;
-                        rB[LY::EN] += k2 * (mR * nR * gR - mL * nL * gL) * idxd;
+                        const T mR = mc + vw.U(LY::M0 + f, ri, rj);
+                        const T mL = mc + vw.U(LY::M0 + f, -ri, -rj);
+                        // the face means' 1/2 times 1 (f == d) or 1/2
+                        const T ck = C(2 + f) * ((f == d) ? (T)0.5 : (T)0.25);
+                        rB[LY::M0 + f] += (rR * pR - rL * pL) * ck;
+                        rB[LY::EN] += (mR * pR - mL * pL) * ck;
                     }
                     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int f = 0; f < 3; f++) {
                         if (f == d) continue;
-                        nuRL(LY::pb(f, d), LY::B0 + f, nR, nL);
                         const T bc_ = vw.U(LY::B0 + f, 0, 0);
                         const T br = vw.U(LY::B0 + f, ri, rj), bl = vw.U(LY::B0 + f, -ri, -rj);
-                        gR = (br - bc_) * gxd;
-                        gL = (bc_ - bl) * gxd;
-                        const T bR = (bc_ + br) / (T)2, bL = (bc_ + bl) / (T)2;
-                        rB[LY::B0 + f] += (gR * nR - gL * nL) * idxd;
-                        rB[LY::EN] += (bR * gR * nR - bL * gL * nL) * idxd;
+                        pR = vw.Rp(LY::pb(f, d), 0, 0) * (br - bc_);
+                        pL = vw.Rp(LY::pb(f, d), -ri, -rj) * (bc_ - bl);
+                        const T cb = C(5 + f);
+                        rB[LY::B0 + f] += (pR - pL) * cb;
+                        rB[LY::EN] += ((bc_ + br) * pR - (bc_ + bl) * pL) * (cb * (T)0.5);
                     }
                     __builtin_amdgcn_sched_barrier(0);
                     // off-diagonal pair (f, d) with f the other in-plane dim
@@ -1428,7 +1446,9 @@ k_hyper_zs_f(Geo g, ZMap zm, Phys ph, const float *__restrict__ uin, const float
     __shared__ float sRf[HB_NB];
     __shared__ double sDz[ZS_KZMAX]; // dz cmax_z of the output planes
     __shared__ float sIz[ZS_KZMAX];  // 1/dz of the output planes
-    hyper_zs_body<BGP, float>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRf, sR, sX, sDz, sIz);
+    __shared__ float sCx[8][HB_TI], sCy[8][HB_TJ], sCz[7][ZS_KZMAX];
+    hyper_zs_body<BGP, float>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRf, sR, sX, sDz, sIz,
+                              sCx, sCy, sCz);
 }
 
 template <bool BGP>
@@ -1441,7 +1461,9 @@ k_hyper_zs_d(Geo g, ZMap zm, Phys ph, const double *__restrict__ uin,
     __shared__ double sU[7][HB_NB], sA[4][HB_NB], sR[HZ_NIP][HB_NB], sX[8][HB_NB];
     __shared__ double sRf[HB_NB];
     __shared__ double sDz[ZS_KZMAX], sIz[ZS_KZMAX];
-    hyper_zs_body<BGP, double>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRf, sR, sX, sDz, sIz);
+    __shared__ double sCx[8][HB_TI], sCy[8][HB_TJ], sCz[7][ZS_KZMAX];
+    hyper_zs_body<BGP, double>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRf, sR, sX, sDz, sIz,
+                               sCx, sCy, sCz);
 }
 
 // the hyperdiffusive sources of every interior cell into dUh (the stage
