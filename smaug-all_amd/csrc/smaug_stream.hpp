@@ -955,8 +955,9 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                                               const unsigned long long *__restrict__ cmb,
                                               T *__restrict__ dUh, T (*sU)[HB_NB],
                                               T (*sA)[HB_NB], T *sRf, T (*sR)[HB_NB],
-                                              T (*sX)[HB_NB], double *sDz, T *sIz,
-                                              T (*sCx)[HB_TI], T (*sCy)[HB_TJ], T (*sCz)[ZS_KZMAX])
+                                              T (*sX)[HB_NB], T *sIz, T (*sCx)[HB_TI],
+                                              T (*sCy)[HB_TJ], T (*sCz)[ZS_KZMAX],
+                                              T (*sKx)[HB_TI], T (*sKy)[HB_TJ], T (*sKz)[ZS_KZMAX])
 {
     typedef Lay<3> LY;
     typedef Aux<3> AX;
@@ -974,19 +975,16 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
     double cmax[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) cmax[d] = __longlong_as_double((long long)cmb[d]);
-    // Metric factors, read once per block: the in-plane ones per thread into
-    // registers (dc = dx cmax, id = 1/dx), those of the chunk's output planes
-    // into LDS (a global load inside the plane loop would be waited for
+    // Metric factors, read once per block: 1/dx of the own cell in-plane
+    // into registers, the rest into LDS tables per column, row and output
+    // plane (a global load inside the plane loop would be waited for
     // together with the prefetched next plane, as vector memory operations
     // retire in order).  nu = (dx cmax) chyp R as in hyper_terms.
-    double dc[2] = {0, 0};
     T id[2] = {0, 0};
     // 1/dx, or 0 where the gradients of a cell along that axis are zeroed
     // (zero1): the gradients below take their zeroing from this factor
     T gx[2] = {0, 0};
     if (act) {
-        dc[0] = g.dx[0][i] * cmax[0];
-        dc[1] = g.dx[1][j] * cmax[1];
         id[0] = (T)g.idx[0][i];
         id[1] = (T)g.idx[1][j];
         gx[0] = zero1(g.per[0], i, n0) ? (T)0 : id[0];
@@ -996,17 +994,24 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
     // terms of a cell along an axis: (dx cmax chyp_q), rounded to T, times
     // 1/dx of the gradients (0 where zero1) and 1/dx of the divergence --
     // sCx per column along i, sCy per row along j, sCz per output plane
-    // along k (slots up to b_1).
+    // along k (slots up to b_1).  The off-diagonal terms take nu/R alone,
+    // (dx cmax chyp) rounded to T: sKx / sKy for m and b of the other
+    // in-plane axis and of k (slots m_o, b_o, m_2, b_2), sKz for m_0, m_1,
+    // b_0, b_1 per output plane.
     constexpr int CF[8] = {LY::RHO, LY::EN, LY::M0, LY::M0 + 1, LY::M0 + 2, LY::B0, LY::B0 + 1, LY::B0 + 2};
     for (int t = threadIdx.x; t < k1 - k0; t += 256) {
         const int kk = k0 + t;
         const double dz = g.dx[2][kk] * cmax[2];
         const T izt = (T)g.idx[2][kk];
         const T s = (zero1(g.per[2], kk, n2) ? (T)0 : izt) * izt;
-        sDz[t] = dz;
         sIz[t] = izt;
 #pragma unroll
         for (int q = 0; q < 7; q++) sCz[q][t] = (T)(dz * ph.chyp[CF[q]]) * s;
+#pragma unroll
+        for (int f = 0; f < 2; f++) {
+            sKz[f][t] = (T)(dz * ph.chyp[LY::M0 + f]);
+            sKz[2 + f][t] = (T)(dz * ph.chyp[LY::B0 + f]);
+        }
     }
     if (threadIdx.x < HB_TI + HB_TJ) {
         const bool alongi = threadIdx.x < HB_TI;
@@ -1023,6 +1028,16 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                 sCx[q][threadIdx.x] = c;
             else
                 sCy[q][threadIdx.x - HB_TI] = c;
+        }
+        const int o = 1 - a; // the other in-plane axis
+        const int KF[4] = {LY::M0 + o, LY::B0 + o, LY::M0 + 2, LY::B0 + 2};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const T kq = (T)(dd * ph.chyp[KF[q]]);
+            if (alongi)
+                sKx[q][threadIdx.x] = kq;
+            else
+                sKy[q][threadIdx.x - HB_TI] = kq;
         }
     }
     __syncthreads();
@@ -1130,7 +1145,6 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
         const int O = P - 1;
         const bool outO = O >= k0 && O < k1;
         if (outO) {
-            const double dcz = sDz[O - k0];
             // idz / 2, or 0 where the k-gradients of plane O are zeroed
             const T hz = zero1(zper, O, n2) ? (T)0 : sIz[O - k0] / (T)2;
 #pragma unroll
@@ -1144,12 +1158,12 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                         // ca = p - e_k (plane P-2), cb = p + e_k (plane P)
                         const T dv = (cur[q].v[f] - hv[q][1][f]) * hz;
                         const T db = (cur[q].b[f] - hv[q][1][2 + f]) * hz;
-                        const T Km = (T)(dcz * ph.chyp[LY::M0 + f]);
+                        const T Km = sKz[f][O - k0];
                         const T sv = Km * (hr[q][0][f] + hr[q][1][f]);
                         const T G = dv * sv / (T)4;
                         Xr = hn[q][0] * G;
                         Xm = G * hn[q][1];
-                        const T Kb = (T)(dcz * ph.chyp[LY::B0 + f]);
+                        const T Kb = sKz[2 + f][O - k0];
                         const T sb = Kb * (hr[q][0][2 + f] + hr[q][1][2 + f]);
                         const T H = db * sb / (T)2;
                         Hb = H;
@@ -1281,8 +1295,8 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
 #pragma unroll
                 for (int d = 0; d < 2; d++) {
                     const int ri = d == 0, rj = d == 1;
-                    const double dcd = dc[d];
                     const T hxd = gx[d] / (T)2;
+                    auto K = [&](int q) -> T { return d == 0 ? sKx[q][ti] : sKy[q][tj]; };
                     // nu/R of field slot q times the gradient and divergence
                     // factors of this cell along d (0 where zero1)
                     auto C = [&](int q) -> T { return d == 0 ? sCx[q][ti] : sCy[q][tj]; };
@@ -1338,16 +1352,14 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                             const T dv = (vw.A(AX::V0 + f, bi, bj) - vw.A(AX::V0 + f, ai, aj)) * hxd;
                             const T db = (vw.U(LY::B0 + f, bi, bj) - vw.U(LY::B0 + f, ai, aj)) * hxd;
                             {
-                                const T K = (T)(dcd * ph.chyp[LY::M0 + f]);
-                                const T s2_ = K * (vw.Rp(LY::pv(f, d), ni, nj) + vw.Rp(LY::pv(f, d), ai, aj));
+                                const T s2_ = K(0) * (vw.Rp(LY::pv(f, d), ni, nj) + vw.Rp(LY::pv(f, d), ai, aj));
                                 const T G = dv * s2_ / (T)4;
                                 const T rhon = vw.rhof(ni, nj);
                                 Xr[side] = rhon * G;
                                 Xm[side] = G * vw.U(LY::M0 + d, ni, nj);
                             }
                             {
-                                const T K = (T)(dcd * ph.chyp[LY::B0 + f]);
-                                const T s2_ = K * (vw.Rp(LY::pb(f, d), ni, nj) + vw.Rp(LY::pb(f, d), ai, aj));
+                                const T s2_ = K(1) * (vw.Rp(LY::pb(f, d), ni, nj) + vw.Rp(LY::pb(f, d), ai, aj));
                                 const T H = db * s2_ / (T)2;
                                 Hb[side] = H;
                                 He[side] = H * vw.U(LY::B0 + d, ni, nj);
@@ -1368,15 +1380,14 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
 #pragma unroll
                 for (int d = 0; d < 2; d++) {
                     const int ri = d == 0, rj = d == 1;
-                    const double dcd = dc[d];
                     const T hxd = gx[d] / (T)2;
                     const T dv = (vw.A(AX::V0 + 2, ri, rj) - vw.A(AX::V0 + 2, -ri, -rj)) * hxd;
                     const T db = (vw.U(LY::B0 + 2, ri, rj) - vw.U(LY::B0 + 2, -ri, -rj)) * hxd;
-                    const T Km = (T)(dcd * ph.chyp[LY::M0 + 2]);
+                    const T Km = d == 0 ? sKx[2][ti] : sKy[2][tj];
                     const T sv = Km * (vw.Rp(LY::pv(2, d), 0, 0) + vw.Rp(LY::pv(2, d), -ri, -rj));
                     const T G = dv * sv / (T)4;
                     const T Xr = vw.rhof(0, 0) * G, Xm = G * vw.U(LY::M0 + d, 0, 0);
-                    const T Kb = (T)(dcd * ph.chyp[LY::B0 + 2]);
+                    const T Kb = d == 0 ? sKx[3][ti] : sKy[3][tj];
                     const T sb = Kb * (vw.Rp(LY::pb(2, d), 0, 0) + vw.Rp(LY::pb(2, d), -ri, -rj));
                     const T H = db * sb / (T)2;
                     const T He = H * vw.U(LY::B0 + d, 0, 0);
@@ -1444,11 +1455,11 @@ k_hyper_zs_f(Geo g, ZMap zm, Phys ph, const float *__restrict__ uin, const float
 {
     __shared__ float sU[7][HB_NB], sA[4][HB_NB], sR[HZ_NIP][HB_NB], sX[8][HB_NB];
     __shared__ float sRf[HB_NB];
-    __shared__ double sDz[ZS_KZMAX]; // dz cmax_z of the output planes
-    __shared__ float sIz[ZS_KZMAX];  // 1/dz of the output planes
+    __shared__ float sIz[ZS_KZMAX]; // 1/dz of the output planes
     __shared__ float sCx[8][HB_TI], sCy[8][HB_TJ], sCz[7][ZS_KZMAX];
-    hyper_zs_body<BGP, float>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRf, sR, sX, sDz, sIz,
-                              sCx, sCy, sCz);
+    __shared__ float sKx[4][HB_TI], sKy[4][HB_TJ], sKz[4][ZS_KZMAX];
+    hyper_zs_body<BGP, float>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRf, sR, sX, sIz, sCx,
+                              sCy, sCz, sKx, sKy, sKz);
 }
 
 template <bool BGP>
@@ -1460,10 +1471,11 @@ k_hyper_zs_d(Geo g, ZMap zm, Phys ph, const double *__restrict__ uin,
 {
     __shared__ double sU[7][HB_NB], sA[4][HB_NB], sR[HZ_NIP][HB_NB], sX[8][HB_NB];
     __shared__ double sRf[HB_NB];
-    __shared__ double sDz[ZS_KZMAX], sIz[ZS_KZMAX];
+    __shared__ double sIz[ZS_KZMAX];
     __shared__ double sCx[8][HB_TI], sCy[8][HB_TJ], sCz[7][ZS_KZMAX];
-    hyper_zs_body<BGP, double>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRf, sR, sX, sDz, sIz,
-                               sCx, sCy, sCz);
+    __shared__ double sKx[4][HB_TI], sKy[4][HB_TJ], sKz[4][ZS_KZMAX];
+    hyper_zs_body<BGP, double>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRf, sR, sX, sIz, sCx,
+                               sCy, sCz, sKx, sKy, sKz);
 }
 
 // the hyperdiffusive sources of every interior cell into dUh (the stage
