@@ -638,55 +638,51 @@ __device__ __forceinline__ void hyper_terms(const Geo &g, const Phys &ph, const 
         unit<D>(d, 1, ri, rj, rk);
         unit<D>(d, -1, li, lj, lk);
         const int xd = xs[d], nd = g.n[d];
-        const double dxd = g.dx[d][xd];
+        const double dcd = g.dx[d][xd] * cmax[d];
         const T idxd = (T)g.idx[d][xd];
-        const bool z1 = zero1(g.per[d], xd, nd);
-
-        auto nuRL = [&](int pair, int field, T &nR, T &nL) {
-            const T K = (T)(dxd * cmax[d] * ph.chyp[field]);
-            nR = K * vw.Rp(pair, 0, 0, 0);
-            nL = K * vw.Rp(pair, li, lj, lk);
-        };
-        T nR, nL, gR, gL;
+        // 1/dx of the gradients, 0 where the reference zeroes them (zero1);
+        // every term below is (nu/R) (R_R g_R - R_L g_L) times this and the
+        // 1/dx of the divergence, with nu/R = dx cmax chyp rounded to T
+        const T gxd = zero1(g.per[d], xd, nd) ? (T)0 : idxd;
+        const T sxd = gxd * idxd, hxd = gxd / (T)2;
+        auto C = [&](int field) -> T { return (T)(dcd * ph.chyp[field]) * sxd; };
+        T pR, pL;
         // density (hyperdifrhosource1.cu:15-120)
-        nuRL(LY::prho(d), LY::RHO, nR, nL);
-        gR = z1 ? (T)0 : (vw.U(LY::RHO, ri, rj, rk) - rc) * idxd;
-        gL = z1 ? (T)0 : (rc - vw.U(LY::RHO, li, lj, lk)) * idxd;
-        dU[LY::RHO] += (nR * gR - nL * gL) * idxd;
+        pR = vw.Rp(LY::prho(d), 0, 0, 0) * (vw.U(LY::RHO, ri, rj, rk) - rc);
+        pL = vw.Rp(LY::prho(d), li, lj, lk) * (rc - vw.U(LY::RHO, li, lj, lk));
+        dU[LY::RHO] += (pR - pL) * C(LY::RHO);
         // energy (hyperdifesource1.cu:18-330)
-        nuRL(LY::pe(d), LY::EN, nR, nL);
-        gR = z1 ? (T)0 : (vw.A(AX::EINT, ri, rj, rk) - eic) * idxd;
-        gL = z1 ? (T)0 : (eic - vw.A(AX::EINT, li, lj, lk)) * idxd;
-        dU[LY::EN] += (nR * gR - nL * gL) * idxd;
+        pR = vw.Rp(LY::pe(d), 0, 0, 0) * (vw.A(AX::EINT, ri, rj, rk) - eic);
+        pL = vw.Rp(LY::pe(d), li, lj, lk) * (eic - vw.A(AX::EINT, li, lj, lk));
+        dU[LY::EN] += (pR - pL) * C(LY::EN);
         // momentum, diagonal part for every f; twice when f == d
-        // (hyperdifmomsource1.cu:17-533 called for ii1 = 0 and, if f == d, ii1 = 1)
-        const T rR = (T)((rhoc + (double)vw.U(LY::RHO, ri, rj, rk) + vw.rhob(ri, rj, rk)) / 2);
-        const T rL = (T)((rhoc + (double)vw.U(LY::RHO, li, lj, lk) + vw.rhob(li, lj, lk)) / 2);
+        // (hyperdifmomsource1.cu:17-533 called for ii1 = 0 and, if f == d, ii1 = 1);
+        // rR, rL, mR, mL, bR, bL are twice the face means, the 1/2 is in the factor
+        const T rR = (T)(rhoc + (double)vw.U(LY::RHO, ri, rj, rk) + vw.rhob(ri, rj, rk));
+        const T rL = (T)(rhoc + (double)vw.U(LY::RHO, li, lj, lk) + vw.rhob(li, lj, lk));
 #pragma unroll
         for (int f = 0; f < D; f++) {
-            nuRL(LY::pv(f, d), LY::M0 + f, nR, nL);
             const T vc = vw.A(AX::V0 + f, 0, 0, 0);
-            gR = z1 ? (T)0 : (vw.A(AX::V0 + f, ri, rj, rk) - vc) * idxd;
-            gL = z1 ? (T)0 : (vc - vw.A(AX::V0 + f, li, lj, lk)) * idxd;
+            pR = vw.Rp(LY::pv(f, d), 0, 0, 0) * (vw.A(AX::V0 + f, ri, rj, rk) - vc);
+            pL = vw.Rp(LY::pv(f, d), li, lj, lk) * (vc - vw.A(AX::V0 + f, li, lj, lk));
             const T mc = vw.U(LY::M0 + f, 0, 0, 0);
-            const T mR = (mc + vw.U(LY::M0 + f, ri, rj, rk)) / (T)2;
-            const T mL = (mc + vw.U(LY::M0 + f, li, lj, lk)) / (T)2;
-            const T k2 = (f == d) ? (T)1 : (T)0.5;
-            dU[LY::M0 + f] += k2 * (rR * nR * gR - rL * nL * gL) * idxd;
-            dU[LY::EN] += k2 * (mR * nR * gR - mL * nL * gL) * idxd;
+            const T mR = mc + vw.U(LY::M0 + f, ri, rj, rk);
+            const T mL = mc + vw.U(LY::M0 + f, li, lj, lk);
+            const T ck = C(LY::M0 + f) * ((f == d) ? (T)0.5 : (T)0.25);
+            dU[LY::M0 + f] += (rR * pR - rL * pL) * ck;
+            dU[LY::EN] += (mR * pR - mL * pL) * ck;
         }
         // magnetic field, diagonal part for f != d (hyperdifbsource1.cu:21-440)
 #pragma unroll
         for (int f = 0; f < D; f++) {
             if (f == d) continue;
-            nuRL(LY::pb(f, d), LY::B0 + f, nR, nL);
             const T bc_ = vw.U(LY::B0 + f, 0, 0, 0);
             const T br = vw.U(LY::B0 + f, ri, rj, rk), bl = vw.U(LY::B0 + f, li, lj, lk);
-            gR = z1 ? (T)0 : (br - bc_) * idxd;
-            gL = z1 ? (T)0 : (bc_ - bl) * idxd;
-            const T bR = (bc_ + br) / (T)2, bL = (bc_ + bl) / (T)2;
-            dU[LY::B0 + f] += (gR * nR - gL * nL) * idxd;
-            dU[LY::EN] += (bR * gR * nR - bL * gL * nL) * idxd;
+            pR = vw.Rp(LY::pb(f, d), 0, 0, 0) * (br - bc_);
+            pL = vw.Rp(LY::pb(f, d), li, lj, lk) * (bc_ - bl);
+            const T cb = C(LY::B0 + f);
+            dU[LY::B0 + f] += (pR - pL) * cb;
+            dU[LY::EN] += ((bc_ + br) * pR - (bc_ + bl) * pL) * (cb * (T)0.5);
         }
 
         // off-diagonal terms for f != d: d/dx_f of products evaluated at
@@ -708,14 +704,10 @@ __device__ __forceinline__ void hyper_terms(const Geo &g, const Phys &ph, const 
                 // ca = cn - e_d, cb = cn + e_d
                 const int ai = ni + li, aj = nj + lj, ak = nk + lk;
                 const int bi = ni + ri, bj = nj + rj, bk = nk + rk;
-                const T dv = z1 ? (T)0
-                                : (vw.A(AX::V0 + f, bi, bj, bk) - vw.A(AX::V0 + f, ai, aj, ak)) *
-                                      idxd / (T)2;
-                const T db = z1 ? (T)0
-                                : (vw.U(LY::B0 + f, bi, bj, bk) - vw.U(LY::B0 + f, ai, aj, ak)) *
-                                      idxd / (T)2;
+                const T dv = (vw.A(AX::V0 + f, bi, bj, bk) - vw.A(AX::V0 + f, ai, aj, ak)) * hxd;
+                const T db = (vw.U(LY::B0 + f, bi, bj, bk) - vw.U(LY::B0 + f, ai, aj, ak)) * hxd;
                 {
-                    const T K = (T)(dxd * cmax[d] * ph.chyp[LY::M0 + f]);
+                    const T K = (T)(dcd * ph.chyp[LY::M0 + f]);
                     const T s2 = K * (vw.Rp(LY::pv(f, d), ni, nj, nk) +
                                       vw.Rp(LY::pv(f, d), ai, aj, ak));
                     const T G = dv * s2 / (T)4;
@@ -724,7 +716,7 @@ __device__ __forceinline__ void hyper_terms(const Geo &g, const Phys &ph, const 
                     Xm[side] = G * vw.U(LY::M0 + d, ni, nj, nk);
                 }
                 {
-                    const T K = (T)(dxd * cmax[d] * ph.chyp[LY::B0 + f]);
+                    const T K = (T)(dcd * ph.chyp[LY::B0 + f]);
                     const T s2 = K * (vw.Rp(LY::pb(f, d), ni, nj, nk) +
                                       vw.Rp(LY::pb(f, d), ai, aj, ak));
                     const T H = db * s2 / (T)2;
