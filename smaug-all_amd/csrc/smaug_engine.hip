@@ -242,7 +242,9 @@ static int launch_update(smaug_engine *e, StageArgs sa, const void *uin, const v
                          e->panel[1][1], &nb);
     const double *bg;
     const Geo g = pc_geo(e, &bg);
-    hipLaunchKernelGGL((k_update<D, T>), dim3(nb), dim3(256), 0, e->stream, g, m, e->phys, sa,
+    // a background that is the same in every cell (bgm == 0) is read once
+    auto kern = g.bgm == 0u ? k_update<D, T, true> : k_update<D, T, false>;
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, e->stream, g, m, e->phys, sa,
                        (const T *)uin, (const T *)un, (T *)uout, (T *)e->ACC,
                        (const T *)e->R, (const T *)e->AUX, bg,
                        (const unsigned long long *)e->d_cmax);

@@ -238,8 +238,14 @@ __global__ __launch_bounds__(256) void ke_update(const RunDesc<T> *__restrict__ 
             sa.stage = stage;
             sa.ct = ct;
             sa.tm = rd.tm;
-            update_cell<D, T>(rd.geo, ph, sa, rd.U[src], rd.U[0], rd.U[dst], rd.ACC, rd.R, rd.AUX,
-                              rd.bg, rd.cm + 4 * stage, x0, x1, x2);
+            // the run's background layout is wave-uniform: read once when
+            // it is the same in every cell
+            if (rd.geo.bgm == 0u)
+                update_cell<D, T, true>(rd.geo, ph, sa, rd.U[src], rd.U[0], rd.U[dst], rd.ACC,
+                                        rd.R, rd.AUX, rd.bg, rd.cm + 4 * stage, x0, x1, x2);
+            else
+                update_cell<D, T, false>(rd.geo, ph, sa, rd.U[src], rd.U[0], rd.U[dst], rd.ACC,
+                                         rd.R, rd.AUX, rd.bg, rd.cm + 4 * stage, x0, x1, x2);
         }
     }
 }

@@ -572,7 +572,9 @@ struct StageArgs {
 
 // Global-memory view of the hyperdiffusion inputs around cell c: field f of
 // the state, aux quantity a, rho_b and ratio pair p at c + (di, dj, dk).
-template <int D, typename T>
+// UNI: a background that is the same in every cell (Geo::bgm == 0), whose
+// rho_b is read once into rb0
+template <int D, typename T, bool UNI = false>
 struct HyGlobal {
     const T *__restrict__ uin;
     const T *__restrict__ aux;
@@ -581,6 +583,7 @@ struct HyGlobal {
     size_t nc;
     size_t bgf; // background field stride and cell mask (Geo::bgf, bgm)
     ci_t bgm;
+    double rb0;
     ci_t c, s1, s2;
     __device__ __forceinline__ ci_t at(int di, int dj, int dk) const
     {
@@ -596,7 +599,10 @@ struct HyGlobal {
     }
     __device__ __forceinline__ double rhob(int di, int dj, int dk) const
     {
-        return ld(bg + bgf, at(di, dj, dk) & bgm);
+        if constexpr (UNI)
+            return rb0;
+        else
+            return ld(bg + bgf, at(di, dj, dk) & bgm);
     }
     __device__ __forceinline__ T Rp(int p, int di, int dj, int dk) const
     {
@@ -733,14 +739,15 @@ __device__ __forceinline__ void hyper_terms(const Geo &g, const Phys &ph, const 
     }
 }
 
-template <int D, typename T>
+template <int D, typename T, bool UNI = false>
 __device__ __forceinline__ void hyper_cell(const Geo &g, const Phys &ph, const T *__restrict__ uin,
                                            const T *__restrict__ R, const T *__restrict__ aux,
                                            const double *__restrict__ bg,
                                            const unsigned long long *__restrict__ cmb,
                                            const int x0, const int x1, const int x2, T *dU)
 {
-    HyGlobal<D, T> vw;
+    HyGlobal<D, T, UNI> vw;
+    vw.rb0 = UNI ? ld(bg + (size_t)g.bgf, 0u) : 0.0;
     vw.uin = uin;
     vw.aux = aux;
     vw.bg = bg;
@@ -857,7 +864,9 @@ __device__ __forceinline__ void rk_write(const StageArgs &sa, const T *__restric
 
 
 // the whole stage update of interior cell (x0, x1, x2)
-template <int D, typename T>
+// UNI: the background is the same in every cell (Geo::bgm == 0); its values
+// are read once instead of at every neighbour
+template <int D, typename T, bool UNI = false>
 __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const StageArgs &sa,
                                             const T *__restrict__ uin,
                                             const T *__restrict__ un,
@@ -877,7 +886,15 @@ __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const 
     // field and aux accessors
     auto U = [&](int f, ci_t cc) -> T { return ld(uin + f * nc, cc); };
     auto A = [&](int a, ci_t cc) -> T { return ld(aux + a * nc, cc); };
-    auto BG = [&](int f, ci_t cc) -> double { return ld(bg + f * (size_t)g.bgf, cc & g.bgm); };
+    double bgv[2 + D];
+#pragma unroll
+    for (int f = 0; f < 2 + D; f++) bgv[f] = UNI ? ld(bg + f * (size_t)g.bgf, 0u) : 0.0;
+    auto BG = [&](int f, ci_t cc) -> double {
+        if constexpr (UNI)
+            return bgv[f];
+        else
+            return ld(bg + f * (size_t)g.bgf, cc & g.bgm);
+    };
 
     T dU[LY::NVE];
 #pragma unroll
@@ -902,7 +919,7 @@ __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const 
         const int xa = xs[a], na = g.n[a];
         if (zero4(g.per[a], xa, na)) continue;
         // d/dx_a q = (q[-2] - 8 q[-1] + 8 q[+1] - q[+2]) / (12 dx)
-        const T i12 = (T)(1.0 / (12.0 * g.dx[a][xa]));
+        const T i12 = (T)(g.idx[a][xa] * (1.0 / 12.0)); // 1/(12 dx), without a division per cell
         T gv[D];
 #pragma unroll
         for (int d = 0; d < D; d++) gv[d] = 0;
@@ -964,13 +981,13 @@ __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const 
         dU[LY::EN] += rc * ge * A(AX::IRHO, c);
     }
 
-    if (ph.hyperdif) hyper_cell<D, T>(g, ph, uin, R, aux, bg, cmb, x0, x1, x2, dU);
+    if (ph.hyperdif) hyper_cell<D, T, UNI>(g, ph, uin, R, aux, bg, cmb, x0, x1, x2, dU);
 
     driver_cell<D, T>(g, ph, sa, rc, rbc, x0, x1, x2, dU);
     rk_write<D, T>(sa, un, uout, acc, nc, c, dU);
 }
 
-template <int D, typename T>
+template <int D, typename T, bool UNI = false>
 __global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, StageArgs sa,
                                                  const T *__restrict__ uin,
                                                  const T *__restrict__ un,
@@ -986,7 +1003,7 @@ __global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, Stag
     const int x1 = 2 + ty * 4 + (threadIdx.x >> 6);
     const int x2 = (D == 3) ? 2 + tk : 0;
     if (x0 > g.n[0] - 3 || x1 > g.n[1] - 3) return;
-    update_cell<D, T>(g, ph, sa, uin, un, uout, acc, R, aux, bg, cmb, x0, x1, x2);
+    update_cell<D, T, UNI>(g, ph, sa, uin, un, uout, acc, R, aux, bg, cmb, x0, x1, x2);
 }
 
 } // namespace smaug
