@@ -577,7 +577,15 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
     int npair = 2 * D + D * D + D * (D - 1);
     for (int k = 0; k < 3; k++) TRYALLOC(e->U[k], e->tsz * nve * e->nc);
     TRYALLOC(e->ACC, e->tsz * nve * e->nc);
-    if (cfg->hyperdif) TRYALLOC(e->R, e->tsz * npair * e->nc);
+    if (cfg->hyperdif) {
+        TRYALLOC(e->R, e->tsz * npair * e->nc);
+        // ratios outside the boxes the ratio kernels write stay 0: the 3-D
+        // source kernel multiplies some of them by a zero factor
+        if (hipMemset(e->R, 0, e->tsz * npair * e->nc) != hipSuccess) {
+            free_engine(e);
+            return fail(SMAUG_EHIP, "hipMemset of the ratio buffer failed");
+        }
+    }
     TRYALLOC(e->AUX, e->tsz * (3 + D) * e->nc);
     if (D == 3 && cfg->hyperdif) TRYALLOC(e->HYP, e->tsz * nve * e->nc);
     TRYALLOC(e->bg, sizeof(double) * nbg * e->nc);

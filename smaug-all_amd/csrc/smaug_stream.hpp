@@ -1153,22 +1153,23 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                 const int xq[2] = {bi_[q], bj_[q]};
 #pragma unroll
                 for (int f = 0; f < 2; f++) {
-                    T Xr = 0, Xm = 0, Hb = 0, He = 0;
-                    if (xq[f] >= 2 && xq[f] <= g.n[f] - 3) {
-                        // ca = p - e_k (plane P-2), cb = p + e_k (plane P)
-                        const T dv = (cur[q].v[f] - hv[q][1][f]) * hz;
-                        const T db = (cur[q].b[f] - hv[q][1][2 + f]) * hz;
-                        const T Km = sKz[f][O - k0];
-                        const T sv = Km * (hr[q][0][f] + hr[q][1][f]);
-                        const T G = dv * sv / (T)4;
-                        Xr = hn[q][0] * G;
-                        Xm = G * hn[q][1];
-                        const T Kb = sKz[2 + f][O - k0];
-                        const T sb = Kb * (hr[q][0][2 + f] + hr[q][1][2 + f]);
-                        const T H = db * sb / (T)2;
-                        Hb = H;
-                        He = H * hn[q][2];
-                    }
+                    // the products vanish off the interior along f (the
+                    // factor is 0 there; every input is finite: loaded
+                    // cells, and ratios that were written or are the zeros
+                    // the buffer was cleared to)
+                    const T hzf = (xq[f] >= 2 && xq[f] <= g.n[f] - 3) ? hz : (T)0;
+                    // ca = p - e_k (plane P-2), cb = p + e_k (plane P)
+                    const T dv = (cur[q].v[f] - hv[q][1][f]) * hzf;
+                    const T db = (cur[q].b[f] - hv[q][1][2 + f]) * hzf;
+                    const T Km = sKz[f][O - k0];
+                    const T sv = Km * (hr[q][0][f] + hr[q][1][f]);
+                    const T G = dv * sv / (T)4;
+                    const T Xr = hn[q][0] * G;
+                    const T Xm = G * hn[q][1];
+                    const T Kb = sKz[2 + f][O - k0];
+                    const T sb = Kb * (hr[q][0][2 + f] + hr[q][1][2 + f]);
+                    const T Hb = db * sb / (T)2;
+                    const T He = Hb * hn[q][2];
                     sX[4 * f + 0][bb[q]] = Xr;
                     sX[4 * f + 1][bb[q]] = Xm;
                     sX[4 * f + 2][bb[q]] = Hb;
@@ -1345,12 +1346,13 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                         for (int side = 0; side < 2; side++) {
                             const int sg = side ? 1 : -1;
                             const int ni = f == 0 ? sg : 0, nj = f == 1 ? sg : 0;
-                            Xr[side] = Xm[side] = Hb[side] = He[side] = 0;
+                            // nu vanishes off the interior: factor 0 there
+                            // (inputs finite, as for the k-products)
                             const int xfn = xf + sg;
-                            if (xfn < 2 || xfn > nf - 3) continue;
+                            const T hs = (xfn < 2 || xfn > nf - 3) ? (T)0 : hxd;
                             const int ai = ni - ri, aj = nj - rj, bi = ni + ri, bj = nj + rj;
-                            const T dv = (vw.A(AX::V0 + f, bi, bj) - vw.A(AX::V0 + f, ai, aj)) * hxd;
-                            const T db = (vw.U(LY::B0 + f, bi, bj) - vw.U(LY::B0 + f, ai, aj)) * hxd;
+                            const T dv = (vw.A(AX::V0 + f, bi, bj) - vw.A(AX::V0 + f, ai, aj)) * hs;
+                            const T db = (vw.U(LY::B0 + f, bi, bj) - vw.U(LY::B0 + f, ai, aj)) * hs;
                             {
                                 const T s2_ = K(0) * (vw.Rp(LY::pv(f, d), ni, nj) + vw.Rp(LY::pv(f, d), ai, aj));
                                 const T G = dv * s2_ / (T)4;
