@@ -990,6 +990,9 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
         gx[0] = zero1(g.per[0], i, n0) ? (T)0 : id[0];
         gx[1] = zero1(g.per[1], j, n1) ? (T)0 : id[1];
     }
+    // 1/(2 dx) of the outer derivative of the off-diagonal products, 0 where
+    // the reference leaves that derivative out (zero1 along its axis)
+    const T hf[2] = {gx[0] / (T)2, gx[1] / (T)2};
     // Per field slot q (rho, e, m_0..2, b_0..2) the factor of the diagonal
     // terms of a cell along an axis: (dx cmax chyp_q), rounded to T, times
     // 1/dx of the gradients (0 where zero1) and 1/dx of the divergence --
@@ -1340,7 +1343,7 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                     // off-diagonal pair (f, d) with f the other in-plane dim
                     const int f = 1 - d;
                     const int xf = xs[f], nf = g.n[f];
-                    if (!zero1(g.per[f], xf, nf)) {
+                    {
                         T Xr[2], Xm[2], Hb[2], He[2];
 #pragma unroll
                         for (int side = 0; side < 2; side++) {
@@ -1367,7 +1370,7 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                                 He[side] = H * vw.U(LY::B0 + d, ni, nj);
                             }
                         }
-                        const T h = id[f] / (T)2;
+                        const T h = hf[f];
                         rB[LY::M0 + d] += (Xr[1] - Xr[0]) * h;
                         rB[LY::EN] += (Xm[1] - Xm[0]) * h;
                         rB[LY::B0 + d] -= (Hb[1] - Hb[0]) * h;
@@ -1417,12 +1420,12 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
         // that follows the staging, and are next rewritten after the first
         // barrier of the next plane, so no barrier is needed here
         if (act && outO) {
-            const int xs[2] = {i, j};
+            // every box cell read here was written this plane: the
+            // neighbours of an active column lie on the grid
 #pragma unroll
             for (int f = 0; f < 2; f++) {
-                if (zero1(g.per[f], xs[f], g.n[f])) continue;
                 const int sf = f == 0 ? 1 : HB_BI;
-                const T h = id[f] / (T)2;
+                const T h = hf[f];
                 const T Xr = sX[4 * f + 0][own + sf] - sX[4 * f + 0][own - sf];
                 const T Xm = sX[4 * f + 1][own + sf] - sX[4 * f + 1][own - sf];
                 const T Hb = sX[4 * f + 2][own + sf] - sX[4 * f + 2][own - sf];
