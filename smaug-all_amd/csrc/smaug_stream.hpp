@@ -370,9 +370,10 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
                 const int xs[2] = {i, j};
 #pragma unroll
                 for (int a = 0; a < 2; a++) {
-                    if (zero4(g.per[a], xs[a], g.n[a])) continue;
+                    // the divergence along a is left out where zero4 holds:
+                    // a zero factor (the box cells read lie on the grid)
                     const int st_ = a == 0 ? 1 : ZS_BI;
-                    const T i12 = i12x[a];
+                    const T i12 = zero4(g.per[a], xs[a], g.n[a]) ? (T)0 : i12x[a];
                     T gv[3] = {0, 0, 0};
 #pragma unroll
                     for (int p = 0; p < 4; p++) {
