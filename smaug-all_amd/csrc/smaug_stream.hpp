@@ -889,14 +889,6 @@ __host__ __device__ constexpr int hz_islot(int p)
                : p - (p > 2) - (p > 5) - (p > 8) - (p > 11) - (p > 14) - (p > 16) - (p > 18);
 }
 
-// z ? 0 : v with v evaluated either way, so that the loads inside v are not
-// put behind a branch
-template <typename T>
-__device__ __forceinline__ T sel0(bool z, T v)
-{
-    return z ? (T)0 : v;
-}
-
 template <typename T>
 struct HzCell {
     T r, e, m[3], b[3], v[3];
