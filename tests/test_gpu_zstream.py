@@ -31,6 +31,9 @@ CASES = [
     ("70x9x13", strat(70, 9, 13), 0),      # two i-tiles, ragged j-tile
     ("20x12x16_kz5", strat(20, 12, 16), 5),  # z-chunks of 5 over 16 planes
     ("64x8x8_kz3", strat(64, 8, 8), 3),
+    # one plane per block: the most blocks and cfast partial maxima the
+    # folded aux pass of the update can leave
+    ("24x10x12_kz1", strat(24, 10, 12), 1),
 ]
 
 
