@@ -58,7 +58,7 @@ __device__ __forceinline__ void prep_st(const Geo &g, const Phys &ph, const St<D
     typedef Aux<D> AX;
     const size_t nc = (size_t)g.nc;
     const double rho = rho_full(s);
-    const double ir = 1.0 / rho;
+    const double ir = rcp64(rho);
     if (full) {
         st(aux + AX::IRHO * nc, c, (T)ir);
         st(aux + AX::PT * nc, c, (T)ptot_pert<D, T>(s, rho, ph.gamma));

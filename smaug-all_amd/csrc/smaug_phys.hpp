@@ -209,13 +209,23 @@ __device__ __forceinline__ double rho_full(const St<D, T> &s)
     return (double)s.r + s.rb;
 }
 
+// 1/x for a positive normal x (a density) in FP64 without the full IEEE
+// division: the hardware estimate and two Newton steps, within an ulp or so
+// of the correctly rounded quotient
+__device__ __forceinline__ double rcp64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
 // total-pressure perturbation, computept3 (dervfields.cu:284-330):
 // (g-1)(e' - m^2/2rho) - (g-2)(b'.b_b + b'^2/2), in double.  m^2/rho is
-// taken as a product with 1/rho, the reciprocal the callers form anyway (the
-// compiler shares it), and every fused multiply-add is written out with
+// taken as a product with 1/rho (rcp64, which the callers form anyway and
+// the compiler shares), and every fused multiply-add is written out with
 // contraction off, so that every kernel that inlines this gets the same bits.
 template <int D, typename T>
-__device__ __forceinline__ double ptot_pert(const St<D, T> &s, double rho, double gam)
+__device__ __forceinline__ double ptot_pert_ir(const St<D, T> &s, double ir, double gam)
 {
 #pragma clang fp contract(off)
     double mm = 0, bgb = 0, bb = 0;
@@ -227,7 +237,13 @@ __device__ __forceinline__ double ptot_pert(const St<D, T> &s, double rho, doubl
         bb = fma(b, b, bb);
     }
     const double ptmp = (gam - 2.0) * fma(0.5, bb, bgb);
-    return (gam - 1.0) * fma(-0.5 * mm, 1.0 / rho, (double)s.e) - ptmp;
+    return (gam - 1.0) * fma(-0.5 * mm, ir, (double)s.e) - ptmp;
+}
+
+template <int D, typename T>
+__device__ __forceinline__ double ptot_pert(const St<D, T> &s, double rho, double gam)
+{
+    return ptot_pert_ir<D, T>(s, rcp64(rho), gam);
 }
 
 // background total pressure, computepbg3 (dervfields.cu:332-368)
@@ -254,13 +270,14 @@ __device__ __forceinline__ double e_int(const St<D, T> &s, double rho)
         bb = fma(b, b, bb);
         mm = fma(m, m, mm);
     }
-    return fma(-0.5, fma(mm, 1.0 / rho, bb), (double)s.e);
+    return fma(-0.5, fma(mm, rcp64(rho), bb), (double)s.e);
 }
 
 // fast magnetosonic speed along d plus |v_d|, computec3 (dervfields.cu:418-486), in double.
 // The five divisions by rho of that formula are products with 1/rho here
-// (one division per cell instead of nine once the d-independent terms are
-// shared): the result differs from the divided form by rounding only.
+// (rcp64, one reciprocal per cell instead of nine divisions once the
+// d-independent terms are shared): the result differs from the divided form
+// by rounding only.
 // Contraction into FMAs is off, so that every kernel that inlines this gets
 // the same bits whatever else the products feed.
 template <int D, typename T>
@@ -268,7 +285,7 @@ __device__ __forceinline__ double cfast(const St<D, T> &s, int d, double gam)
 {
 #pragma clang fp contract(off)
     const double rho = rho_full(s);
-    const double ir = 1.0 / rho;
+    const double ir = rcp64(rho);
     double mm = 0, bb = 0, bgb = 0, bbb = 0;
 #pragma unroll
     for (int t = 0; t < D; t++) {

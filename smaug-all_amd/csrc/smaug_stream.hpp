@@ -98,12 +98,12 @@ __device__ __forceinline__ void zs_derive(ZCell<T> &z, const St<3, T> &s, double
                                           double *rb_out = nullptr, double *ptb_out = nullptr)
 {
     const double rho = rho_full(s);
-    const double ird = 1.0 / rho;
+    const double ird = rcp64(rho);
     z.r = s.r;
     z.e = s.e;
     z.eb = (T)s.eb;
     z.ir = (T)ird;
-    z.pt = (T)ptot_pert<3, T>(s, rho, gam);
+    z.pt = (T)ptot_pert_ir<3, T>(s, ird, gam);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         z.m[d] = s.m[d];
