@@ -636,7 +636,11 @@ __device__ __forceinline__ void hyper_terms(const Geo &g, const Phys &ph, const 
 #pragma unroll
     for (int d = 0; d < D; d++) cmax[d] = __longlong_as_double((long long)cmb[d]);
     const T eic = vw.A(AX::EINT, 0, 0, 0);
-    const double rhoc = (double)rc + rbc;
+    // full density rho' + rho_b of a cell, formed in double and rounded to T
+    auto F = [&](int di, int dj, int dk) -> T {
+        return (T)((double)vw.U(LY::RHO, di, dj, dk) + vw.rhob(di, dj, dk));
+    };
+    const T fc = (T)((double)rc + rbc);
 
 #pragma unroll
     for (int d = 0; d < D; d++) {
@@ -664,8 +668,8 @@ __device__ __forceinline__ void hyper_terms(const Geo &g, const Phys &ph, const 
         // momentum, diagonal part for every f; twice when f == d
         // (hyperdifmomsource1.cu:17-533 called for ii1 = 0 and, if f == d, ii1 = 1);
         // rR, rL, mR, mL, bR, bL are twice the face means, the 1/2 is in the factor
-        const T rR = (T)(rhoc + (double)vw.U(LY::RHO, ri, rj, rk) + vw.rhob(ri, rj, rk));
-        const T rL = (T)(rhoc + (double)vw.U(LY::RHO, li, lj, lk) + vw.rhob(li, lj, lk));
+        const T rR = fc + F(ri, rj, rk);
+        const T rL = fc + F(li, lj, lk);
 #pragma unroll
         for (int f = 0; f < D; f++) {
             const T vc = vw.A(AX::V0 + f, 0, 0, 0);
@@ -717,7 +721,7 @@ __device__ __forceinline__ void hyper_terms(const Geo &g, const Phys &ph, const 
                     const T s2 = K * (vw.Rp(LY::pv(f, d), ni, nj, nk) +
                                       vw.Rp(LY::pv(f, d), ai, aj, ak));
                     const T G = dv * s2 / (T)4;
-                    const T rhon = (T)((double)vw.U(LY::RHO, ni, nj, nk) + vw.rhob(ni, nj, nk));
+                    const T rhon = F(ni, nj, nk);
                     Xr[side] = rhon * G;
                     Xm[side] = G * vw.U(LY::M0 + d, ni, nj, nk);
                 }
@@ -846,9 +850,10 @@ __device__ __forceinline__ void rk_write(const StageArgs &sa, const T *__restric
     //  s1: out = un + dt/2 k, acc += dt/3 k
     //  s2: out = un + dt k,   acc += dt/3 k
     //  s3: out = acc + dt/6 k
+    // dt/6 and dt/3 as products: no FP64 division per cell
     const int s = sa.stage;
-    const T a = (T)(s == 2 ? dt : (s == 3 ? dt / 6 : dt / 2));
-    const T b = (T)(s == 0 ? dt / 6 : dt / 3);
+    const T a = (T)(s == 2 ? dt : (s == 3 ? dt * (1.0 / 6.0) : dt * 0.5));
+    const T b = (T)(s == 0 ? dt * (1.0 / 6.0) : dt * (1.0 / 3.0));
 #pragma unroll
     for (int f = 0; f < LY::NVE; f++) {
         const size_t o = f * nc;

@@ -216,8 +216,8 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
     // stage result = old + rk_a L(w) (old = the accumulator in RK stage 3),
     // accumulator = (u_n or accumulator) + rk_b L(w) in RK stages 0-2
     const int rs = sa.stage;
-    const T rk_a = sa.mode == 1 ? (T)dt : (T)(rs == 2 ? dt : (rs == 3 ? dt / 6 : dt / 2));
-    const T rk_b = (T)(rs == 0 ? dt / 6 : dt / 3);
+    const T rk_a = sa.mode == 1 ? (T)dt : (T)(rs == 2 ? dt : (rs == 3 ? dt * (1.0 / 6.0) : dt * 0.5));
+    const T rk_b = (T)(rs == 0 ? dt * (1.0 / 6.0) : dt * (1.0 / 3.0));
     const bool rk_old_acc = sa.mode == 0 && rs == 3;
     T i12x[2] = {0, 0};
     // driver 1: v = pxy P2(k); driver 2 (torsional): vx = pxy Q2(k),
