@@ -64,9 +64,10 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-ot", action="store_true")
     ap.add_argument("--shape", default=None,
-                    help="fixed 3-D update form and z-stream depth 'FORM,KZ' instead of the "
-                         "autotuner (FORM as in smaug_set_zstream: 0 per-cell, 1 z-streamed, "
-                         "3 z-streamed with LDS-plane sources), e.g. for profiling runs")
+                    help="fixed 3-D update form, z-stream depth and (optionally) ratio z-stream "
+                         "depth 'FORM,KZ[,KZR]' instead of the autotuner (FORM as in "
+                         "smaug_set_zstream: 0 per-cell, 1 z-streamed, 3 z-streamed with "
+                         "LDS-plane sources), e.g. for profiling runs")
     ap.add_argument("--ens-steps", type=int, default=200)
     ap.add_argument("--ot-steps", type=int, default=500,
                     help="RK4 steps of the Orszag-Tang leg (500 is the workload's count)")
@@ -84,14 +85,19 @@ def run_path(setup, precision, warmup, steps, dt_scale=1.0, tune=True, shape=Non
     e.set_state()
     info = {}
     if shape is not None:
-        form, kz = (int(x) for x in shape.split(","))
+        vals = [int(x) for x in shape.split(",")]
+        form, kz = vals[0], vals[1]
         e.set_zstream(form, kz)
-        info["tuned"] = {"update_form": FORMS[form], "kz": kz, "fixed": True}
+        if len(vals) > 2:
+            e.set_ratio_form(True, vals[2])
+        info["tuned"] = {"update_form": FORMS[form], "kz": kz,
+                         "ratio_kz": e.get_ratio_form()[1], "fixed": True}
     elif tune:
         best, ms = e.autotune(3)
         form, kz = e.get_zstream()
-        info["tuned"] = {"update_form": FORMS[form],
-                         "kz": kz, "panel_ratio": list(best[0]), "panel_update": list(best[1])}
+        info["tuned"] = {"update_form": FORMS[form], "kz": kz,
+                         "ratio_kz": e.get_ratio_form()[1],
+                         "panel_ratio": list(best[0]), "panel_update": list(best[1])}
         e.set_state()  # the tuning launches leave the state untouched; re-fill anyway
     e.rk4(0.0, dt, warmup)
     t0 = warmup * dt

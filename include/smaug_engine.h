@@ -106,6 +106,8 @@ int smaug_set_zstream(smaug_engine *e, int enable, int kz);
 /* 3-D grids: the hyperviscosity ratios z-streamed (1, the default; kz output
  * planes per block, 0 picks 32) or per cell (0) */
 int smaug_set_ratio_form(smaug_engine *e, int zstreamed, int kz);
+/* the form of the ratio pass in use (1 z-streamed) and its depth (0 per cell) */
+int smaug_get_ratio_form(smaug_engine *e, int *zstreamed, int *kz);
 /* the form of the 3-D update pass in use (enable as above) and its depth */
 int smaug_get_zstream(smaug_engine *e, int *enable, int *kz);
 /* 3-D grids: when the uploaded background is the same on every k-plane (a

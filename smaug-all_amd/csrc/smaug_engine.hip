@@ -1167,6 +1167,15 @@ extern "C" int smaug_set_ratio_form(smaug_engine *e, int zstreamed, int kz)
     return 0;
 }
 
+extern "C" int smaug_get_ratio_form(smaug_engine *e, int *zstreamed, int *kz)
+{
+    if (!e || !zstreamed || !kz) return fail(SMAUG_EINVAL, "null argument");
+    unsigned nb;
+    *zstreamed = e->D == 3 && e->ratio_zs ? 1 : 0;
+    *kz = *zstreamed ? make_rmap(e, &nb).kz : 0;
+    return 0;
+}
+
 extern "C" int smaug_set_bg_plane(smaug_engine *e, int enable)
 {
     if (!e) return fail(SMAUG_EINVAL, "null argument");

@@ -26,7 +26,7 @@ EXPORTS = [
     "smaug_time_rk4", "smaug_ncell", "smaug_ninterior", "smaug_copy_probe",
     "smaug_set_panel", "smaug_get_panel", "smaug_time_pass", "smaug_autotune",
     "smaug_set_zstream", "smaug_get_zstream", "smaug_time_passes", "smaug_set_ratio_form",
-    "smaug_set_bg_plane", "smaug_get_bg_plane",
+    "smaug_get_ratio_form", "smaug_set_bg_plane", "smaug_get_bg_plane",
     "smaug_sums", "smaug_ens_sums", "smaug_ens_time_passes",
     "smaug_ens_create", "smaug_ens_destroy", "smaug_ens_nrun", "smaug_ens_ninterior",
     "smaug_ens_set_state", "smaug_ens_get_state", "smaug_ens_rk4", "smaug_ens_time_rk4",
@@ -90,6 +90,7 @@ def load_library(path: str = LIB_PATH):
         lib.smaug_set_zstream.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         lib.smaug_get_zstream.argtypes = [vp, ip, ip]
         lib.smaug_set_ratio_form.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        lib.smaug_get_ratio_form.argtypes = [vp, ip, ip]
         lib.smaug_set_bg_plane.argtypes = [vp, ctypes.c_int]
         lib.smaug_get_bg_plane.argtypes = [vp, ip]
         lib.smaug_sums.argtypes = [vp, dp]
@@ -241,6 +242,12 @@ class Engine:
     def set_ratio_form(self, zstreamed, kz=0):
         """3-D: z-streamed hyperviscosity ratios (default) or per cell"""
         self._check(self.lib.smaug_set_ratio_form(self.h, int(zstreamed), kz))
+
+    def get_ratio_form(self):
+        """(1 if the 3-D ratio pass is z-streamed, else 0; its z-stream depth)"""
+        zs, kz = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.smaug_get_ratio_form(self.h, ctypes.byref(zs), ctypes.byref(kz)))
+        return zs.value, kz.value
 
     def get_zstream(self):
         """(form of the 3-D update pass: 0 per cell, 1 z-streamed, 2 z-streamed
