@@ -99,6 +99,9 @@ struct smaug_engine {
     unsigned long long *d_cmax;
     double *d_cpart; // per-block partial cfast maxima of the aux/cmax pass
     int ncpart;      // its capacity in blocks
+    void *AUXB;      // 2-D: the aux quantities of the next stage's input, formed by
+                     // the update of RK stages 0-2 while it reads AUX (swapped per stage)
+    void *aux_in;    // the aux quantities of the current stage's input (AUX or AUXB)
     double *d_tm; // [t, dt]
     Geo geo;
     Phys phys;
@@ -154,14 +157,14 @@ static int launch_cmax(smaug_engine *e, const void *u, int fill = 0)
         const unsigned ny = (unsigned)((e->n[2] + PREP_KC - 1) / PREP_KC);
         nb = nx * ny;
         hipLaunchKernelGGL((k_prep_col<T>), dim3(nx, ny), dim3(256), 0, e->stream, e->geo,
-                           e->phys, (T *)u, (const double *)e->bgp, (T *)e->AUX, e->d_cpart,
+                           e->phys, (T *)u, (const double *)e->bgp, (T *)e->aux_in, e->d_cpart,
                            PREP_KC, e->zstream ? 0 : 1, fill);
     } else {
         nb = (unsigned)prep_blocks(e->nc);
         const double *bg;
         const Geo g = pc_geo(e, &bg);
         hipLaunchKernelGGL((k_prep<D, T>), dim3(nb), dim3(256), 0, e->stream, g, e->phys,
-                           (const T *)u, bg, (T *)e->AUX, e->d_cpart);
+                           (const T *)u, bg, (T *)e->aux_in, e->d_cpart);
     }
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_cmax_reduce<D>, dim3(1), dim3(1024), 0, e->stream,
@@ -216,10 +219,10 @@ static int launch_ratio(smaug_engine *e, const void *u)
             ZMap m = make_rmap(e, &nb);
             if constexpr (sizeof(T) == 4)
                 hipLaunchKernelGGL(k_ratio_zs_f, dim3(nb), dim3(RZ_NT), 0, e->stream, e->geo, m,
-                                   (const T *)u, (const T *)e->AUX, (T *)e->R);
+                                   (const T *)u, (const T *)e->aux_in, (T *)e->R);
             else
                 hipLaunchKernelGGL(k_ratio_zs_d, dim3(nb), dim3(RZ_NT), 0, e->stream, e->geo, m,
-                                   (const T *)u, (const T *)e->AUX, (T *)e->R);
+                                   (const T *)u, (const T *)e->aux_in, (T *)e->R);
             HIPCHK(hipGetLastError());
             return 0;
         }
@@ -228,14 +231,18 @@ static int launch_ratio(smaug_engine *e, const void *u)
     TileMap m = make_map(e->n[0] - 3, e->n[1] - 3, D == 3 ? e->n[2] - 3 : 1, e->panel[0][0],
                          e->panel[0][1], &nb); // box [1, n-3]
     hipLaunchKernelGGL((k_ratio<D, T>), dim3(nb), dim3(256), 0, e->stream, e->geo, m, (const T *)u,
-                       (const T *)e->AUX, (T *)e->R);
+                       (const T *)e->aux_in, (T *)e->R);
     HIPCHK(hipGetLastError());
     return 0;
 }
 
+// aux_out (2-D, RK stages 0-2): the update also forms the aux quantities
+// and cfast partials of its output there, k_ghost_prep2 fills the output's
+// ghost cells and forms theirs, and the partials are reduced into cmax --
+// the next stage's aux pass without a pass over the interior
 template <int D, typename T>
 static int launch_update(smaug_engine *e, StageArgs sa, const void *uin, const void *un,
-                         void *uout)
+                         void *uout, void *aux_out = nullptr)
 {
     unsigned nb;
     TileMap m = make_map(e->n[0] - 4, e->n[1] - 4, D == 3 ? e->n[2] - 4 : 1, e->panel[1][0],
@@ -243,11 +250,35 @@ static int launch_update(smaug_engine *e, StageArgs sa, const void *uin, const v
     const double *bg;
     const Geo g = pc_geo(e, &bg);
     // a background that is the same in every cell (bgm == 0) is read once
-    auto kern = g.bgm == 0u ? k_update<D, T, true> : k_update<D, T, false>;
+    const bool uni = g.bgm == 0u;
+    if (aux_out) {
+        if constexpr (D == 2) {
+            const idx_t ng = ghost_count2(e->n[0], e->n[1]);
+            const unsigned gb = (unsigned)((ng + 255) / 256);
+            if ((idx_t)nb + gb > (idx_t)e->ncpart)
+                return fail(SMAUG_EINVAL, "cfast partials overflow");
+            auto kern = uni ? k_update<D, T, true, true> : k_update<D, T, false, true>;
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, e->stream, g, m, e->phys, sa,
+                               (const T *)uin, (const T *)un, (T *)uout, (T *)e->ACC,
+                               (const T *)e->R, (const T *)e->aux_in, bg,
+                               (const unsigned long long *)e->d_cmax, (T *)aux_out, e->d_cpart);
+            HIPCHK(hipGetLastError());
+            hipLaunchKernelGGL((k_ghost_prep2<T>), dim3(gb), dim3(256), 0, e->stream, g, e->phys,
+                               (T *)uout, bg, (T *)aux_out, e->d_cpart + (size_t)4 * nb);
+            HIPCHK(hipGetLastError());
+            hipLaunchKernelGGL(k_cmax_reduce<D>, dim3(1), dim3(1024), 0, e->stream,
+                               (const double *)e->d_cpart, (int)(nb + gb), e->d_cmax);
+            HIPCHK(hipGetLastError());
+            return 0;
+        } else {
+            return fail(SMAUG_EINVAL, "the folded per-cell aux pass is for 2-D grids");
+        }
+    }
+    auto kern = uni ? k_update<D, T, true> : k_update<D, T, false>;
     hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, e->stream, g, m, e->phys, sa,
                        (const T *)uin, (const T *)un, (T *)uout, (T *)e->ACC,
-                       (const T *)e->R, (const T *)e->AUX, bg,
-                       (const unsigned long long *)e->d_cmax);
+                       (const T *)e->R, (const T *)e->aux_in, bg,
+                       (const unsigned long long *)e->d_cmax, nullptr, nullptr);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -293,12 +324,12 @@ static int launch_hyper_sources(smaug_engine *e, const void *uin)
         if constexpr (sizeof(T) == 4)
             hipLaunchKernelGGL(bp ? k_hyper_zs_f<true> : k_hyper_zs_f<false>, dim3(nb), dim3(256),
                                0, e->stream, e->geo, m, e->phys, (const T *)uin, (const T *)e->R,
-                               (const T *)e->AUX, bg, (const unsigned long long *)e->d_cmax,
+                               (const T *)e->aux_in, bg, (const unsigned long long *)e->d_cmax,
                                (T *)e->HYP);
         else
             hipLaunchKernelGGL(bp ? k_hyper_zs_d<true> : k_hyper_zs_d<false>, dim3(nb), dim3(256),
                                0, e->stream, e->geo, m, e->phys, (const T *)uin, (const T *)e->R,
-                               (const T *)e->AUX, bg, (const unsigned long long *)e->d_cmax,
+                               (const T *)e->aux_in, bg, (const unsigned long long *)e->d_cmax,
                                (T *)e->HYP);
         HIPCHK(hipGetLastError());
         return 0;
@@ -310,19 +341,19 @@ static int launch_hyper_sources(smaug_engine *e, const void *uin)
         if (e->hyper_lds) {
             if constexpr (sizeof(T) == 4)
                 hipLaunchKernelGGL(k_hyper_lds_f, dim3(nh), dim3(256), 0, e->stream, e->geo, tm,
-                                   e->phys, (const T *)uin, (const T *)e->R, (const T *)e->AUX,
+                                   e->phys, (const T *)uin, (const T *)e->R, (const T *)e->aux_in,
                                    (const double *)e->bg, (const unsigned long long *)e->d_cmax,
                                    (T *)e->HYP);
             else
                 hipLaunchKernelGGL(k_hyper_lds_d, dim3(nh), dim3(256), 0, e->stream, e->geo, tm,
-                                   e->phys, (const T *)uin, (const T *)e->R, (const T *)e->AUX,
+                                   e->phys, (const T *)uin, (const T *)e->R, (const T *)e->aux_in,
                                    (const double *)e->bg, (const unsigned long long *)e->d_cmax,
                                    (T *)e->HYP);
         } else {
             const double *bg;
             const Geo g = pc_geo(e, &bg);
             hipLaunchKernelGGL((k_hyper<3, T>), dim3(nh), dim3(256), 0, e->stream, g, tm,
-                               e->phys, (const T *)uin, (const T *)e->R, (const T *)e->AUX, bg,
+                               e->phys, (const T *)uin, (const T *)e->R, (const T *)e->aux_in, bg,
                                (const unsigned long long *)e->d_cmax, (T *)e->HYP);
         }
         HIPCHK(hipGetLastError());
@@ -398,11 +429,11 @@ static int launch_pre_update(smaug_engine *e, const void *uin)
 
 template <int D, typename T>
 static int launch_stage_update(smaug_engine *e, StageArgs sa, const void *uin, const void *un,
-                               void *uout, bool fold = false)
+                               void *uout, bool fold = false, void *aux_out = nullptr)
 {
     if constexpr (D == 3)
         if (e->zstream) return launch_zstream<T>(e, sa, uin, un, uout, fold);
-    return launch_update<D, T>(e, sa, uin, un, uout);
+    return launch_update<D, T>(e, sa, uin, un, uout, aux_out);
 }
 
 template <int D, typename T>
@@ -426,15 +457,23 @@ __global__ void k_advance_time(double *tm)
 // unfilled and the aux/cmax pass of the next stage fills their ghosts; with
 // the z-streamed update as well, that pass is folded into the update of
 // RK stages 0-2 (launch_zstream) and the next stage starts at the ratios.
+// 2-D grids fold it into the per-cell update of RK stages 0-2 likewise
+// (launch_update), alternating two aux buffers.
 template <int D, typename T>
-static int stage(smaug_engine *e, StageArgs sa, const void *uin, const void *un, void *uout,
-                 hipEvent_t ev0, hipEvent_t ev1, const hipEvent_t *evk = nullptr)
+static int stage_launches(smaug_engine *e, StageArgs sa, const void *uin, const void *un,
+                          void *uout, hipEvent_t ev0, hipEvent_t ev1, const hipEvent_t *evk)
 {
     int rc;
     const bool defer = D == 3 && bg_planar(e) && sa.mode == 0;
     const bool fold = defer && e->zstream;
+    // 2-D: the updates of RK stages 0-2 form the next stage's aux quantities
+    // in the other aux buffer (the update reads the current one at the
+    // neighbours) and fill the output's ghosts; the stages alternate buffers
+    const bool fold2 = D == 2 && sa.mode == 0 && e->AUXB;
+    e->aux_in = fold2 && (sa.stage & 1) ? e->AUXB : e->AUX;
+    void *aux_out = fold2 && sa.stage < 3 ? ((sa.stage & 1) ? e->AUX : e->AUXB) : nullptr;
     if (evk) HIPCHK(hipEventRecord(evk[0], e->stream));
-    if (!(fold && sa.stage > 0))
+    if (!((fold || fold2) && sa.stage > 0))
         if ((rc = launch_cmax<D, T>(e, uin, defer && sa.stage > 0))) return rc;
     if (evk) HIPCHK(hipEventRecord(evk[1], e->stream));
     if (e->phys.hyperdif)
@@ -443,13 +482,23 @@ static int stage(smaug_engine *e, StageArgs sa, const void *uin, const void *un,
     if ((rc = launch_pre_update<D, T>(e, uin))) return rc;
     if (ev0) HIPCHK(hipEventRecord(ev0, e->stream));
     if (evk) HIPCHK(hipEventRecord(evk[3], e->stream));
-    if ((rc = launch_stage_update<D, T>(e, sa, uin, un, uout, fold && sa.stage < 3))) return rc;
+    if ((rc = launch_stage_update<D, T>(e, sa, uin, un, uout, fold && sa.stage < 3, aux_out)))
+        return rc;
     if (ev1) HIPCHK(hipEventRecord(ev1, e->stream));
     if (evk) HIPCHK(hipEventRecord(evk[4], e->stream));
-    if (sa.mode != 2 && !(defer && sa.stage < 3))
+    if (sa.mode != 2 && !((defer || fold2) && sa.stage < 3))
         if ((rc = launch_bc<D, T>(e, uout))) return rc;
     if (evk) HIPCHK(hipEventRecord(evk[5], e->stream));
     return 0;
+}
+
+template <int D, typename T>
+static int stage(smaug_engine *e, StageArgs sa, const void *uin, const void *un, void *uout,
+                 hipEvent_t ev0, hipEvent_t ev1, const hipEvent_t *evk = nullptr)
+{
+    const int rc = stage_launches<D, T>(e, sa, uin, un, uout, ev0, ev1, evk);
+    e->aux_in = e->AUX; // every other launch reads the aux quantities from AUX
+    return rc;
 }
 
 template <int D, typename T>
@@ -524,6 +573,7 @@ static void free_engine(smaug_engine *e)
     if (e->ACC) hipFree(e->ACC);
     if (e->R) hipFree(e->R);
     if (e->AUX) hipFree(e->AUX);
+    if (e->AUXB) hipFree(e->AUXB);
     if (e->HYP) hipFree(e->HYP);
     if (e->d_part) hipFree(e->d_part);
     if (e->bg) hipFree(e->bg);
@@ -587,6 +637,8 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
         }
     }
     TRYALLOC(e->AUX, e->tsz * (3 + D) * e->nc);
+    if (D == 2) TRYALLOC(e->AUXB, e->tsz * (3 + D) * e->nc);
+    e->aux_in = e->AUX;
     if (D == 3 && cfg->hyperdif) TRYALLOC(e->HYP, e->tsz * nve * e->nc);
     TRYALLOC(e->bg, sizeof(double) * nbg * e->nc);
     if (D == 3) TRYALLOC(e->bgp, sizeof(double) * nbg * (size_t)e->n[0] * e->n[1]);
@@ -604,6 +656,12 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
             const idx_t nzs = 8 * ((tiles + 7) / 8) * (e->n[2] - 4) +
                               (ghost_count(e->n[0], e->n[1], e->n[2]) + 255) / 256;
             if (nzs > nb) nb = nzs;
+        } else {
+            // the folded aux pass of the 2-D update: its blocks for any XCD
+            // panel (8 ceil(np / 8) tpx tpy <= 11 tiles) and the ghost cells
+            const idx_t tiles = ((idx_t)(e->n[0] - 4 + 63) / 64) * ((e->n[1] - 4 + 3) / 4);
+            const idx_t nup = 11 * tiles + (ghost_count2(e->n[0], e->n[1]) + 255) / 256;
+            if (nup > nb) nb = nup;
         }
         e->ncpart = (int)nb;
         TRYALLOC(e->d_cpart, sizeof(double) * 4 * nb);

@@ -434,6 +434,59 @@ __global__ __launch_bounds__(256) void k_ghost_prep(Geo g, Phys ph, T *__restric
     block_max_part<3>(mx, red, part + 4 * blockIdx.x);
 }
 
+// Ghost cells of a 2-D grid, each once: rows j = 0, 1, n1-2, n1-1 in full,
+// then the cells i = 0, 1, n0-2, n0-1 of the rows between
+__host__ __device__ __forceinline__ idx_t ghost_count2(int n0, int n1)
+{
+    return (idx_t)4 * n0 + (idx_t)4 * (n1 - 4);
+}
+
+// The ghost-cell share of the aux pass of a 2-D state whose interior aux and
+// cfast maxima the stage update that wrote it has formed (k_update with
+// FOLD): every ghost cell takes the state of its image, and its aux (all of
+// them) and cfast are formed with its own background, by the same load and
+// arithmetic as the aux pass.  The block's maxima go to part[4 b + d].
+template <typename T>
+__global__ __launch_bounds__(256) void k_ghost_prep2(Geo g, Phys ph, T *__restrict__ u,
+                                                     const double *__restrict__ bg,
+                                                     T *__restrict__ aux, double *part)
+{
+    typedef Lay<2> LY;
+    __shared__ double red[3][4];
+    double mx[3] = {0.0, 0.0, 0.0};
+    const int n0 = g.n[0], n1 = g.n[1];
+    const int t = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (t < (int)ghost_count2(n0, n1)) {
+        int i, j;
+        if (t < 4 * n0) {
+            const int q = t / n0;
+            j = q < 2 ? q : n1 - 4 + q;
+            i = t - q * n0;
+        } else {
+            const int r = t - 4 * n0;
+            j = 2 + (r >> 2);
+            const int q = r & 3;
+            i = q < 2 ? q : n0 - 4 + q;
+        }
+        const ci_t s1 = (ci_t)g.s1;
+        const size_t nc = (size_t)g.nc;
+        const ci_t c = (ci_t)i + s1 * (ci_t)j;
+        const ci_t img = (ci_t)bc_map(i, n0, g.per[0]) + s1 * (ci_t)bc_map(j, n1, g.per[1]);
+        St<2, T> s;
+        load_pert32<2, T>(s, u, nc, img);
+        load_bg32<2, T>(s, bg, (size_t)g.bgf, c & g.bgm);
+        st(u + LY::RHO * nc, c, s.r);
+        st(u + LY::EN * nc, c, s.e);
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            st(u + (LY::M0 + d) * nc, c, s.m[d]);
+            st(u + (LY::B0 + d) * nc, c, s.b[d]);
+        }
+        prep_st<2, T>(g, ph, s, aux, c, mx, 1);
+    }
+    block_max_part<2>(mx, red, part + 4 * blockIdx.x);
+}
+
 // number of ghost-slab elements of a grid (4 layers per slab, slabs may overlap)
 template <int D>
 __host__ __device__ __forceinline__ idx_t bc_count(int n0, int n1, int n2)
@@ -829,7 +882,7 @@ __device__ __forceinline__ void driver_cell(const Geo &g, const Phys &ph, const 
 template <int D, typename T>
 __device__ __forceinline__ void rk_write(const StageArgs &sa, const T *__restrict__ un,
                                          T *__restrict__ uout, T *__restrict__ acc, size_t nc,
-                                         ci_t c, const T *dU)
+                                         ci_t c, const T *dU, T *ov = nullptr)
 {
     typedef Lay<D> LY;
     const double dt = sa.tm[1];
@@ -858,10 +911,14 @@ __device__ __forceinline__ void rk_write(const StageArgs &sa, const T *__restric
     for (int f = 0; f < LY::NVE; f++) {
         const size_t o = f * nc;
         if (s == 3) {
-            st(uout + o, c, ld(acc + o, c) + a * dU[f]);
+            const T v = ld(acc + o, c) + a * dU[f];
+            st(uout + o, c, v);
+            if (ov) ov[f] = v;
         } else {
             const T base = ld(un + o, c);
-            st(uout + o, c, base + a * dU[f]);
+            const T v = base + a * dU[f];
+            st(uout + o, c, v);
+            if (ov) ov[f] = v;
             st(acc + o, c, (s == 0 ? base : ld(acc + o, c)) + b * dU[f]);
         }
     }
@@ -870,8 +927,11 @@ __device__ __forceinline__ void rk_write(const StageArgs &sa, const T *__restric
 
 // the whole stage update of interior cell (x0, x1, x2)
 // UNI: the background is the same in every cell (Geo::bgm == 0); its values
-// are read once instead of at every neighbour
-template <int D, typename T, bool UNI = false>
+// are read once instead of at every neighbour.  FOLD (RK stages 0-2): the
+// aux quantities of the stage result (all of them, as the aux pass forms
+// them) go to aux_out and its cfast into the running maxima mx, so that the
+// next stage needs no aux/cmax pass over the interior.
+template <int D, typename T, bool UNI = false, bool FOLD = false>
 __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const StageArgs &sa,
                                             const T *__restrict__ uin,
                                             const T *__restrict__ un,
@@ -880,7 +940,9 @@ __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const 
                                             const T *__restrict__ aux,
                                             const double *__restrict__ bg,
                                             const unsigned long long *__restrict__ cmb,
-                                            const int x0, const int x1, const int x2)
+                                            const int x0, const int x1, const int x2,
+                                            T *__restrict__ aux_out = nullptr,
+                                            double *mx = nullptr)
 {
     typedef Lay<D> LY;
     typedef Aux<D> AX;
@@ -989,10 +1051,28 @@ __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const 
     if (ph.hyperdif) hyper_cell<D, T, UNI>(g, ph, uin, R, aux, bg, cmb, x0, x1, x2, dU);
 
     driver_cell<D, T>(g, ph, sa, rc, rbc, x0, x1, x2, dU);
-    rk_write<D, T>(sa, un, uout, acc, nc, c, dU);
+    if constexpr (FOLD) {
+        T ov[LY::NVE];
+        rk_write<D, T>(sa, un, uout, acc, nc, c, dU, ov);
+        St<D, T> so;
+        so.r = ov[LY::RHO];
+        so.e = ov[LY::EN];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            so.m[d] = ov[LY::M0 + d];
+            so.b[d] = ov[LY::B0 + d];
+            so.bb[d] = BG(2 + d, c);
+        }
+        so.rb = rbc;
+        so.eb = BG(0, c);
+        prep_st<D, T>(g, ph, so, aux_out, c, mx, 1);
+    } else {
+        rk_write<D, T>(sa, un, uout, acc, nc, c, dU);
+    }
 }
 
-template <int D, typename T, bool UNI = false>
+// FOLD: as update_cell; the block's cfast maxima go to part[4 b + d]
+template <int D, typename T, bool UNI = false, bool FOLD = false>
 __global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, StageArgs sa,
                                                  const T *__restrict__ uin,
                                                  const T *__restrict__ un,
@@ -1000,15 +1080,29 @@ __global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, Stag
                                                  const T *__restrict__ R,
                                                  const T *__restrict__ aux,
                                                  const double *__restrict__ bg,
-                                                 const unsigned long long *__restrict__ cmb)
+                                                 const unsigned long long *__restrict__ cmb,
+                                                 T *__restrict__ aux_out = nullptr,
+                                                 double *part = nullptr)
 {
     int tx, ty, tk;
-    if (!map_block(tm, tx, ty, tk)) return;
+    if (!map_block(tm, tx, ty, tk)) {
+        if (FOLD && threadIdx.x < D) part[4 * blockIdx.x + threadIdx.x] = 0.0;
+        return;
+    }
     const int x0 = 2 + tx * 64 + (threadIdx.x & 63);
     const int x1 = 2 + ty * 4 + (threadIdx.x >> 6);
     const int x2 = (D == 3) ? 2 + tk : 0;
-    if (x0 > g.n[0] - 3 || x1 > g.n[1] - 3) return;
-    update_cell<D, T, UNI>(g, ph, sa, uin, un, uout, acc, R, aux, bg, cmb, x0, x1, x2);
+    if constexpr (FOLD) {
+        __shared__ double red[3][4];
+        double mx[3] = {0.0, 0.0, 0.0};
+        if (x0 <= g.n[0] - 3 && x1 <= g.n[1] - 3)
+            update_cell<D, T, UNI, true>(g, ph, sa, uin, un, uout, acc, R, aux, bg, cmb, x0, x1,
+                                         x2, aux_out, mx);
+        block_max_part<D>(mx, red, part + 4 * blockIdx.x);
+    } else {
+        if (x0 > g.n[0] - 3 || x1 > g.n[1] - 3) return;
+        update_cell<D, T, UNI>(g, ph, sa, uin, un, uout, acc, R, aux, bg, cmb, x0, x1, x2);
+    }
 }
 
 } // namespace smaug
