@@ -114,28 +114,28 @@ def pass_kernels(form, bg_plane, fp32=True):
     """the kernels of each pass of a 3-D RK4 step for the update form and
     background layout in use: (name as rocprofv3 prints it, namespace and
     arguments dropped; launches per step).  With the k-plane background and
-    the z-streamed update, the updates of stages 0-2 also form the next
-    stage's aux quantities and cfast maxima (k_ghost_prep does the ghost
-    cells), so the aux pass runs once per step, for stage 0."""
+    the z-streamed update, the update of every stage also forms the next
+    stage's aux quantities and cfast maxima (k_ghost_prep fills the ghost
+    cells and does theirs), stage 3 those of the next step's stage 0: the
+    step has no aux pass and no separate ghost fill."""
     t = "float" if fp32 else "double"
     s = "f" if fp32 else "d"
     bp = "true" if bg_plane else "false"
     fold = bool(form) and bg_plane
     src = {1: f"k_hyper_zs_{s}<{bp}>", 2: f"k_hyper<3, {t}>", 3: f"k_hyper_lds_{s}"}
     if fold:
-        upd = [(f"k_zstream_{s}<true, true>", 3), (f"k_ghost_prep<{t}>", 3),
-               (f"k_zstream_{s}<true, false>", 1)]
+        upd = [(f"k_zstream_{s}<true, true>", 4), (f"k_ghost_prep<{t}>", 4)]
     elif form:
         upd = [(f"k_zstream_{s}<{bp}, false>", 4)]
     else:
         upd = [(f"k_update<3, {t}>", 4)]
     return {
-        "aux+cmax": [(f"k_prep_col<{t}>", 1 if fold else 4)] if bg_plane
+        "aux+cmax": [] if fold else [(f"k_prep_col<{t}>", 4)] if bg_plane
         else [(f"k_prep<3, {t}>", 4)],
         "ratio": [(f"k_ratio_zs_{s}", 4)],
         "hyperdiffusive sources": [(src.get(form, ""), 4)],
         "stage update": upd,
-        "ghost fill": [(f"k_bc<3, {t}>", 1 if bg_plane else 4)],
+        "ghost fill": [] if fold else [(f"k_bc<3, {t}>", 1 if bg_plane else 4)],
     }
 
 

@@ -102,6 +102,8 @@ struct smaug_engine {
     void *AUXB;      // 2-D: the aux quantities of the next stage's input, formed by
                      // the update of RK stages 0-2 while it reads AUX (swapped per stage)
     void *aux_in;    // the aux quantities of the current stage's input (AUX or AUXB)
+    int aux0_ok;     // AUX and cmax hold the aux pass of U0 (left by the last folded
+                     // RK4 step); any other change of the state clears it
     double *d_tm; // [t, dt]
     Geo geo;
     Phys phys;
@@ -456,7 +458,8 @@ __global__ void k_advance_time(double *tm)
 // With the k-plane background, the outputs of RK stages 0-2 are left
 // unfilled and the aux/cmax pass of the next stage fills their ghosts; with
 // the z-streamed update as well, that pass is folded into the update of
-// RK stages 0-2 (launch_zstream) and the next stage starts at the ratios.
+// every stage (launch_zstream), and every stage starts at the ratios: the
+// step's stage 3 does the aux pass of the next step's stage 0.
 // 2-D grids fold it into the per-cell update of RK stages 0-2 likewise
 // (launch_update), alternating two aux buffers.
 template <int D, typename T>
@@ -473,7 +476,9 @@ static int stage_launches(smaug_engine *e, StageArgs sa, const void *uin, const 
     e->aux_in = fold2 && (sa.stage & 1) ? e->AUXB : e->AUX;
     void *aux_out = fold2 && sa.stage < 3 ? ((sa.stage & 1) ? e->AUX : e->AUXB) : nullptr;
     if (evk) HIPCHK(hipEventRecord(evk[0], e->stream));
-    if (!((fold || fold2) && sa.stage > 0))
+    // 3-D fold: the aux pass of every stage is done by the update before it,
+    // that of stage 0 by the previous step's stage 3 (or ensure_aux0)
+    if (!fold && !(fold2 && sa.stage > 0))
         if ((rc = launch_cmax<D, T>(e, uin, defer && sa.stage > 0))) return rc;
     if (evk) HIPCHK(hipEventRecord(evk[1], e->stream));
     if (e->phys.hyperdif)
@@ -482,11 +487,10 @@ static int stage_launches(smaug_engine *e, StageArgs sa, const void *uin, const 
     if ((rc = launch_pre_update<D, T>(e, uin))) return rc;
     if (ev0) HIPCHK(hipEventRecord(ev0, e->stream));
     if (evk) HIPCHK(hipEventRecord(evk[3], e->stream));
-    if ((rc = launch_stage_update<D, T>(e, sa, uin, un, uout, fold && sa.stage < 3, aux_out)))
-        return rc;
+    if ((rc = launch_stage_update<D, T>(e, sa, uin, un, uout, fold, aux_out))) return rc;
     if (ev1) HIPCHK(hipEventRecord(ev1, e->stream));
     if (evk) HIPCHK(hipEventRecord(evk[4], e->stream));
-    if (sa.mode != 2 && !((defer || fold2) && sa.stage < 3))
+    if (sa.mode != 2 && !fold && !((defer || fold2) && sa.stage < 3))
         if ((rc = launch_bc<D, T>(e, uout))) return rc;
     if (evk) HIPCHK(hipEventRecord(evk[5], e->stream));
     return 0;
@@ -499,6 +503,25 @@ static int stage(smaug_engine *e, StageArgs sa, const void *uin, const void *un,
     const int rc = stage_launches<D, T>(e, sa, uin, un, uout, ev0, ev1, evk);
     e->aux_in = e->AUX; // every other launch reads the aux quantities from AUX
     return rc;
+}
+
+// RK4 steps whose stages all fold the aux pass into the update before them
+template <int D, typename T>
+static bool folds_all(const smaug_engine *e)
+{
+    return D == 3 && e->zstream && bg_planar(e);
+}
+
+// before RK4 steps that fold: the aux pass of U0 unless the last folded
+// step left it (eager, ahead of the captured steps)
+template <int D, typename T>
+static int ensure_aux0(smaug_engine *e)
+{
+    if (!folds_all<D, T>(e) || e->aux0_ok) return 0;
+    int rc = launch_cmax<D, T>(e, e->U[0]);
+    if (rc) return rc;
+    e->aux0_ok = 1;
+    return 0;
 }
 
 template <int D, typename T>
@@ -768,6 +791,7 @@ static int bg_uniform(int nbg, idx_t nc, const double *bg)
 extern "C" int smaug_set_state(smaug_engine *e, const double *w, const double *bg)
 {
     if (!e || !w || !bg) return fail(SMAUG_EINVAL, "null argument");
+    e->aux0_ok = 0;
     int nve = 2 + 2 * e->D, nbg = 2 + e->D;
     {
         const int ok = bg_uniform(nbg, e->nc, bg);
@@ -854,6 +878,7 @@ static int do_rhs(smaug_engine *e, double t, double *rhs)
 extern "C" int smaug_rhs(smaug_engine *e, double t, double *rhs)
 {
     if (!e || !rhs) return fail(SMAUG_EINVAL, "null argument");
+    e->aux0_ok = 0;
     return DISPATCH(e, do_rhs, e, t, rhs);
 }
 
@@ -876,6 +901,7 @@ static int do_cmax(smaug_engine *e, double *cm)
 extern "C" int smaug_cmax(smaug_engine *e, double cmax[3])
 {
     if (!e || !cmax) return fail(SMAUG_EINVAL, "null argument");
+    e->aux0_ok = 0;
     return DISPATCH(e, do_cmax, e, cmax);
 }
 
@@ -904,6 +930,7 @@ static int do_courant(smaug_engine *e, double *out)
 extern "C" int smaug_maxcourant(smaug_engine *e, double *out)
 {
     if (!e || !out) return fail(SMAUG_EINVAL, "null argument");
+    e->aux0_ok = 0;
     return DISPATCH(e, do_courant, e, out);
 }
 
@@ -929,6 +956,7 @@ static int run_steps(smaug_engine *e, int which, double t0, double dt, int nstep
 {
     if (which == 1) {
         // the Euler step swaps buffers on the host, so it runs eagerly
+        e->aux0_ok = 0;
         int rc = set_time(e, t0, dt);
         if (rc) return rc;
         for (int s = 0; s < nsteps; s++)
@@ -939,6 +967,7 @@ static int run_steps(smaug_engine *e, int which, double t0, double dt, int nstep
     int rc = capture<D, T>(e, which);
     if (rc) return rc;
     if ((rc = set_time(e, t0, dt))) return rc;
+    if ((rc = ensure_aux0<D, T>(e))) return rc;
     for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(e->gexec[which], e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return 0;
@@ -963,6 +992,7 @@ static int time_rk4(smaug_engine *e, double t0, double dt, int nsteps, double *t
     int rc = capture<D, T>(e, 0);
     if (rc) return rc;
     if ((rc = set_time(e, t0, dt))) return rc;
+    if ((rc = ensure_aux0<D, T>(e))) return rc;
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
@@ -999,6 +1029,7 @@ static int time_passes(smaug_engine *e, double t0, double dt, double *ms)
 {
     int rc = set_time(e, t0, dt);
     if (rc) return rc;
+    if ((rc = ensure_aux0<D, T>(e))) return rc;
     std::vector<hipEvent_t> evk(24);
     for (auto &v : evk) HIPCHK(hipEventCreate(&v));
     if ((rc = rk4_step<D, T>(e, nullptr, &evk))) return rc;
@@ -1034,6 +1065,7 @@ extern "C" int smaug_time_rk4(smaug_engine *e, double t0, double dt, int nsteps,
 // ---------------------------------------------------------------------------
 static void drop_graphs(smaug_engine *e)
 {
+    e->aux0_ok = 0;
     for (int k = 0; k < 2; k++)
         if (e->gvalid[k]) {
             hipGraphExecDestroy(e->gexec[k]);
@@ -1118,6 +1150,7 @@ extern "C" int smaug_time_pass(smaug_engine *e, int pass, int reps, double *ms)
 {
     if (!e || pass < 0 || pass > 1 || reps < 1 || !ms) return fail(SMAUG_EINVAL, "bad argument");
     if (pass == 0 && !e->phys.hyperdif) return fail(SMAUG_EINVAL, "no ratio pass without hyperdiffusion");
+    e->aux0_ok = 0;
     return DISPATCH(e, time_pass, e, pass, reps, ms);
 }
 
@@ -1129,6 +1162,7 @@ extern "C" int smaug_time_pass(smaug_engine *e, int pass, int reps, double *ms)
 extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double best_ms[2])
 {
     if (!e || reps < 1) return fail(SMAUG_EINVAL, "bad argument");
+    e->aux0_ok = 0;
     static const int TX[] = {1, 2, 4, 8}, TY[] = {1, 2, 4, 8, 16, 32};
     const int zs0 = e->zstream, rz0 = e->ratio_zs;
     e->zstream = 0;

@@ -153,7 +153,8 @@ def test_bg_plane_off_for_k_varying_background():
 @pytest.mark.parametrize("bc", [(4, 0, 0), (4, 4, 0), (4, 0, 4)], ids=["cpp", "ccp", "cpc"])
 def test_folded_ghost_fill_periodic_vs_oracle(bc):
     # the k-plane background path fills the ghosts of RK stage outputs in the
-    # next aux pass; with periodic horizontal faces the images wrap around
+    # ghost-cell share of the folded aux pass (k_ghost_prep); with periodic
+    # horizontal faces the images wrap around
     # (the vertical axis stays contcd4: a periodic vertical axis would join
     # the dense bottom of the atmosphere to its top)
     s = dataclasses.replace(strat(20, 12, 16), bc=bc)
@@ -171,3 +172,30 @@ def test_folded_ghost_fill_periodic_vs_oracle(bc):
     # every cell, ghosts included
     err = rel_err(got, ref)
     assert np.all(err < TOL_ORACLE_FP64), err
+
+
+@pytest.mark.parametrize("prec", [FP64, MIXED], ids=["fp64", "mixed"])
+def test_steps_split_across_calls_match(prec):
+    # with the k-plane background every stage's aux pass is done by the
+    # update before it, and a step hands the aux quantities of its result to
+    # the next: steps split over several calls, with calls between them that
+    # recompute or replace the aux quantities, give the same bits as one call
+    name, s, kz = CASES[1]
+    print(f"[case] split rk4 calls {name} prec={prec}", flush=True)
+    w0 = Oracle(s).boundary(perturbed(s, amp=1e-3, seed=16))
+    dt = 0.5 * configs.hyperviscous_dt(s)
+    e = Engine(s, prec)
+    e.set_state(w0)
+    assert e.get_bg_plane()
+    e.rk4(0.0, dt, 5)
+    ref = e.get_state()
+    e.set_state(w0)
+    e.rk4(0.0, dt, 2)
+    e.cmax()
+    e.rk4(2 * dt, dt, 1)
+    e.rhs(t=3 * dt)
+    e.rk4(3 * dt, dt, 1)
+    e.rk4(4 * dt, dt, 1)
+    got = e.get_state()
+    e.close()
+    assert np.array_equal(got, ref), rel_err(got, ref)
