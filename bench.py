@@ -115,16 +115,16 @@ def pass_kernels(form, bg_plane, fp32=True):
     background layout in use: (name as rocprofv3 prints it, namespace and
     arguments dropped; launches per step).  With the k-plane background and
     the z-streamed update, the update of every stage also forms the next
-    stage's aux quantities and cfast maxima (k_ghost_prep fills the ghost
-    cells and does theirs), stage 3 those of the next step's stage 0: the
-    step has no aux pass and no separate ghost fill."""
+    stage's aux quantities and cfast maxima (k_ghost_prep, the ghost fill,
+    does the ghost cells'), stage 3 those of the next step's stage 0: the
+    step has no aux pass."""
     t = "float" if fp32 else "double"
     s = "f" if fp32 else "d"
     bp = "true" if bg_plane else "false"
     fold = bool(form) and bg_plane
     src = {1: f"k_hyper_zs_{s}<{bp}>", 2: f"k_hyper<3, {t}>", 3: f"k_hyper_lds_{s}"}
     if fold:
-        upd = [(f"k_zstream_{s}<true, true>", 4), (f"k_ghost_prep<{t}>", 4)]
+        upd = [(f"k_zstream_{s}<true, true>", 4)]
     elif form:
         upd = [(f"k_zstream_{s}<{bp}, false>", 4)]
     else:
@@ -135,7 +135,8 @@ def pass_kernels(form, bg_plane, fp32=True):
         "ratio": [(f"k_ratio_zs_{s}", 4)],
         "hyperdiffusive sources": [(src.get(form, ""), 4)],
         "stage update": upd,
-        "ghost fill": [] if fold else [(f"k_bc<3, {t}>", 1 if bg_plane else 4)],
+        "ghost fill": [(f"k_ghost_prep<{t}>", 4)] if fold
+        else [(f"k_bc<3, {t}>", 1 if bg_plane else 4)],
     }
 
 
@@ -356,12 +357,10 @@ def main():
 
     # the roofline is given for the dominant kernel of the step: of the
     # passes that run one kernel in every stage, the one with the longest
-    # mean launch (HIP events on the engine stream).  The stage-update pass
-    # is not one kernel: two forms of the update (those of stages 0-2 also
-    # form the next stage's aux quantities) and the ghost-cell aux kernel,
-    # each of which takes less time per step than the source kernel.  Every
-    # pass of a stage processes the stage of every cell, so its algorithmic
-    # bytes per launch are the step figure / 4 stages x cells.
+    # mean launch (HIP events on the engine stream; for the stage update the
+    # events around each update launch of one more step).  Every pass of a
+    # stage processes the stage of every cell, so its algorithmic bytes per
+    # launch are the step figure / 4 stages x cells.
     bytes_launch = BYTES_PER_CELL_STEP["mixed"] / STAGES_PER_STEP * cells
     pms = info["passes_ms"]
     pk = pass_kernels(info["form"], info["bg_plane"])

@@ -239,9 +239,9 @@ static int launch_ratio(smaug_engine *e, const void *u)
 }
 
 // aux_out (2-D, RK stages 0-2): the update also forms the aux quantities
-// and cfast partials of its output there, k_ghost_prep2 fills the output's
-// ghost cells and forms theirs, and the partials are reduced into cmax --
-// the next stage's aux pass without a pass over the interior
+// and cfast partials of its output there (launch_ghost_fold then does the
+// ghost cells and the reduction) -- the next stage's aux pass without a
+// pass over the interior
 template <int D, typename T>
 static int launch_update(smaug_engine *e, StageArgs sa, const void *uin, const void *un,
                          void *uout, void *aux_out = nullptr)
@@ -264,12 +264,6 @@ static int launch_update(smaug_engine *e, StageArgs sa, const void *uin, const v
                                (const T *)uin, (const T *)un, (T *)uout, (T *)e->ACC,
                                (const T *)e->R, (const T *)e->aux_in, bg,
                                (const unsigned long long *)e->d_cmax, (T *)aux_out, e->d_cpart);
-            HIPCHK(hipGetLastError());
-            hipLaunchKernelGGL((k_ghost_prep2<T>), dim3(gb), dim3(256), 0, e->stream, g, e->phys,
-                               (T *)uout, bg, (T *)aux_out, e->d_cpart + (size_t)4 * nb);
-            HIPCHK(hipGetLastError());
-            hipLaunchKernelGGL(k_cmax_reduce<D>, dim3(1), dim3(1024), 0, e->stream,
-                               (const double *)e->d_cpart, (int)(nb + gb), e->d_cmax);
             HIPCHK(hipGetLastError());
             return 0;
         } else {
@@ -401,16 +395,41 @@ static int launch_zstream(smaug_engine *e, StageArgs sa, const void *uin, const 
     HIPCHK(hipGetLastError());
     if (fold) {
         const idx_t ng = ghost_count(e->n[0], e->n[1], e->n[2]);
-        const unsigned gb = (unsigned)((ng + 255) / 256);
+        if ((idx_t)nb + (ng + 255) / 256 > (idx_t)e->ncpart)
+            return fail(SMAUG_EINVAL, "cfast partials overflow");
+    }
+    return 0;
+}
+
+// The ghost fill of an update output whose interior aux pass the update
+// folded in: the ghost cells take their images' state and get their aux
+// quantities and cfast partials (after the update's partials), and the
+// partials are reduced into cmax.  3-D: k_ghost_prep (k-plane background,
+// aux in AUX); 2-D: k_ghost_prep2 (aux in aux_out).
+template <int D, typename T>
+static int launch_ghost_fold(smaug_engine *e, void *uout, void *aux_out)
+{
+    unsigned nb, gb;
+    if constexpr (D == 3) {
+        make_zmap(e, &nb);
+        gb = (unsigned)((ghost_count(e->n[0], e->n[1], e->n[2]) + 255) / 256);
         if ((idx_t)nb + gb > (idx_t)e->ncpart) return fail(SMAUG_EINVAL, "cfast partials overflow");
         hipLaunchKernelGGL((k_ghost_prep<T>), dim3(gb), dim3(256), 0, e->stream, e->geo, e->phys,
                            (T *)uout, (const double *)e->bgp, (T *)e->AUX,
                            e->d_cpart + (size_t)4 * nb, 0);
-        HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_cmax_reduce<3>, dim3(1), dim3(1024), 0, e->stream,
-                           (const double *)e->d_cpart, (int)(nb + gb), e->d_cmax);
-        HIPCHK(hipGetLastError());
+    } else {
+        make_map(e->n[0] - 4, e->n[1] - 4, 1, e->panel[1][0], e->panel[1][1], &nb);
+        gb = (unsigned)((ghost_count2(e->n[0], e->n[1]) + 255) / 256);
+        if ((idx_t)nb + gb > (idx_t)e->ncpart) return fail(SMAUG_EINVAL, "cfast partials overflow");
+        const double *bg;
+        const Geo g = pc_geo(e, &bg);
+        hipLaunchKernelGGL((k_ghost_prep2<T>), dim3(gb), dim3(256), 0, e->stream, g, e->phys,
+                           (T *)uout, bg, (T *)aux_out, e->d_cpart + (size_t)4 * nb);
     }
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_cmax_reduce<D>, dim3(1), dim3(1024), 0, e->stream,
+                       (const double *)e->d_cpart, (int)(nb + gb), e->d_cmax);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 
@@ -490,8 +509,11 @@ static int stage_launches(smaug_engine *e, StageArgs sa, const void *uin, const 
     if ((rc = launch_stage_update<D, T>(e, sa, uin, un, uout, fold, aux_out))) return rc;
     if (ev1) HIPCHK(hipEventRecord(ev1, e->stream));
     if (evk) HIPCHK(hipEventRecord(evk[4], e->stream));
-    if (sa.mode != 2 && !fold && !((defer || fold2) && sa.stage < 3))
+    if (fold || aux_out) {
+        if ((rc = launch_ghost_fold<D, T>(e, uout, aux_out))) return rc;
+    } else if (sa.mode != 2 && !((defer || fold2) && sa.stage < 3)) {
         if ((rc = launch_bc<D, T>(e, uout))) return rc;
+    }
     if (evk) HIPCHK(hipEventRecord(evk[5], e->stream));
     return 0;
 }
