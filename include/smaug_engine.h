@@ -112,9 +112,9 @@ int smaug_get_ratio_form(smaug_engine *e, int *zstreamed, int *kz);
 int smaug_get_zstream(smaug_engine *e, int *enable, int *kz);
 /* 3-D grids: when the uploaded background is the same on every k-plane (a
  * stratified atmosphere whose vertical axis is i), smaug_set_state also
- * keeps it as one k-plane, and the aux/cmax pass and the z-streamed
- * source and update kernels read that plane, once per column, instead of
- * the full background (enable = 1, the default; 0 always reads the full
+ * keeps it as one k-plane, and the z-streamed source and update kernels
+ * and the ghost fill read that plane, once per column, instead of the full
+ * background (enable = 1, the default; 0 always reads the full
  * arrays).  Same arithmetic either way.  smaug_get_bg_plane reports whether
  * the plane is in use. */
 int smaug_set_bg_plane(smaug_engine *e, int enable);
@@ -128,7 +128,10 @@ int smaug_sums(smaug_engine *e, double *sums);
 /* one more RK4 step, with HIP events between its passes: ms[p] is the mean
  * duration over the four stages of pass p = 0 aux quantities and cmax,
  * 1 hyperviscosity ratios, 2 hyperdiffusive sources (3-D z-streamed form
- * only; 0 otherwise), 3 stage update, 4 ghost fill */
+ * only; 0 otherwise), 3 stage update, 4 ghost fill.  Where the stage update
+ * also computes the aux quantities of its output (3-D grids with the
+ * k-plane background; 2-D stages after the first), pass 0 is empty there and
+ * passes 3 and 4 carry that work. */
 int smaug_time_passes(smaug_engine *e, double t0, double dt, double ms[5]);
 
 /* measured copy-bandwidth ceiling: GB/s (read + write bytes) of a plain
