@@ -52,3 +52,15 @@ def test_library_exports_header(lib, header):
 def test_python_binding_lists_every_engine_symbol():
     import smaugamd.engine as eng
     assert sorted(eng.EXPORTS) == declared("smaug_engine.h")
+
+
+def test_grid_below_minimum_rejected_before_device_use():
+    # smaug_create checks the configuration before it looks for a device, so
+    # the size error is the one reported here as on a GPU host
+    import sys
+    sys.path.insert(0, os.path.join(REPO, "smaug-all_amd"))
+    from smaugamd import configs
+    from smaugamd.engine import FP64, Engine, SmaugError
+    s = configs.stratified(nz=3, nx=4, ny=4)   # 7 cells along the vertical axis
+    with pytest.raises(SmaugError, match=r"n\[0\]=7: need >= 8"):
+        Engine(s, FP64)
