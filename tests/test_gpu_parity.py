@@ -87,6 +87,37 @@ def test_rk4_steps_fp64(name, setup):
     assert np.all(err < 1e-11), err
 
 
+
+# mixed path after five RK4 steps against the FP64 oracle run from the same
+# FP32-rounded state, per field, normalised by the field's amplitude: a stated
+# bound on the FP32 stencil arithmetic and state storage over several steps
+# (measured: at most 7e-7 L-infinity and 4.5e-7 L2 on these grids)
+TOL_RK4_MIXED_LINF = 1e-5
+TOL_RK4_MIXED_L2 = 5e-6
+
+
+@pytest.mark.parametrize("name,setup", small_setups(), ids=lambda x: x if isinstance(x, str) else "")
+def test_rk4_steps_mixed_vs_oracle(name, setup):
+    print(f"[case] rk4 mixed vs oracle {name}", flush=True)
+    o = Oracle(setup)
+    w0 = o.boundary(perturbed(setup, amp=1e-3, seed=4))
+    w0 = o.boundary(w0.astype(np.float32).astype(np.float64))
+    dt = 0.3 / configs.initial_max_courant(setup) if setup.dt == 0 else setup.dt * 0.5
+    if name.startswith("ot"):
+        dt = 1e-3
+    ref, _ = o.run(w0, 0.0, dt, 5, rk4=True)
+    e = Engine(setup, MIXED)
+    e.set_state(w0)
+    e.rk4(0.0, dt, 5)
+    got = e.get_state()
+    e.close()
+    m = interior_mask(setup)
+    linf, l2 = rel_err(got, ref, m), rel_l2(got, ref, m)
+    print(name, "mixed rk4 5 steps rel Linf", linf, "rel L2", l2)
+    assert np.all(np.isfinite(got))
+    assert np.all(linf < TOL_RK4_MIXED_LINF), linf
+    assert np.all(l2 < TOL_RK4_MIXED_L2), l2
+
 def test_euler_steps_fp64():
     setup = configs.brio_wu(ncells=64)
     print("[case] euler bw64", flush=True)
