@@ -31,19 +31,21 @@ class SmGrid(ctypes.Structure):
     ]
 
 
-_lib = None
+_libs = {}
 
 
 def build():
     subprocess.check_call(["make", "-s", "-C", HERE])
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def lib(path=None):
+    """the oracle library; ``path`` names another build of the same source
+    (the mutation tests load copies with a planted mistake)"""
+    path = LIB if path is None else path
+    if path not in _libs:
+        if path == LIB and not os.path.exists(LIB):
             build()
-        L = ctypes.CDLL(LIB)
+        L = ctypes.CDLL(path)
         dp = ctypes.POINTER(ctypes.c_double)
         gp = ctypes.POINTER(SmGrid)
         L.sm_ncell.argtypes = [gp]
@@ -61,8 +63,8 @@ def lib():
                              ctypes.c_int]
         L.sm_run.restype = ctypes.c_double
         L.sm_num_threads.restype = ctypes.c_int
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _dp(a):
@@ -72,9 +74,9 @@ def _dp(a):
 class Oracle:
     """Oracle bound to one set-up's grid (configs.Setup-like object)."""
 
-    def __init__(self, s):
+    def __init__(self, s, lib_path=None):
         self.s = s
-        self.L = lib()
+        self.L = lib(lib_path)
         g = SmGrid()
         g.ndim = s.ndim
         self._keep = []

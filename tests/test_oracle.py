@@ -390,3 +390,262 @@ def test_brio_wu_states_match_reference_ini():
         assert np.allclose(got, row, rtol=0, atol=1e-15), (got, row)
     assert s.gamma == fx["eqpar"][0]
     assert fx["y_uniform"] and fx["distinct_rows"] == 2
+
+
+# ---------------------------------------------------------------- hyperdiffusive sources
+# Closed forms on polynomial states of a 3-D box with uniform but unequal
+# spacings.  With q = a*x^3 along d (x = cell index) the differences in
+# index units are D1(f) = q[f+1]-q[f] = a(3f^2+3f+1) and D3(f) = 6a, so
+# away from the pads nu_R(x) = h c chyp 6/D1(x+2) and
+# nu_L(x) = h c chyp 6/D1(x+1).  Each source is then written from the
+# reference's definition of the term:
+#   rho, e:       (nu_R dR q - nu_L dL q)/h       hyperdifrhosource1.cu:15-110,
+#                                                  hyperdifesource1.cu:18-545
+#   m diagonal:   (rhoR nu_R dR v - rhoL nu_L dL v)/(2h) into m_f, the same
+#                 with the face-mean m_f into e    hyperdifmomsource1.cu:17-533
+#   m off-diag.:  d_f[rho (nu_R+nu_L)/4 d_d v_f] into m_d, with m_d for rho
+#                 into e                           hyperdifmomsourcene1.cu:379-463
+#   b diagonal:   (nu_R dR b_f - nu_L dL b_f)/h into b_f, face-mean b_f into e
+#                                                  hyperdifbsource1.cu:21-440
+#   b off-diag.:  -d_f[(nu_R+nu_L)/2 d_d b_f] into b_d, times b_d into e
+#                                                  hyperdifbsourcene1.cu:18-820
+# with the (ii, ii0) and (jj, ii0, mm, sb) patterns of iosmaug.cpp:1046-1144.
+# Every term is compared cell by cell, so a flipped sign, a changed factor,
+# a dropped heating term, a transposed derivative or a wrong cmax/dx/chyp
+# index fails one of them.  dR, dL are one-sided differences /h and d the
+# centred difference (grad13n) /2h; both are exact on the polynomials used.
+
+A3 = 2.0 ** -10           # cubic amplitude: every value below is exact in FP64
+BOX_N = (14, 13, 12)
+BOX_H = (0.7, 1.3, 0.9)
+BOX_CMAX = np.array([1.7, 2.3, 0.6])
+CHI = 0.3
+RHOB = 2.0
+
+
+def box(n=BOX_N, h=BOX_H, bc=(4, 4, 4)):
+    nc = int(np.prod(n))
+    s = configs.Setup(name="box", ndim=3, n=tuple(n), bc=bc, gamma=5.0 / 3.0,
+                      g=(0.0, 0.0, 0.0), chyp=np.zeros(13), hyperdif=1, driver=0,
+                      drv=np.zeros(8), dx=[np.full(n[d], h[d]) for d in range(3)],
+                      pos=[np.arange(n[d]) * h[d] for d in range(3)],
+                      w=np.zeros((8, nc)), bg=np.zeros((5, nc)))
+    s.bg[0] = 10.0
+    s.bg[1] = RHOB
+    return s
+
+
+def hyper_part(s, w, lib_path=None):
+    """L(w) with hyperdiffusion minus L(w) without it, at the fixed cmax"""
+    o = Oracle(s, lib_path)
+    o.g.hyperdif = 1
+    on = o.rhs(w, cmax=BOX_CMAX)
+    o.g.hyperdif = 0
+    off = o.rhs(w, cmax=BOX_CMAX)
+    return on - off
+
+
+def _D1(f):
+    return 3.0 * f * f + 3.0 * f + 1.0
+
+
+def _cubic_terms(x, d, chi):
+    """nu_R, nu_L and the one-sided differences dR, dL (per unit amplitude)
+    of q = x^3 along d at index x"""
+    h = BOX_H[d]
+    K = h * BOX_CMAX[d] * chi
+    nuR = K * 6.0 / _D1(x + 2)
+    nuL = K * 6.0 / _D1(x + 1)
+    dR = _D1(x) / h
+    dL = _D1(x - 1) / h
+    return nuR, nuL, dR, dL
+
+
+def _core(s, dims):
+    """cells whose index along each dim in ``dims`` lies in [3, n-4] (no pad
+    and no edge rule reaches them) and that are interior along the others"""
+    xi = cell_coords(s)
+    m = interior_mask(s)
+    for d in dims:
+        m = m & (xi[d] >= 3) & (xi[d] <= s.n[d] - 4)
+    return m
+
+
+def _close(got, expect, m, what):
+    scale = np.abs(expect[m]).max()
+    assert scale > 0, what
+    err = np.abs(got[m] - expect[m]).max()
+    assert err <= 1e-11 * scale, f"{what}: max error {err:.3e} of {scale:.3e}"
+
+
+def _zero(hp, fields, m, what):
+    for f in fields:
+        assert np.abs(hp[f][m]).max() <= 1e-14 * (1 + np.abs(hp).max()), f"{what}: field {f}"
+
+
+def check_rho_and_energy_sources(lib_path=None):
+    for fld in (0, 4):
+        for d in range(3):
+            s = box()
+            s.chyp[fld] = CHI
+            x = cell_coords(s)[d].astype(float)
+            w = np.zeros((8, s.ncell))
+            w[fld] = A3 * x ** 3 + (5.0 if fld == 4 else 0.0)
+            hp = hyper_part(s, w, lib_path)
+            nuR, nuL, dR, dL = _cubic_terms(x, d, CHI)
+            expect = A3 * (nuR * dR - nuL * dL) / BOX_H[d]
+            m = _core(s, [d])
+            _close(hp[fld], expect, m, f"field {fld} along {d}")
+            _zero(hp, [f for f in range(8) if f != fld], m, f"field {fld} along {d}")
+
+
+def check_momentum_diagonal_sources(lib_path=None):
+    # v_d = m_d/rho = a x_d^3 with rho = rho_b: the (v_d, d) pair is diagonal
+    # in both calls of the (ii, ii0) loop, so m_d gets it twice
+    for d in range(3):
+        s = box()
+        s.chyp[1 + d] = CHI
+        x = cell_coords(s)[d].astype(float)
+        w = np.zeros((8, s.ncell))
+        w[1 + d] = RHOB * A3 * x ** 3
+        hp = hyper_part(s, w, lib_path)
+        nuR, nuL, dR, dL = _cubic_terms(x, d, CHI)
+        h = BOX_H[d]
+        m = _core(s, [d])
+        em = 2 * 0.5 * A3 * (RHOB * nuR * dR - RHOB * nuL * dL) / h
+        mR = RHOB * A3 * (x ** 3 + (x + 1) ** 3) / 2
+        mL = RHOB * A3 * (x ** 3 + (x - 1) ** 3) / 2
+        ee = 2 * 0.5 * A3 * (mR * nuR * dR - mL * nuL * dL) / h
+        _close(hp[1 + d], em, m, f"m{d} diagonal")
+        _close(hp[4], ee, m, f"m{d} diagonal heating")
+        _zero(hp, [f for f in range(8) if f not in (1 + d, 4)], m, f"m{d} diagonal")
+
+
+def check_momentum_offdiagonal_sources(lib_path=None):
+    # v_f = a x_d^3 exactly (m_f = rho v_f, rho = rho_b + r x_f), m_d = s x_f + m0
+    r, sl, m0 = 2.0 ** -6, 2.0 ** -5, 0.125
+    for d in range(3):
+        for f in range(3):
+            if f == d:
+                continue
+            s = box()
+            s.chyp[1 + f] = CHI
+            xi = cell_coords(s)
+            x, y = xi[d].astype(float), xi[f].astype(float)
+            rho = RHOB + r * y
+            w = np.zeros((8, s.ncell))
+            w[0] = r * y
+            w[1 + f] = rho * A3 * x ** 3
+            w[1 + d] = sl * y + m0
+            hp = hyper_part(s, w, lib_path)
+            nuR, nuL, dR, dL = _cubic_terms(x, d, CHI)
+            hd, hf = BOX_H[d], BOX_H[f]
+            m = _core(s, [d, f])
+            # diagonal (v_f, d) into m_f, heating with the face-mean m_f
+            emf = 0.5 * A3 * rho * (nuR * dR - nuL * dL) / hd
+            mR = rho * A3 * (x ** 3 + (x + 1) ** 3) / 2
+            mL = rho * A3 * (x ** 3 + (x - 1) ** 3) / 2
+            ediag = 0.5 * A3 * (mR * nuR * dR - mL * nuL * dL) / hd
+            # off-diagonal: d_f[rho g6] into m_d, d_f[g6 m_d] into e, with
+            # g6 = d_d v_f (nu_R + nu_L)/4 independent of x_f
+            g6 = A3 * (6 * x * x + 2) / (2 * hd) * (nuR + nuL) / 4
+            emd = g6 * r / hf
+            eoff = g6 * sl / hf
+            _close(hp[1 + f], emf, m, f"(v{f}, dim {d}) diagonal into m{f}")
+            _close(hp[1 + d], emd, m, f"(v{f}, dim {d}) off-diagonal into m{d}")
+            _close(hp[4], ediag + eoff, m, f"(v{f}, dim {d}) heating")
+            t = 3 - d - f
+            _zero(hp, [0, 1 + t, 5, 6, 7], m, f"(v{f}, dim {d})")
+
+
+def check_magnetic_sources(lib_path=None):
+    # b_f = a x_d^3 (1 + r x_f), b_d = beta: nu of (b_f, d) is the cubic's,
+    # d_d b_f is linear in x_f
+    r, beta = 2.0 ** -4, 0.75
+    for d in range(3):
+        for f in range(3):
+            if f == d:
+                continue
+            s = box()
+            s.chyp[5 + f] = CHI
+            xi = cell_coords(s)
+            x, y = xi[d].astype(float), xi[f].astype(float)
+            fac = 1.0 + r * y
+            w = np.zeros((8, s.ncell))
+            w[5 + f] = A3 * x ** 3 * fac
+            w[5 + d] = beta
+            hp = hyper_part(s, w, lib_path)
+            nuR, nuL, dR, dL = _cubic_terms(x, d, CHI)
+            hd, hf = BOX_H[d], BOX_H[f]
+            m = _core(s, [d, f])
+            ebf = A3 * fac * (nuR * dR - nuL * dL) / hd
+            bR = A3 * fac * (x ** 3 + (x + 1) ** 3) / 2
+            bL = A3 * fac * (x ** 3 + (x - 1) ** 3) / 2
+            ediag = A3 * fac * (bR * nuR * dR - bL * nuL * dL) / hd
+            ebd = -(nuR + nuL) / 2 * A3 * (6 * x * x + 2) / (2 * hd) * r / hf
+            _close(hp[5 + f], ebf, m, f"(b{f}, dim {d}) diagonal into b{f}")
+            _close(hp[5 + d], ebd, m, f"(b{f}, dim {d}) off-diagonal into b{d}")
+            _close(hp[4], ediag + beta * ebd, m, f"(b{f}, dim {d}) heating")
+            t = 3 - d - f
+            _zero(hp, [0, 1, 2, 3, 5 + t], m, f"(b{f}, dim {d})")
+
+
+def check_hyperdiffusion_dissipates(lib_path=None):
+    """on a rough periodic field each diagonal hyperdiffusion term alone
+    lowers sum q^2 (sum of q dq/dt < 0) for q = rho', e', v_f and b_f"""
+    rng = np.random.default_rng(17)
+    for fld in (0, 4, 1, 2, 3, 5, 6, 7):
+        s = box(n=(12, 12, 12), bc=(0, 0, 0))
+        s.chyp[fld] = CHI
+        w = np.zeros((8, s.ncell))
+        w[fld] = 1e-2 * rng.standard_normal(s.ncell) + (5.0 if fld == 4 else 0.0)
+        w = Oracle(s, lib_path).boundary(w)
+        hp = hyper_part(s, w, lib_path)
+        m = interior_mask(s)
+        q = w[fld] / RHOB if 1 <= fld <= 3 else w[fld] - (5.0 if fld == 4 else 0.0)
+        prod = (q * hp[fld])[m]
+        assert prod.sum() < -0.5 * np.abs(prod).sum(), f"field {fld} is not dissipated"
+
+
+def check_maxcourant_stretched(lib_path=None):
+    """uniform state on a grid stretched along one dim: the Courant limit is
+    cf_d / min(dx_d) along the dim that maximises it (each cell divides by
+    its own dx; the smallest cell is not at the edge)"""
+    gam = 5.0 / 3.0
+    rho, p = 1.3, 0.7
+    v = np.array([0.2, -0.4, 0.1])
+    B = np.array([0.5, 0.3, -0.6])
+    cs2 = gam * p / rho
+    va2 = B @ B / rho
+    cf = [math.sqrt(0.5 * (cs2 + va2 + math.sqrt((cs2 + va2) ** 2 - 4 * cs2 * B[d] ** 2 / rho)))
+          + abs(v[d]) for d in range(3)]
+    for sd in range(3):
+        s = box(n=(12, 11, 10))
+        dxs = []
+        for d in range(3):
+            n = s.n[d]
+            if d == sd:
+                t = np.linspace(-1.0, 1.0, n)
+                dxs.append(0.05 * (1.0 + 3.0 * t * t + 0.5 * t))  # minimum inside
+            else:
+                dxs.append(np.full(n, 1.0))
+        s.dx = dxs
+        s.bg[:] = 0.0
+        w = np.zeros((8, s.ncell))
+        w[0] = rho
+        w[1:4] = (rho * v)[:, None]
+        w[4] = p / (gam - 1) + 0.5 * rho * v @ v + 0.5 * B @ B
+        w[5:8] = B[:, None]
+        got = Oracle(s, lib_path).maxcourant(w)
+        expect = cf[sd] / dxs[sd].min()
+        assert abs(got - expect) <= 1e-12 * expect, (sd, got, expect)
+
+
+SOURCE_CHECKS = [check_rho_and_energy_sources, check_momentum_diagonal_sources,
+                 check_momentum_offdiagonal_sources, check_magnetic_sources,
+                 check_hyperdiffusion_dissipates, check_maxcourant_stretched]
+
+
+@pytest.mark.parametrize("check", SOURCE_CHECKS, ids=lambda c: c.__name__[6:])
+def test_hyperdiffusive_sources_and_courant(check):
+    check()
