@@ -125,6 +125,21 @@ int smaug_get_bg_plane(smaug_engine *e, int *active);
  * (Neumaier) summation on the device and over the per-block partials */
 int smaug_sums(smaug_engine *e, double *sums);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the sums of
+ * smaug_sums for an evolved state d_w that the caller holds on the device
+ * (layout of smaug_set_state; double when precision is SMAUG_FP64, float when
+ * SMAUG_MIXED; n[2] ignored for ndim 2), written as 2+2*ndim doubles to the
+ * device array d_sums.  The work is enqueued on `stream` (a hipStream_t;
+ * NULL for the default stream) and the call returns without waiting for it.
+ * Allocates nothing: the per-block partials live in a static device array of
+ * 64 KiB that the library holds on each device it runs on until the process
+ * ends.  Every call on one device shares that array, so calls that overlap
+ * on one device (other streams, other host threads) race and must be
+ * ordered by the caller.  Grids above 2^29 cells are rejected. */
+int smaug_sums_noalloc(int ndim, const int n[3], int precision, const void *d_w,
+                       double *d_sums, void *stream);
+
 /* one more RK4 step, with HIP events between its passes: ms[p] is the mean
  * duration over the four stages of pass p = 0 aux quantities and cmax,
  * 1 hyperviscosity ratios, 2 hyperdiffusive sources (3-D z-streamed form

@@ -10,6 +10,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
+
 using namespace smaug;
 
 static thread_local char g_err[512] = "";
@@ -1358,6 +1360,106 @@ extern "C" int smaug_sums(smaug_engine *e, double *sums)
     if (!e || !sums) return fail(SMAUG_EINVAL, "null argument");
     if (!e->d_part) HIPCHK(hipMalloc((void **)&e->d_part, sizeof(double) * 2 * 8 * sums_blocks()));
     return field_sums(e->D, e->prec, e->geo, e->U[0], e->stream, e->d_part, sums);
+}
+
+// ---------------------------------------------------------------------------
+// conservation diagnostics without allocation (smaug_sums_noalloc)
+//
+// The per-block partials of k_sums live in `scratch`, a static device array
+// that every call on a device shares (2 * 8 * sums_blocks() doubles, 64 KiB).
+// Host code never passes `scratch` itself to a kernel: its device address is
+// looked up once per device with hipGetSymbolAddress and kept in
+// g_scratch_ptr.  k_sums writes every partial of every block, so nothing in
+// the array needs clearing between calls.
+// ---------------------------------------------------------------------------
+static constexpr int SCRATCH_DOUBLES = 2 * 8 * sums_blocks();
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+static constexpr int MAX_DEVICES = 64;
+static double *g_scratch_ptr[MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// the device address of `scratch` on the current device (looked up on the
+// first call there)
+static int scratch_on_current_device(double **out)
+{
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(SMAUG_ENODEV, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+// the per-block (sum, compensation) pairs folded field by field in block
+// order with Neumaier's step, the same operations in the same order as the
+// host pass of field_sums, so both give the same bits
+__global__ __launch_bounds__(64) void k_sums_merge(const double *__restrict__ part, int nb,
+                                                   int nve, double *__restrict__ out)
+{
+    const int f = threadIdx.x;
+    if (f >= nve) return;
+    double s = 0.0, c = 0.0;
+    for (int b = 0; b < nb; b++) {
+        const double x = part[(size_t)b * 2 * nve + 2 * f];
+        const double t = s + x;
+        c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+        s = t;
+        c += part[(size_t)b * 2 * nve + 2 * f + 1];
+    }
+    out[f] = s + c;
+}
+
+extern "C" int smaug_sums_noalloc(int ndim, const int n[3], int precision, const void *d_w,
+                                  double *d_sums, void *stream)
+{
+    if (!n || !d_w || !d_sums) return fail(SMAUG_EINVAL, "null argument");
+    if (ndim != 2 && ndim != 3) return fail(SMAUG_EINVAL, "ndim must be 2 or 3 (got %d)", ndim);
+    if (precision != SMAUG_FP64 && precision != SMAUG_MIXED)
+        return fail(SMAUG_EINVAL, "precision must be SMAUG_FP64 or SMAUG_MIXED");
+    for (int d = 0; d < ndim; d++)
+        if (n[d] < 5) return fail(SMAUG_EINVAL, "n[%d]=%d: need >= 5 cells incl. ghosts", d, n[d]);
+    const long long nc = (long long)n[0] * n[1] * (ndim == 3 ? n[2] : 1);
+    if (nc > MAX_CELLS)
+        return fail(SMAUG_EINVAL,
+                    "%lld cells exceed the limit of %lld cells per grid of smaug_sums_noalloc", nc,
+                    MAX_CELLS);
+    const int nve = 2 + 2 * ndim, nb = sums_blocks();
+    if (2 * nve * nb > SCRATCH_DOUBLES)
+        return fail(SMAUG_EINVAL, "scratch of %d doubles is smaller than the %d partials", SCRATCH_DOUBLES,
+                    2 * nve * nb);
+    double *part = nullptr;
+    int rc = scratch_on_current_device(&part);
+    if (rc) return rc;
+    Geo g;
+    memset((void *)&g, 0, sizeof(g));
+    g.n[0] = n[0];
+    g.n[1] = n[1];
+    g.n[2] = ndim == 3 ? n[2] : 1;
+    g.s1 = n[0];
+    g.s2 = (idx_t)n[0] * n[1];
+    g.nc = nc;
+    hipStream_t st = (hipStream_t)stream;
+    if (ndim == 2) {
+        if (precision == SMAUG_FP64)
+            hipLaunchKernelGGL((k_sums<2, double>), dim3(nb), dim3(256), 0, st, g, (const double *)d_w, part);
+        else
+            hipLaunchKernelGGL((k_sums<2, float>), dim3(nb), dim3(256), 0, st, g, (const float *)d_w, part);
+    } else {
+        if (precision == SMAUG_FP64)
+            hipLaunchKernelGGL((k_sums<3, double>), dim3(nb), dim3(256), 0, st, g, (const double *)d_w, part);
+        else
+            hipLaunchKernelGGL((k_sums<3, float>), dim3(nb), dim3(256), 0, st, g, (const float *)d_w, part);
+    }
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_sums_merge, dim3(1), dim3(64), 0, st, (const double *)part, nb, nve, d_sums);
+    HIPCHK(hipGetLastError());
+    return 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -27,7 +27,7 @@ EXPORTS = [
     "smaug_set_panel", "smaug_get_panel", "smaug_time_pass", "smaug_autotune",
     "smaug_set_zstream", "smaug_get_zstream", "smaug_time_passes", "smaug_set_ratio_form",
     "smaug_get_ratio_form", "smaug_set_bg_plane", "smaug_get_bg_plane",
-    "smaug_sums", "smaug_ens_sums", "smaug_ens_time_passes",
+    "smaug_sums", "smaug_sums_noalloc", "smaug_ens_sums", "smaug_ens_time_passes",
     "smaug_ens_create", "smaug_ens_destroy", "smaug_ens_nrun", "smaug_ens_ninterior",
     "smaug_ens_set_state", "smaug_ens_get_state", "smaug_ens_rk4", "smaug_ens_time_rk4",
 ]
@@ -94,6 +94,7 @@ def load_library(path: str = LIB_PATH):
         lib.smaug_set_bg_plane.argtypes = [vp, ctypes.c_int]
         lib.smaug_get_bg_plane.argtypes = [vp, ip]
         lib.smaug_sums.argtypes = [vp, dp]
+        lib.smaug_sums_noalloc.argtypes = [ctypes.c_int, ip, ctypes.c_int, vp, vp, vp]
         lib.smaug_ens_sums.argtypes = [vp, ctypes.c_int, dp]
         lib.smaug_ens_time_passes.argtypes = [vp, dp, dp, dp]
         lib.smaug_time_passes.argtypes = [vp, ctypes.c_double, ctypes.c_double, dp]
@@ -109,6 +110,19 @@ def load_library(path: str = LIB_PATH):
         lib.smaug_ens_time_rk4.argtypes = [vp, dp, dp, ctypes.c_int, dp]
         _lib = lib
     return _lib
+
+
+def sums_noalloc(ndim: int, n, precision: int, d_w: int, d_sums: int, stream: int = 0) -> int:
+    """smaug_sums_noalloc on device addresses (integers, e.g. a torch
+    tensor's data_ptr()); returns the library's status code"""
+    lib = load_library()
+    nn = (ctypes.c_int * 3)(*[int(x) for x in (list(n) + [1, 1, 1])[:3]])
+    return lib.smaug_sums_noalloc(ndim, nn, precision, ctypes.c_void_p(d_w),
+                                  ctypes.c_void_p(d_sums), ctypes.c_void_p(stream or None))
+
+
+def last_error() -> str:
+    return load_library().smaug_last_error().decode()
 
 
 def _dptr(a: np.ndarray):

@@ -64,3 +64,16 @@ def test_grid_below_minimum_rejected_before_device_use():
     s = configs.stratified(nz=3, nx=4, ny=4)   # 7 cells along the vertical axis
     with pytest.raises(SmaugError, match=r"n\[0\]=7: need >= 8"):
         Engine(s, FP64)
+
+
+def test_sums_noalloc_rejects_bad_sizes_before_device_use():
+    # argument and size checks come before any HIP call
+    from smaugamd import engine as eng
+    big = (2048, 2048, 2048)  # 2^33 cells: above the 2^29-cell limit
+    assert eng.sums_noalloc(3, big, eng.MIXED, 1, 1) == 1
+    assert "exceed the limit" in eng.last_error()
+    assert eng.sums_noalloc(3, (4, 8, 8), eng.FP64, 1, 1) == 1
+    assert "n[0]=4" in eng.last_error()
+    assert eng.sums_noalloc(4, (8, 8, 8), eng.FP64, 1, 1) == 1
+    assert eng.sums_noalloc(3, (8, 8, 8), 7, 1, 1) == 1
+    assert eng.sums_noalloc(3, (8, 8, 8), eng.FP64, 0, 1) == 1
