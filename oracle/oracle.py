@@ -54,6 +54,12 @@ def lib(path=None):
         L.sm_cmax.argtypes = [gp, dp, dp, dp]
         L.sm_maxcourant.argtypes = [gp, dp, dp]
         L.sm_maxcourant.restype = ctypes.c_double
+        L.sm_maxcourant_dim.argtypes = [gp, dp, dp, ctypes.c_int]
+        L.sm_maxcourant_dim.restype = ctypes.c_double
+        L.sm_numax.argtypes = [gp, dp, dp, dp]
+        L.sm_numax.restype = ctypes.c_double
+        L.sm_dtvisc.argtypes = [gp, ctypes.c_double]
+        L.sm_dtvisc.restype = ctypes.c_double
         L.sm_nu.argtypes = [gp, dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp, dp]
         L.sm_grad4.argtypes = [gp, dp, ctypes.c_int, dp]
         L.sm_rhs.argtypes = [gp, dp, dp, ctypes.c_double, dp, dp]
@@ -120,6 +126,24 @@ class Oracle:
         bg = np.ascontiguousarray(self.s.bg if bg is None else bg)
         w = self._w(w)
         return self.L.sm_maxcourant(ctypes.byref(self.g), _dp(w), _dp(bg))
+
+    def maxcourant_dim(self, w, d, bg=None):
+        bg = np.ascontiguousarray(self.s.bg if bg is None else bg)
+        w = self._w(w)
+        return self.L.sm_maxcourant_dim(ctypes.byref(self.g), _dp(w), _dp(bg), d)
+
+    def numax(self, w, cmax=None, bg=None):
+        bg = np.ascontiguousarray(self.s.bg if bg is None else bg)
+        w = self._w(w)
+        cp = None
+        if cmax is not None:
+            cm = np.ascontiguousarray(cmax, dtype=np.float64)
+            self._keep.append(cm)
+            cp = _dp(cm)
+        return self.L.sm_numax(ctypes.byref(self.g), _dp(w), _dp(bg), cp)
+
+    def dtvisc(self, numax):
+        return self.L.sm_dtvisc(ctypes.byref(self.g), float(numax))
 
     def nu(self, w, field, dim, cmax_d, bg=None):
         bg = np.ascontiguousarray(self.s.bg if bg is None else bg)

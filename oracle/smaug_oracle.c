@@ -307,20 +307,30 @@ void sm_cmax(const sm_grid *g, const double *w, const double *bg, double cmax[3]
     }
 }
 
-/* cucomputemaxcourant: max over cells of cfast_d / dx_d, all dims */
-double sm_maxcourant(const sm_grid *g, const double *w, const double *bg)
+/* cucomputemaxcourant for one direction: max over cells of cfast_d / dx_d
+ * (computedervfields.cu:2118-2296, zeropadmaxcourant_parallel 1296-1340) */
+double sm_maxcourant_dim(const sm_grid *g, const double *w, const double *bg, int d)
 {
     geo G = mkgeo(g);
     double m = 0.0;
-    for (int d = 0; d < G.D; d++) {
 #pragma omp parallel for collapse(2) reduction(max : m) schedule(static)
-        for (int k_ = 0; k_ < G.n[2]; k_++)
-            for (int j_ = 0; j_ < G.n[1]; j_++)
-                for (int i_ = 0; i_ < G.n[0]; i_++) {
-                    long c = CIDX(G);
-                    double v = cfast_at(&G, g, w, bg, c, d) / g->dx[d][coord(d, i_, j_, k_)];
-                    if (v > m) m = v;
-                }
+    for (int k_ = 0; k_ < G.n[2]; k_++)
+        for (int j_ = 0; j_ < G.n[1]; j_++)
+            for (int i_ = 0; i_ < G.n[0]; i_++) {
+                long c = CIDX(G);
+                double v = cfast_at(&G, g, w, bg, c, d) / g->dx[d][coord(d, i_, j_, k_)];
+                if (v > m) m = v;
+            }
+    return m;
+}
+
+/* the Courant limit over every direction: max_d of sm_maxcourant_dim */
+double sm_maxcourant(const sm_grid *g, const double *w, const double *bg)
+{
+    double m = 0.0;
+    for (int d = 0; d < g->ndim; d++) {
+        double v = sm_maxcourant_dim(g, w, bg, d);
+        if (v > m) m = v;
     }
     return m;
 }
@@ -409,6 +419,67 @@ void sm_nu(const sm_grid *g, const double *w, const double *bg, int field, int d
     hyper_q(&G, w, bg, field, q);
     nu_from_q(&G, g, q, field, dim, cmax_d, nuR, nuL);
     free(q);
+}
+
+/* Largest hyperviscosity coefficient of a state: max of nu_R and nu_L over
+ * the cells and over the (field, dim) pairs the hyperdiffusion visits (rho
+ * and e along every dim, v_f along every dim, b_f along every dim but f),
+ * each with cmax[dim].  With moddton the reference keeps this maximum in
+ * p->maxviscoef while it forms the coefficients of a step
+ * (hyperdifvisc1r.cu:1303-1330, hyperdifvisc1l.cu:1373-1411, reset at
+ * iosmaug.cpp:973). */
+double sm_numax(const sm_grid *g, const double *w, const double *bg, const double *cmax_in)
+{
+    geo G = mkgeo(g);
+    int D = G.D;
+    double cmax[3];
+    if (cmax_in) {
+        for (int d = 0; d < 3; d++) cmax[d] = cmax_in[d];
+    } else {
+        sm_cmax(g, w, bg, cmax);
+    }
+    double *q = alloc(G.nc), *nuR = alloc(G.nc), *nuL = alloc(G.nc);
+    double m = 0.0;
+    int fields[8], nf = 0;
+    fields[nf++] = F_RHO;
+    fields[nf++] = F_EN(D);
+    for (int f = 0; f < D; f++) fields[nf++] = F_MOM(D, f);
+    for (int f = 0; f < D; f++) fields[nf++] = F_B(D, f);
+    for (int a = 0; a < nf; a++) {
+        int fld = fields[a];
+        hyper_q(&G, w, bg, fld, q);
+        for (int dim = 0; dim < D; dim++) {
+            if (fld >= F_B(D, 0) && fld - F_B(D, 0) == dim) continue;
+            nu_from_q(&G, g, q, fld, dim, cmax[dim], nuR, nuL);
+            for (long c = 0; c < G.nc; c++) {
+                if (nuR[c] > m) m = nuR[c];
+                if (nuL[c] > m) m = nuL[c];
+            }
+        }
+    }
+    free(q);
+    free(nuR);
+    free(nuL);
+    return m;
+}
+
+/* Hyperviscous time-step limit, SAC getdt_visc (vacusr.viscosity.t:512-546,
+ * dtdiffpar = 0.25): 0.25 dx^2 / numax, minimised over the dims and the
+ * cells' own dx; 0 when numax is 0 (no limit).  The reference's cugetdtvisc1
+ * keeps this formula in a comment (nushk1.cu:688-694); its reduction
+ * (nushk1.cu:700-761) carries each direction's result into the next
+ * direction's maximum, which SAC does not. */
+double sm_dtvisc(const sm_grid *g, double numax)
+{
+    if (!(numax > 0)) return 0.0;
+    geo G = mkgeo(g);
+    double best = -1.0;
+    for (int d = 0; d < G.D; d++)
+        for (int x = 0; x < G.n[d]; x++) {
+            double t = 0.25 * g->dx[d][x] * g->dx[d][x] / numax;
+            if (best < 0 || t < best) best = t;
+        }
+    return best;
 }
 
 /* ------------------------------------------------------------------------

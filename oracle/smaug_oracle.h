@@ -47,6 +47,15 @@ void sm_cmax(const sm_grid *g, const double *w, const double *bg, double cmax[3]
 
 /* Courant limit max_d max_c cfast_d/dx_d (cucomputemaxcourant) */
 double sm_maxcourant(const sm_grid *g, const double *w, const double *bg);
+/* the same for one direction d: max_c cfast_d/dx_d */
+double sm_maxcourant_dim(const sm_grid *g, const double *w, const double *bg, int d);
+
+/* largest hyperviscosity coefficient nu_R / nu_L over the cells and the
+ * (field, dim) pairs of the hyperdiffusion (p->maxviscoef with moddton);
+ * cmax may be NULL (then computed) */
+double sm_numax(const sm_grid *g, const double *w, const double *bg, const double *cmax);
+/* hyperviscous dt limit 0.25 min(dx^2) / numax (SAC getdt_visc); 0 if numax is 0 */
+double sm_dtvisc(const sm_grid *g, double numax);
 
 /* hyperviscosity coefficients nu_R, nu_L for one (field, dim) (cuhyperdifvisc1r/1l) */
 void sm_nu(const sm_grid *g, const double *w, const double *bg, int field, int dim,

@@ -641,9 +641,64 @@ def check_maxcourant_stretched(lib_path=None):
         assert abs(got - expect) <= 1e-12 * expect, (sd, got, expect)
 
 
+def check_numax_and_dtvisc(lib_path=None):
+    """numax: a field linear along d has D3 = 0 except where a window reaches
+    a pad, where nu = h_d cmax_d chyp 6/5 (q[-1] = q[5] gives D3 = 6,
+    max D1 = 5); no other pair of the state has a nonzero coefficient, and a
+    b_f that varies only along f has none at all.  dtvisc: 0.25 min dx^2 /
+    numax over the dims and the cells' own dx."""
+    chyp = np.zeros(13)
+    chyp[[0, 1, 2, 3, 4, 5, 6, 7]] = [0.11, 0.23, 0.29, 0.31, 0.17, 0.13, 0.19, 0.37]
+    for fld in range(8):
+        for d in range(3):
+            s = box()
+            s.chyp = chyp.copy()
+            x = cell_coords(s)[d].astype(float)
+            w = np.zeros((8, s.ncell))
+            w[fld] = (RHOB * x) if 1 <= fld <= 3 else x + (5.0 if fld == 4 else 0.0)
+            # e' cancels the kinetic or magnetic part of the internal energy
+            # e' - (b'^2 + m^2/rho)/2 (exact on these integers), so only
+            # field fld carries a coefficient
+            if 1 <= fld <= 3:
+                w[4] = w[fld] ** 2 / (2 * RHOB)
+            elif fld >= 5:
+                w[4] = w[fld] ** 2 / 2
+            got = Oracle(s, lib_path).numax(w, cmax=BOX_CMAX)
+            if fld >= 5 and fld - 5 == d:
+                expect = 0.0  # (b_d, d) is not a hyperdiffusion pair
+            else:
+                expect = BOX_H[d] * BOX_CMAX[d] * chyp[fld] * 6.0 / 5.0
+            assert abs(got - expect) <= 1e-14 * max(expect, 1e-300), (fld, d, got, expect)
+    # one-sided profiles: linear at one end and flat at the other, so the
+    # pad value 6/5 comes from nu_L at x = 2 alone (q = min(x, 5)) or from
+    # nu_R at x = n-3 alone (q = max(x, n-6)); the kink gives 1
+    for d in range(3):
+        s = box()
+        s.chyp = chyp.copy()
+        x = cell_coords(s)[d].astype(float)
+        n = s.n[d]
+        for q in (np.minimum(x, 5.0), np.maximum(x, n - 6.0)):
+            w = np.zeros((8, s.ncell))
+            w[0] = q
+            got = Oracle(s, lib_path).numax(w, cmax=BOX_CMAX)
+            expect = BOX_H[d] * BOX_CMAX[d] * chyp[0] * 6.0 / 5.0
+            assert abs(got - expect) <= 1e-14 * expect, (d, got, expect)
+    s = box(n=(12, 11, 10))
+    for d in range(3):
+        t = np.linspace(-1.0, 1.0, s.n[d])
+        s.dx[d] = (0.3 + d * 0.1) * (1.0 + 2.0 * t * t + 0.3 * t)
+    o = Oracle(s, lib_path)
+    numax = 0.37
+    expect = 0.25 * min(float((dx ** 2).min()) for dx in s.dx) / numax
+    got = o.dtvisc(numax)
+    assert abs(got - expect) <= 1e-15 * expect, (got, expect)
+    assert o.dtvisc(0.0) == 0.0
+
+
 SOURCE_CHECKS = [check_rho_and_energy_sources, check_momentum_diagonal_sources,
                  check_momentum_offdiagonal_sources, check_magnetic_sources,
-                 check_hyperdiffusion_dissipates, check_maxcourant_stretched]
+                 check_hyperdiffusion_dissipates, check_maxcourant_stretched,
+                 check_numax_and_dtvisc]
 
 
 @pytest.mark.parametrize("check", SOURCE_CHECKS, ids=lambda c: c.__name__[6:])

@@ -60,6 +60,24 @@ MUTANTS = [
     ("nu_uses_wrong_cmax",
      "nu_from_q(&G, g, q, fld, dim, cmax[dim], nuR, nuL);\n                for (int ii1",
      "nu_from_q(&G, g, q, fld, dim, cmax[0], nuR, nuL);\n                for (int ii1"),
+    ("numax_right_side_only",
+     "if (nuL[c] > m) m = nuL[c];",
+     "(void)0;"),
+    ("numax_left_side_only",
+     "if (nuR[c] > m) m = nuR[c];",
+     "(void)0;"),
+    ("numax_visits_b_along_own_dim",
+     "if (fld >= F_B(D, 0) && fld - F_B(D, 0) == dim) continue;",
+     "(void)0;"),
+    ("numax_one_cmax",
+     "nu_from_q(&G, g, q, fld, dim, cmax[dim], nuR, nuL);\n            for (long c",
+     "nu_from_q(&G, g, q, fld, dim, cmax[0], nuR, nuL);\n            for (long c"),
+    ("dtvisc_dx_not_squared",
+     "double t = 0.25 * g->dx[d][x] * g->dx[d][x] / numax;",
+     "double t = 0.25 * g->dx[d][x] / numax;"),
+    ("dtvisc_first_cell_only",
+     "for (int x = 0; x < G.n[d]; x++) {\n            double t = 0.25",
+     "for (int x = 0; x < 1; x++) {\n            double t = 0.25"),
     ("maxcourant_edge_dx",
      "cfast_at(&G, g, w, bg, c, d) / g->dx[d][coord(d, i_, j_, k_)];",
      "cfast_at(&G, g, w, bg, c, d) / g->dx[d][0];"),
