@@ -15,8 +15,12 @@
  *   smaug_main -i in.ini [-o out.ini] [-n steps] [-dt dt] [-rk4] [-mixed]
  *              [-bc b0,b1[,b2]] [-hyperdif 0|1] [-chyp rho,e,mom,b]
  *              [-courant c] [-moddton] [-driver A,P,sh,z0,sz,xc,yc] [-torsional]
- *              [-t0 t]
- * -torsional selects the reference's torsional driver (usersource.cu), with
+ *              [-t0 t] [-tend T] [-dtlog file]
+ * -tend stops the onestep loop once the elapsed time (the sum of the steps'
+ * dt) reaches T, the last step shortened to land on T, as SAC does with
+ * tmaxexact (dev/sac/src/vac.t:86, 606); without -n the loop then runs until
+ * T.  -dtlog writes the step number and dt of every onestep step to a file,
+ * one line each.  -torsional selects the reference's torsional driver (usersource.cu), with
  * the -driver values if given and the reference's constants otherwise
  */
 #include <stdio.h>
@@ -52,14 +56,21 @@ static int parse_list(const char *s, double *v, int maxn)
 
 int main(int argc, char **argv)
 {
-    const char *in = NULL, *out = NULL;
+    const char *in = NULL, *out = NULL, *dtlog = NULL;
     int nt = 1, rk4 = 0, mixed = 0, hyperdif = 1, moddton = 0, bcset = 0, driver = 0, have_drv = 0;
-    double dtarg = 0, courant = 0.3, t0 = 0;
+    double dtarg = 0, courant = 0.3, t0 = 0, tend = 0;
+    int nset = 0;
     double bcv[3] = {4, 4, 4}, chv[4] = {0.02, 0.02, 0.4, 0.02}, drv[7] = {0};
     for (int a = 1; a < argc; a++) {
         if (!strcmp(argv[a], "-i") && a + 1 < argc) in = argv[++a];
         else if (!strcmp(argv[a], "-o") && a + 1 < argc) out = argv[++a];
-        else if (!strcmp(argv[a], "-n") && a + 1 < argc) nt = atoi(argv[++a]);
+        else if (!strcmp(argv[a], "-n") && a + 1 < argc) {
+            nt = atoi(argv[++a]);
+            nset = 1;
+        } else if (!strcmp(argv[a], "-tend") && a + 1 < argc)
+            tend = atof(argv[++a]);
+        else if (!strcmp(argv[a], "-dtlog") && a + 1 < argc)
+            dtlog = argv[++a];
         else if (!strcmp(argv[a], "-dt") && a + 1 < argc) dtarg = atof(argv[++a]);
         else if (!strcmp(argv[a], "-t0") && a + 1 < argc) t0 = atof(argv[++a]);
         else if (!strcmp(argv[a], "-rk4")) rk4 = 1;
@@ -79,6 +90,11 @@ int main(int argc, char **argv)
             return 2;
         }
     }
+    if (tend > 0 && rk4) {
+        fprintf(stderr, "-tend applies to the onestep loop only\n");
+        return 2;
+    }
+    if (tend > 0 && !nset) nt = 1000000000;
     if (!in) {
         fprintf(stderr, "usage: %s -i config.ini [-o out.ini] [-n steps] [-dt dt] [-rk4] ...\n", argv[0]);
         return 2;
@@ -168,6 +184,13 @@ int main(int argc, char **argv)
         CALL(cuadvancerk4(&p, &d_p, &d_wmod, nt, mixed));
         p->it = nt;
     }
+    double elapsed = 0.0; /* sum of the steps' dt */
+    FILE *flog = NULL;
+    if (dtlog && !(flog = fopen(dtlog, "w"))) {
+        fprintf(stderr, "cannot open %s\n", dtlog);
+        return 1;
+    }
+    int nrun = p->rkon == 1 ? nt : 0;
     for (int n = 1; p->rkon == 0 && n <= nt; n++) {
         int order = 0, ordero = 1;
         p->it = n;
@@ -184,6 +207,7 @@ int main(int argc, char **argv)
             if (n > 1) CALL(cugetdtvisc1(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2));
             if ((p->dtdiffvisc) > 1.0e-8 && (p->dt) > (p->dtdiffvisc)) p->dt = (p->dtdiffvisc);
         }
+        if (tend > 0 && elapsed + p->dt > tend) p->dt = tend - elapsed;
         p->qt = (p->qt) + dt;
         for (int dir = 0; dir < NDIM; dir++) CALL(cucomputevels(&p, &d_p, &d_wmod, &d_wd, order, dir));
         for (int dir = 0; dir < NDIM; dir++) {
@@ -273,13 +297,19 @@ int main(int argc, char **argv)
         for (int ii = 0; ii <= (b1 + (NDIM - 1)); ii++)
             for (int idir = 0; idir < NDIM; idir++)
                 CALL(cuboundary(&p, &bp, &d_p, &d_bp, &d_state, &d_wmod, ordero, idir, ii));
+        elapsed += p->dt;
+        nrun = n;
+        if (flog) fprintf(flog, "%d %.17g\n", n, p->dt);
+        if (tend > 0 && elapsed >= tend) break;
     }
+    if (flog) fclose(flog);
     /* the newest state is in slot 1 after an onestep loop: bring it to slot 0 */
     if (p->rkon == 0) CALL(cucomputedervfields(&p, &d_p, &d_wmod, &d_wd, 0));
-    CALL(cuupdate(&p, &wmod, &wmod, &d_wtemp2, &st, &d_p, &d_wmod, &d_wmod, &d_wtemp2, &d_state, nt));
+    CALL(cuupdate(&p, &wmod, &wmod, &d_wtemp2, &st, &d_p, &d_wmod, &d_wmod, &d_wtemp2, &d_state, nrun));
     st->it = p->it;
     st->t = p->qt;
-    printf("smaug_main: %d steps, t = %.10g, dt = %.10g\n", nt, p->qt, p->dt);
+    printf("smaug_main: %d steps, t = %.10g, dt = %.17g, elapsed = %.17g\n", nrun, p->qt, p->dt,
+           p->rkon == 1 ? nt * dt : elapsed);
     if (out) CALL(writeasciivacconfig((char *)out, *p, meta, wmod, wd, hlines, *st, 0));
     CALL(cufinish(&p, &wmod, &wnew, &st, &d_p, &d_bp, &d_wmod, &d_wnew, &d_wmod, &d_dwn1, &d_wd,
                   &d_state, &d_wtemp, &d_wtemp1, &d_wtemp2));
