@@ -57,7 +57,9 @@ int smaug_create(const smaug_cfg *cfg, smaug_engine **out);
 int smaug_destroy(smaug_engine *e);
 
 /* upload state (FP64 host arrays; rounded to FP32 for evolved fields in
- * mixed mode) and fill the ghosts */
+ * mixed mode) and fill the ghosts.  Along a periodic dim the background of
+ * every ghost cell must equal that of its periodic image (the ghost fill
+ * copies the perturbation only); SMAUG_EINVAL otherwise. */
 int smaug_set_state(smaug_engine *e, const double *w, const double *bg);
 /* download the evolved state of the current step (FP64 host array) */
 int smaug_get_state(smaug_engine *e, double *w);

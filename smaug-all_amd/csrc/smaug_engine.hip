@@ -51,6 +51,34 @@ int smaug::check_cfg(const smaug_cfg *cfg)
     return 0;
 }
 
+int smaug::check_periodic_bg(int D, const int n[3], const int bc[3], const double *bg)
+{
+    const int n2 = D == 3 ? n[2] : 1;
+    const size_t nc = (size_t)n[0] * n[1] * n2;
+    const size_t st[3] = {1, (size_t)n[0], (size_t)n[0] * n[1]};
+    for (int d = 0; d < D; d++) {
+        if (bc[d] != 0) continue;
+        const int nd = n[d];
+        for (int f = 0; f < 2 + D; f++) {
+            const double *b = bg + (size_t)f * nc;
+            for (size_t c = 0; c < nc; c++) {
+                const int x = (int)((c / st[d]) % (size_t)nd);
+                int img = -1;
+                if (x < 2) img = nd - 4 + x;
+                else if (x >= nd - 2) img = x - (nd - 4);
+                if (img < 0) continue;
+                const size_t ci = c + ((long long)img - x) * (long long)st[d];
+                if (memcmp(&b[c], &b[ci], sizeof(double)))
+                    return fail(SMAUG_EINVAL,
+                                "dim %d is periodic but background field %d is not: ghost cell "
+                                "%d differs from its image %d (the ghost fill copies only the "
+                                "perturbation)", d, f, x, img);
+            }
+        }
+    }
+    return 0;
+}
+
 void smaug::axis_arrays(const smaug_cfg *cfg, int d, int nd, std::vector<double> &dx,
                         std::vector<double> &idx, std::vector<double> &prof,
                         std::vector<double> &profq)
@@ -815,6 +843,10 @@ static int bg_uniform(int nbg, idx_t nc, const double *bg)
 extern "C" int smaug_set_state(smaug_engine *e, const double *w, const double *bg)
 {
     if (!e || !w || !bg) return fail(SMAUG_EINVAL, "null argument");
+    {
+        const int rc = check_periodic_bg(e->D, e->n, e->cfg.bc, bg);
+        if (rc) return rc;
+    }
     e->aux0_ok = 0;
     int nve = 2 + 2 * e->D, nbg = 2 + e->D;
     {

@@ -609,6 +609,9 @@ static int ens_set_state(smaug_ensemble *e, int r, const double *w, const double
 extern "C" int smaug_ens_set_state(smaug_ensemble *e, int run, const double *w, const double *bg)
 {
     if (!e || !w || !bg || run < 0 || run >= e->nrun) return fail(SMAUG_EINVAL, "bad argument");
+    const smaug_cfg &c = e->cfg[run];
+    const int rc = check_periodic_bg(e->D, c.n, c.bc, bg);
+    if (rc) return rc;
     return EDISPATCH(e, ens_set_state, e, run, w, bg);
 }
 

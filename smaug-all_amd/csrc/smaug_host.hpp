@@ -17,6 +17,13 @@ int fail(int code, const char *fmt, ...);
 // other grids (dimensionality, sizes, boundaries, metric pointers)
 int check_cfg(const smaug_cfg *cfg);
 
+// along every periodic dim (boundtype 0) the background of each ghost cell
+// must equal that of its periodic image: the ghost fill copies only the
+// perturbation, so a background that is not periodic would join, say, the
+// dense bottom's perturbation to the tenuous top's background (a negative
+// full density there).  Returns SMAUG_EINVAL with a message otherwise.
+int check_periodic_bg(int D, const int n[3], const int bc[3], const double *bg);
+
 // dx, 1/dx and the separable driver profile along dim d (length n[d]; unit
 // arrays along the unused third dim of a 2-D grid).  The driver profile is
 // exp(-(z-z0)^2/sz^2) along dim 0 and exp(-(x-xc)^2/sh^2) along the

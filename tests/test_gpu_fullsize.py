@@ -57,3 +57,41 @@ def test_fullsize_zstream_matches_per_cell(full, prec):
     err = rel_err(out["zstream"][1], out["per-cell"][1])
     print("rk4 rel err", err, flush=True)
     assert np.all(err < TOL_RK4[prec]), err
+
+
+@pytest.mark.timeout(900)
+def test_large_grid_zstream_matches_per_cell_mixed():
+    """the large stratified atmosphere at its own size, 768 x 512 x 512
+    interior cells over 12 x 8 x 8 Mm (201 M cells, 62 GB per engine in
+    mixed precision), with the launch shape its timed runs use (update and
+    sources at z-stream depth 64, ratios at 8): two mixed RK4 steps of the
+    z-streamed kernels against the per-cell kernels"""
+    s = configs.stratified(nz=768, nx=512, ny=512, lz=12.0e6, lx=8.0e6, ly=8.0e6)
+    print(f"[case] large grid {s.n} mixed, z-streamed vs per-cell", flush=True)
+    rng = np.random.default_rng(33)
+    w = s.w.copy()
+    rho = s.bg[1]
+    # small perturbations everywhere, float32 noise scaled per cell
+    w[0] += 1e-3 * rho * rng.standard_normal(s.ncell, dtype=np.float32)
+    w[4] += 1e-3 * s.bg[0] * rng.standard_normal(s.ncell, dtype=np.float32)
+    w = w.astype(np.float32).astype(np.float64)
+    m = interior_mask(s)
+    out = {}
+    for form in ("zstream", "per-cell"):
+        e = Engine(s, MIXED)
+        if form == "per-cell":
+            e.set_zstream(False)
+            e.set_ratio_form(False)
+            e.set_bg_plane(False)
+        else:
+            e.set_zstream(True, 64)
+            e.set_ratio_form(True, 8)
+        e.set_state(w)
+        e.rk4(1.0, s.dt, 2)
+        out[form] = e.get_state()
+        e.close()
+        print(f"[case] large grid {form} done", flush=True)
+    assert np.all(np.isfinite(out["zstream"]))
+    err = rel_err(out["zstream"], out["per-cell"], m)
+    print("large grid rk4 rel err", err, flush=True)
+    assert np.all(err < TOL_RK4[MIXED]), err

@@ -11,7 +11,7 @@ from helpers import interior_mask, perturbed, rel_err
 pytestmark = pytest.mark.gpu
 
 from smaugamd import configs  # noqa: E402
-from smaugamd.engine import FP64, MIXED, Engine  # noqa: E402
+from smaugamd.engine import FP64, MIXED, Engine, SmaugError  # noqa: E402
 from oracle.oracle import Oracle  # noqa: E402
 
 # the two update kernels differ only in the summation order of the twelve
@@ -150,14 +150,27 @@ def test_bg_plane_off_for_k_varying_background():
     assert np.all(err < TOL_ORACLE_FP64), err
 
 
-@pytest.mark.parametrize("bc", [(4, 0, 0), (4, 4, 0), (4, 0, 4)], ids=["cpp", "ccp", "cpc"])
-def test_folded_ghost_fill_periodic_vs_oracle(bc):
+def _uniform_bg(s):
+    """the set-up with its background replaced by the values of one interior
+    cell everywhere and no gravity: a background that is periodic along
+    every dim"""
+    bg = np.repeat(s.bg[:, [s.n[0] * 3 + 2]], s.ncell, axis=1)
+    return dataclasses.replace(s, bg=bg, g=(0.0, 0.0, 0.0))
+
+
+@pytest.mark.parametrize("bc,uniform", [((4, 0, 0), False), ((4, 4, 0), False),
+                                        ((4, 0, 4), False), ((0, 0, 0), True),
+                                        ((0, 4, 4), True)],
+                         ids=["cpp", "ccp", "cpc", "ppp_uniform", "pcc_uniform"])
+def test_folded_ghost_fill_periodic_vs_oracle(bc, uniform):
     # the k-plane background path fills the ghosts of RK stage outputs in the
     # ghost-cell share of the folded aux pass (k_ghost_prep); with periodic
-    # horizontal faces the images wrap around
-    # (the vertical axis stays contcd4: a periodic vertical axis would join
-    # the dense bottom of the atmosphere to its top)
+    # faces the images wrap around.  A periodic vertical axis needs a
+    # background that is periodic along it (here uniform): the stratified
+    # one is rejected (test_periodic_axis_needs_periodic_background)
     s = dataclasses.replace(strat(20, 12, 16), bc=bc)
+    if uniform:
+        s = _uniform_bg(s)
     print(f"[case] folded ghost fill bc={bc} n={s.n}", flush=True)
     o = Oracle(s)
     w0 = o.boundary(perturbed(s, amp=1e-3, seed=14))
@@ -172,6 +185,18 @@ def test_folded_ghost_fill_periodic_vs_oracle(bc):
     # every cell, ghosts included
     err = rel_err(got, ref)
     assert np.all(err < TOL_ORACLE_FP64), err
+
+
+def test_periodic_axis_needs_periodic_background():
+    # a periodic vertical axis over the stratified background would give the
+    # top ghost cells the bottom's density perturbation on the corona's
+    # background (a negative full density): set_state rejects it
+    s = dataclasses.replace(strat(20, 12, 16), bc=(0, 4, 4))
+    print(f"[case] periodic vertical axis, stratified background n={s.n}", flush=True)
+    e = Engine(s, FP64)
+    with pytest.raises(SmaugError, match="dim 0 is periodic but background field"):
+        e.set_state(s.w, s.bg)
+    e.close()
 
 
 @pytest.mark.parametrize("prec", [FP64, MIXED], ids=["fp64", "mixed"])
@@ -199,3 +224,50 @@ def test_steps_split_across_calls_match(prec):
     got = e.get_state()
     e.close()
     assert np.array_equal(got, ref), rel_err(got, ref)
+
+
+# the launch shapes the timed runs use: update and source z-stream depths 64
+# and 128, ratio depths 8 and 16 (the autotuner's picks at 256^3 and
+# 768x512x512), on a thin grid deep enough along k for several chunks
+THIN = strat(12, 12, 260)
+
+
+@pytest.mark.parametrize("kz,kzr", [(64, 8), (128, 16), (128, 8), (64, 16)],
+                         ids=["kz64_r8", "kz128_r16", "kz128_r8", "kz64_r16"])
+def test_timed_shapes_rk4_fp64_vs_oracle(kz, kzr):
+    print(f"[case] timed shape kz={kz} kzr={kzr} n={THIN.n}", flush=True)
+    o = Oracle(THIN)
+    w0 = o.boundary(perturbed(THIN, amp=1e-3, seed=17))
+    dt = 0.5 * configs.hyperviscous_dt(THIN)
+    ref, _ = o.run(w0, 0.25, dt, 3, rk4=True)
+    e = Engine(THIN, FP64)
+    e.set_zstream(True, kz)
+    e.set_ratio_form(True, kzr)
+    assert e.get_zstream() == (1, kz) and e.get_ratio_form() == (1, kzr)
+    e.set_state(w0)
+    e.rk4(0.25, dt, 3)
+    got = e.get_state()
+    e.close()
+    err = rel_err(got, ref)
+    print(f"kz={kz} kzr={kzr} rk4 rel err vs oracle", err)
+    assert np.all(err < TOL_ORACLE_FP64), err
+
+
+@pytest.mark.parametrize("kz,kzr", [(64, 8), (128, 8)], ids=["kz64_r8", "kz128_r8"])
+def test_timed_shapes_mixed_matches_per_cell(kz, kzr):
+    print(f"[case] timed shape mixed kz={kz} kzr={kzr} n={THIN.n}", flush=True)
+    o = Oracle(THIN)
+    w = o.boundary(perturbed(THIN, amp=1e-2, seed=18))
+    w = o.boundary(w.astype(np.float32).astype(np.float64))
+    e = Engine(THIN, MIXED)
+    e.set_state(w)
+    e.set_zstream(False)
+    e.set_ratio_form(False)
+    ref = e.rhs(t=2.0)
+    e.set_zstream(True, kz)
+    e.set_ratio_form(True, kzr)
+    got = e.rhs(t=2.0)
+    e.close()
+    err = rel_err(got, ref, interior_mask(THIN))
+    print(f"kz={kz} kzr={kzr} mixed rhs rel err vs per-cell", err)
+    assert np.all(err < TOL_RHS[MIXED]), err
