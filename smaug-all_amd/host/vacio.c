@@ -14,10 +14,14 @@
  *   line 5  variable names
  *   then one row per cell, i fastest, then j, then k:
  *   x y [z] followed by the NVAR fields (rho m.. e b.. eb rhob bb..)
- * Binary layout (writevacconfig): raw fwrite of name[79], it (int32),
+ * Binary layout (writevacconfig, the reference's single-domain writer):
+ * the file is <name>_<it>.out; raw fwrite of name[79], it (int32),
  * t (double), ndim neqpar nw (int32 x3), dims (int32 x ndim), eqpar
- * (double x neqpar), names[79], then each of the ndim position arrays and
- * the NVAR field arrays over all cells in the same cell order, then '\n'.
+ * (double x neqpar), names[79] (the reference's fixed string), then each of
+ * the ndim position arrays and the NVAR field arrays over all cells in the
+ * same cell order, then '\n'.  The reference's reader (readwrite.c:587-780)
+ * consumes 80 bytes for the 79-byte name; readbinvacconfig here reads the
+ * 79 that the writer writes.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -145,10 +149,18 @@ int writeasciivacconfig(char *cfgfile, params p, meta md, real *w, real *wd, cha
     return fclose(f) ? 2 : 0;
 }
 
+#ifdef USE_SAC_3D
+static const char *BINNAMES = "x y z rho mx my mz e bx by bz gamma eta g1 g2 g3";
+#else
+static const char *BINNAMES = "x y rho mx my e bx by  gamma eta g1 g2";
+#endif
+
 int writevacconfig(char *name, int n, params p, meta md, real *w, real *wd, state st)
 {
     (void)n;
-    FILE *f = fopen(name, "wb");
+    char fname[4096];
+    snprintf(fname, sizeof fname, "%s_%d.out", name, st.it);
+    FILE *f = fopen(fname, "wb");
     if (!f) return 1;
     char buf[79];
     memset(buf, ' ', sizeof(buf));
@@ -166,8 +178,8 @@ int writevacconfig(char *name, int n, params p, meta md, real *w, real *wd, stat
     fwrite(ib, sizeof(int), 3, f);
     fwrite(p.n, sizeof(int), NDIM, f);
     fwrite(e, sizeof(double), ne, f);
-    memset(buf, ' ', sizeof(buf));
-    memcpy(buf, VARNAMES, strlen(VARNAMES) < 79 ? strlen(VARNAMES) : 79);
+    memset(buf, 0, sizeof(buf));
+    memcpy(buf, BINNAMES, strlen(BINNAMES) < 79 ? strlen(BINNAMES) : 79);
     fwrite(buf, 1, 79, f);
     long nc = ncells(&p);
     for (int d = 0; d < NDIM; d++) fwrite(wd + (long)(pos1 + d) * nc, sizeof(double), nc, f);

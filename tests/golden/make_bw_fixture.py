@@ -1,11 +1,14 @@
 """Writes bw_ini_summary.json from the reference's Brio-Wu initial
-configuration (dev/smaug/configs/zero1_BW_asc.ini, a VAC ascii config file).
+configuration (dev/smaug/configs/zero1_BW_asc.ini, a VAC ascii config file),
+and the file itself, gzipped, as zero1_BW_asc.ini.gz (the I/O tests read it
+through the C reader of the drop-in driver).
 
 Run in the build container only (the reference tree is not present on the
 GPU box).  The summary holds data, not code: the header values, the grid
 shape, the cell-centre spacing, the index of the left/right interface and
 the two distinct state rows (columns h m1 m2 e b1 b2 eb rhob bg1 bg2).
 """
+import gzip
 import json
 import os
 import sys
@@ -45,6 +48,10 @@ def main():
     }
     json.dump(out, open(OUT, "w"), indent=1)
     print("wrote", OUT)
+    gz = os.path.join(os.path.dirname(OUT), "zero1_BW_asc.ini.gz")
+    with open(SRC, "rb") as fi, gzip.GzipFile(gz, "wb", compresslevel=9, mtime=0) as fo:
+        fo.write(fi.read())
+    print("wrote", gz)
 
 
 if __name__ == "__main__":

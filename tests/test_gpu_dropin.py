@@ -119,3 +119,41 @@ def test_driver_torsional_onestep_and_rk4(tmp_path):
         err = rel_err(data[: s.nve], ref)
         print("driver torsional rk4" if rk4 else "driver torsional onestep", "rel err", err)
         assert np.all(err < 1e-11), err
+
+
+def test_driver_onestep_reference_bw_ini(tmp_path):
+    """the reference's own Brio-Wu initial state (configs/zero1_BW_asc.ini,
+    800 x 6 cells, committed gzipped) stepped by the drop-in driver against
+    the oracle.  Its y axis (two interior cells) is widened to the 8 cells per
+    axis the path needs by repeating its rows: the state is the same in every
+    row, and at 6 cells a contcd4 ghost's image (n-5 = 1) would itself be a
+    ghost cell."""
+    import gzip
+    import shutil
+    ini = str(tmp_path / "zero1_BW_asc.ini")
+    with gzip.open(os.path.join(ROOT, "tests", "golden", "zero1_BW_asc.ini.gz"), "rb") as fi, \
+            open(ini, "wb") as fo:
+        shutil.copyfileobj(fi, fo)
+    hdr, pos, data = vacio.read_ascii(ini)
+    n0, n1 = hdr["dims"]
+    row = data[:, :n0]
+    assert np.array_equal(data, np.tile(row, (1, n1)))  # the same in every row
+    ny = 8
+    s = configs.brio_wu(ncells=n0 - 4, ny=ny - 4)
+    s.gamma = hdr["eqpar"][0]
+    s.w = np.tile(row[:6], (1, ny))
+    s.bg = np.tile(row[6:], (1, ny))
+    x = pos[0][:n0]
+    s.pos[0] = x
+    s.dx[0] = np.full(n0, x[1] - x[0])
+    inp = roundtrip_state(s, tmp_path, "bw_ref")
+    out = str(tmp_path / "bw_ref_out.ini")
+    dt, n = 1e-4, 10
+    run_driver(2, ["-i", inp, "-o", out, "-n", str(n), "-dt", repr(dt), "-bc", "4,4"], tmp_path)
+    _, _, res = vacio.read_ascii(out)
+    ref, _ = Oracle(s).run(s.w, dt, dt, n, rk4=False)
+    err = rel_err(res[: s.nve], ref)
+    print("driver onestep on the reference's Brio-Wu file: rel err", err)
+    assert np.all(err < 1e-11), err
+    # the waves have left the interface
+    assert np.abs(ref[0] - s.w[0]).max() > 1e-3
