@@ -11,8 +11,10 @@ fill), replayed from a captured hipGraph; inputs are resident in HBM.
 Printed JSON line (rank 0): the contract fields plus
   fp64        : the plain FP64 GPU path on the same workload and steps
   accuracy    : per-field relative L2 / Linf of the mixed result against the
-                FP64 GPU result after the timed steps (normalised by each
-                field's amplitude)
+                FP64 GPU result after the workload's 1000 RK4 steps (a leg
+                of its own, whatever --steps says), the FP64 path's change
+                under a 1e-9 change of dt, and the error floor that implies
+                for any FP32-stored state
   roofline    : the longest kernel of a stage against HBM peak, with the
                 mean launch time and profiled HBM traffic of every pass
   cpu_baseline: the CPU oracle (OpenMP) on a bounded sample of the workload
@@ -69,6 +71,10 @@ def parse():
                          "smaug_set_zstream: 0 per-cell, 1 z-streamed, 3 z-streamed with "
                          "LDS-plane sources), e.g. for profiling runs")
     ap.add_argument("--ens-steps", type=int, default=200)
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="RK4 steps of the mixed-vs-FP64 accuracy leg (the stratified "
+                         "workload's horizon)")
+    ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--ot-steps", type=int, default=500,
                     help="RK4 steps of the Orszag-Tang leg (500 is the workload's count)")
     return ap.parse_args()
@@ -108,6 +114,58 @@ def run_path(setup, precision, warmup, steps, dt_scale=1.0, tune=True, shape=Non
     w = e.get_state()
     e.close()
     return tot_ms, stage_ms, w, info
+
+
+def _field_errors(a, b, m, names):
+    import numpy as np
+
+    l2, linf = {}, {}
+    for f, name in enumerate(names):
+        ref = b[f][m]
+        d = a[f][m] - ref
+        amp = np.abs(ref).max()
+        rms = np.sqrt(np.mean(ref ** 2))
+        l2[name] = float(np.sqrt(np.mean(d ** 2)) / rms) if rms > 0 else 0.0
+        linf[name] = float(np.abs(d).max() / amp) if amp > 0 else 0.0
+    return l2, linf
+
+
+def accuracy_leg(setup, nsteps):
+    """the mixed path against the FP64 path after nsteps RK4 steps from the
+    workload's initial state (the stratified workload's horizon is 1000
+    steps, whatever --steps says), and the FP64 path against itself with dt
+    changed by 1e-9: the workload's own conditioning.  The change of the FP64
+    result per unit relative change of dt, times FP32's unit roundoff 2^-24,
+    is the error floor of any FP32-stored state on this workload."""
+    import numpy as np
+
+    from smaugamd.engine import FP64, MIXED, Engine
+
+    names = ["rho", "m1", "m2", "m3", "e", "b1", "b2", "b3"]
+    n0, n1, n2 = setup.n
+    m = np.zeros((n2, n1, n0), dtype=bool)  # interior cells
+    m[2:-2, 2:-2, 2:-2] = True
+    m = m.reshape(-1)
+    out = {}
+    for key, prec, scale in (("mixed", MIXED, 1.0), ("fp64", FP64, 1.0),
+                             ("fp64_dt", FP64, 1.0 + 1e-9)):
+        e = Engine(setup, prec)
+        e.set_state()
+        e.rk4(0.0, setup.dt * scale, nsteps)
+        out[key] = e.get_state()
+        e.close()
+    l2, linf = _field_errors(out["mixed"], out["fp64"], m, names)
+    sens, _ = _field_errors(out["fp64_dt"], out["fp64"], m, names)
+    u32 = 2.0 ** -24
+    floor = {k: v / 1e-9 * u32 for k, v in sens.items()}
+    return {"reference": "FP64 GPU path (parity-tested against the FP64 oracle)",
+            "steps": nsteps, "rel_l2": l2, "rel_linf": linf,
+            "target_rel_l2": 1e-5,
+            "meets_target": bool(all(v <= 1e-5 for v in l2.values())),
+            "fp64_rel_l2_under_1e-9_dt_perturbation": sens,
+            "fp32_storage_floor_rel_l2": floor,
+            "note": "fp32_storage_floor = (FP64 change per unit relative dt change) x 2^-24: "
+                    "the error a state stored in FP32 cannot go below on this workload"}
 
 
 def pass_kernels(form, bg_plane, fp32=True):
@@ -185,14 +243,26 @@ def cpu_baseline(n):
     t = time.perf_counter()
     o.run(w0, 0.0, s.dt, steps)
     el = time.perf_counter() - t
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {
         "value": s.ninterior * steps / el / 1e6,
         "unit": "Mcell-updates/s",
         "cores": Oracle.threads(),
+        "cpu_model": model,
+        "cpus_available": len(os.sched_getaffinity(0)),
+        "cpus_online": os.cpu_count(),
         "kind": "port",
         "sample": f"{steps} RK4 steps of the stratified workload on {n}x{nh}x{nh} interior "
                   f"cells (full vertical column, {nh}x{nh} horizontal), FP64, "
-                  f"{el:.1f} s wall",
+                  f"{el:.1f} s wall, {Oracle.threads()} OpenMP threads",
     }
 
 
@@ -330,30 +400,9 @@ def main():
             "passes_ms": info64["passes_ms"],
             "speedup_mixed_over_fp64": (tot64 / tot),
         }
-        n0, n1, n2 = setup.n
-        m = np.zeros((n2, n1, n0), dtype=bool)  # interior cells
-        m[2:-2, 2:-2, 2:-2] = True
-        m = m.reshape(-1)
-        names = ["rho", "m1", "m2", "m3", "e", "b1", "b2", "b3"]
-        l2, linf = {}, {}
-        for f in range(setup.nve):
-            ref = wd[f][m]
-            d = wm[f][m] - ref
-            amp = np.abs(ref).max()
-            rms = np.sqrt(np.mean(ref ** 2))
-            l2[names[f]] = float(np.sqrt(np.mean(d ** 2)) / rms) if rms > 0 else 0.0
-            linf[names[f]] = float(np.abs(d).max() / amp) if amp > 0 else 0.0
-        # conditioning of the workload itself: the FP64 path against the FP64
-        # path run with dt*(1+1e-9)
-        _, _, wd2, _ = run_path(setup, FP64, a.warmup, a.steps, dt_scale=1 + 1e-9, tune=False)
-        sens = {}
-        for f in range(setup.nve):
-            ref = wd[f][m]
-            rms = np.sqrt(np.mean(ref ** 2))
-            sens[names[f]] = float(np.sqrt(np.mean((wd2[f][m] - ref) ** 2)) / rms) if rms > 0 else 0.0
-        acc = {"reference": "FP64 GPU path (parity-tested against the FP64 oracle)",
-               "steps": a.warmup + a.steps + 2, "rel_l2": l2, "rel_linf": linf,
-               "fp64_rel_l2_under_1e-9_dt_perturbation": sens}
+    if not a.no_accuracy:
+        acc = accuracy_leg(setup, a.accuracy_steps)
+        print(f"[bench] accuracy leg: {acc['steps']} steps", file=sys.stderr, flush=True)
 
     # the roofline is given for the dominant kernel of the step: of the
     # passes that run one kernel in every stage, the one with the longest

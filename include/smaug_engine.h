@@ -151,8 +151,9 @@ int smaug_sums_noalloc(int ndim, const int n[3], int precision, const void *d_w,
  * passes 3 and 4 carry that work. */
 int smaug_time_passes(smaug_engine *e, double t0, double dt, double ms[5]);
 
-/* measured copy-bandwidth ceiling: GB/s (read + write bytes) of a plain
- * streaming copy of nbytes, mean of reps launches */
+/* measured copy-bandwidth ceiling: GB/s (read + write bytes) of a streaming
+ * copy of nbytes, mean of reps launches, the best over 1, 4 or 8 16-byte
+ * loads in flight per thread and 8, 16 or 32 workgroups per CU */
 int smaug_copy_probe(long long nbytes, int reps, double *gbs);
 
 /* sizes */
