@@ -12,9 +12,8 @@ Printed JSON line (rank 0): the contract fields plus
   fp64        : the plain FP64 GPU path on the same workload and steps
   accuracy    : per-field relative L2 / Linf of the mixed result against the
                 FP64 GPU result after the workload's 1000 RK4 steps (a leg
-                of its own, whatever --steps says), the FP64 path's change
-                under a 1e-9 change of dt, and the error floor that implies
-                for any FP32-stored state
+                of its own, whatever --steps says), and the FP64 path's
+                change under a 1e-9 change of dt
   roofline    : the longest kernel of a stage against HBM peak, with the
                 mean launch time and profiled HBM traffic of every pass
   cpu_baseline: the CPU oracle (OpenMP) on a bounded sample of the workload
@@ -134,9 +133,10 @@ def accuracy_leg(setup, nsteps):
     """the mixed path against the FP64 path after nsteps RK4 steps from the
     workload's initial state (the stratified workload's horizon is 1000
     steps, whatever --steps says), and the FP64 path against itself with dt
-    changed by 1e-9: the workload's own conditioning.  The change of the FP64
-    result per unit relative change of dt, times FP32's unit roundoff 2^-24,
-    is the error floor of any FP32-stored state on this workload."""
+    changed by 1e-9: how far a smooth perturbation of that size carries on
+    this workload.  (Rounding a stored state to FP32 perturbs it at the grid
+    scale, which the hyperviscosity switch max|D3|/max|D1| responds to far
+    more strongly, so the second figure is no bound for the first.)"""
     import numpy as np
 
     from smaugamd.engine import FP64, MIXED, Engine
@@ -156,16 +156,11 @@ def accuracy_leg(setup, nsteps):
         e.close()
     l2, linf = _field_errors(out["mixed"], out["fp64"], m, names)
     sens, _ = _field_errors(out["fp64_dt"], out["fp64"], m, names)
-    u32 = 2.0 ** -24
-    floor = {k: v / 1e-9 * u32 for k, v in sens.items()}
     return {"reference": "FP64 GPU path (parity-tested against the FP64 oracle)",
             "steps": nsteps, "rel_l2": l2, "rel_linf": linf,
             "target_rel_l2": 1e-5,
             "meets_target": bool(all(v <= 1e-5 for v in l2.values())),
-            "fp64_rel_l2_under_1e-9_dt_perturbation": sens,
-            "fp32_storage_floor_rel_l2": floor,
-            "note": "fp32_storage_floor = (FP64 change per unit relative dt change) x 2^-24: "
-                    "the error a state stored in FP32 cannot go below on this workload"}
+            "fp64_rel_l2_under_1e-9_dt_perturbation": sens}
 
 
 def pass_kernels(form, bg_plane, fp32=True):
