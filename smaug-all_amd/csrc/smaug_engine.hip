@@ -624,11 +624,17 @@ static int euler_step(smaug_engine *e)
 }
 
 // dispatch over (D, precision)
-#define DISPATCH(e, FN, ...)                                                               \
+// DISPATCH_KEEP runs the instance of FN for the engine's dimensionality and
+// precision.  DISPATCH does the same after clearing aux0_ok: every entry
+// point that reaches the state goes through it, so a new one invalidates the
+// aux quantities of U0 by default; smaug_rk4 alone keeps them, as its
+// folded steps leave them valid for the next call.
+#define DISPATCH_KEEP(e, FN, ...)                                                          \
     ((e)->D == 2 ? ((e)->prec == SMAUG_FP64 ? FN<2, double>(__VA_ARGS__)                   \
                                             : FN<2, float>(__VA_ARGS__))                   \
                  : ((e)->prec == SMAUG_FP64 ? FN<3, double>(__VA_ARGS__)                   \
                                             : FN<3, float>(__VA_ARGS__)))
+#define DISPATCH(e, FN, ...) ((e)->aux0_ok = 0, DISPATCH_KEEP(e, FN, __VA_ARGS__))
 
 // ---------------------------------------------------------------------------
 // C-ABI
@@ -847,7 +853,7 @@ extern "C" int smaug_set_state(smaug_engine *e, const double *w, const double *b
         const int rc = check_periodic_bg(e->D, e->n, e->cfg.bc, bg);
         if (rc) return rc;
     }
-    e->aux0_ok = 0;
+    e->aux0_ok = 0; // U0 is written below, ahead of the dispatched ghost fill
     int nve = 2 + 2 * e->D, nbg = 2 + e->D;
     {
         const int ok = bg_uniform(nbg, e->nc, bg);
@@ -934,7 +940,6 @@ static int do_rhs(smaug_engine *e, double t, double *rhs)
 extern "C" int smaug_rhs(smaug_engine *e, double t, double *rhs)
 {
     if (!e || !rhs) return fail(SMAUG_EINVAL, "null argument");
-    e->aux0_ok = 0;
     return DISPATCH(e, do_rhs, e, t, rhs);
 }
 
@@ -957,7 +962,6 @@ static int do_cmax(smaug_engine *e, double *cm)
 extern "C" int smaug_cmax(smaug_engine *e, double cmax[3])
 {
     if (!e || !cmax) return fail(SMAUG_EINVAL, "null argument");
-    e->aux0_ok = 0;
     return DISPATCH(e, do_cmax, e, cmax);
 }
 
@@ -986,7 +990,6 @@ static int do_courant(smaug_engine *e, double *out)
 extern "C" int smaug_maxcourant(smaug_engine *e, double *out)
 {
     if (!e || !out) return fail(SMAUG_EINVAL, "null argument");
-    e->aux0_ok = 0;
     return DISPATCH(e, do_courant, e, out);
 }
 
@@ -1032,7 +1035,7 @@ static int run_steps(smaug_engine *e, int which, double t0, double dt, int nstep
 extern "C" int smaug_rk4(smaug_engine *e, double t0, double dt, int nsteps)
 {
     if (!e || nsteps < 0) return fail(SMAUG_EINVAL, "bad argument");
-    return DISPATCH(e, run_steps, e, 0, t0, dt, nsteps);
+    return DISPATCH_KEEP(e, run_steps, e, 0, t0, dt, nsteps);
 }
 
 extern "C" int smaug_euler(smaug_engine *e, double t0, double dt, int nsteps)
@@ -1206,7 +1209,6 @@ extern "C" int smaug_time_pass(smaug_engine *e, int pass, int reps, double *ms)
 {
     if (!e || pass < 0 || pass > 1 || reps < 1 || !ms) return fail(SMAUG_EINVAL, "bad argument");
     if (pass == 0 && !e->phys.hyperdif) return fail(SMAUG_EINVAL, "no ratio pass without hyperdiffusion");
-    e->aux0_ok = 0;
     return DISPATCH(e, time_pass, e, pass, reps, ms);
 }
 
@@ -1218,7 +1220,6 @@ extern "C" int smaug_time_pass(smaug_engine *e, int pass, int reps, double *ms)
 extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double best_ms[2])
 {
     if (!e || reps < 1) return fail(SMAUG_EINVAL, "bad argument");
-    e->aux0_ok = 0;
     static const int TX[] = {1, 2, 4, 8}, TY[] = {1, 2, 4, 8, 16, 32};
     const int zs0 = e->zstream, rz0 = e->ratio_zs;
     e->zstream = 0;
