@@ -51,7 +51,11 @@ int cufinish(struct params **p, real **w, real **wnew, struct state **state,
              struct params **d_p, struct bparams **d_bp, real **d_w, real **d_wnew,
              real **d_wmod, real **d_dwn1, real **d_wd, struct state **d_state, real **d_wtemp,
              real **d_wtemp1, real **d_wtemp2);
-/* smaugcukernels.h:14 */
+/* smaugcukernels.h:14.  Ghost cells of one field along idir by its boundtype:
+ * 0 periodic, 3 cont (both layers copy the face's edge cell), 4 contcd4,
+ * 5 fixed.  A fixed face keeps the values that the call with p->it == -1
+ * recorded (the driver makes that call once per fixed face before the time
+ * loop, iosmaug.cpp:645-659); using one before it is recorded is an error. */
 int cuboundary(struct params **p, struct bparams **bp, struct params **d_p,
                struct bparams **d_bp, struct state **d_s, real **d_wmod, int order, int idir,
                int field);

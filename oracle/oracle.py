@@ -28,6 +28,7 @@ class SmGrid(ctypes.Structure):
         ("drv", ctypes.c_double * 8),
         ("dx", ctypes.POINTER(ctypes.c_double) * 3),
         ("pos", ctypes.POINTER(ctypes.c_double) * 3),
+        ("wfix", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -104,6 +105,15 @@ class Oracle:
             g.dx[d] = _dp(dx)
             g.pos[d] = _dp(pos)
         self.g = g
+        if getattr(s, "wfix", None) is not None:
+            self.set_fixed(s.wfix)
+
+    def set_fixed(self, w):
+        """the state whose ghost values the fixed (bc 5) faces keep"""
+        wf = np.ascontiguousarray(w, dtype=np.float64).copy()
+        assert wf.shape == (self.s.nve, self.s.ncell)
+        self._wfix = wf
+        self.g.wfix = _dp(wf)
 
     def _w(self, w):
         w = np.ascontiguousarray(w, dtype=np.float64)

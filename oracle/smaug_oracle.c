@@ -228,7 +228,11 @@ static void g13(const geo *G, const sm_grid *g, const double *q, int d, double *
 /* ------------------------------------------------------------------------
  * Ghost fill.  contcd4 (boundary.cu:16-88): 0 <- 4, 1 <- 3, n-1 <- n-5,
  * n-2 <- n-4.  periodic (gradops.cu:1271-1317, models/boundary.cu.ot):
- * 0,1 <- n-4,n-3 and n-2,n-1 <- 2,3.  Each direction is filled over the
+ * 0,1 <- n-4,n-3 and n-2,n-1 <- 2,3.  cont (SAC vacgrid.t "cont"): both
+ * ghost layers of a face copy the face's edge cell, 0,1 <- 2 and
+ * n-2,n-1 <- n-3.  fixed: the ghosts keep the values of g->wfix, the state
+ * captured when the fixed faces were initialised (bc3_setfixed_dir /
+ * bc3_fixed_dir, gradops.cu:1201-1262).  Each direction is filled over the
  * whole transverse range, directions in order, as the driver loop does
  * (iosmaug.cpp:1157-1166).
  * -------------------------------------------------------------------- */
@@ -241,15 +245,24 @@ void sm_boundary(const sm_grid *g, double *w)
         for (int d = 0; d < G.D; d++) {
             long s = G.s[d];
             int n = G.n[d];
-            int per = (g->bc[d] == 0);
+            int bt = g->bc[d];
+            const double *qf = g->wfix ? g->wfix + (long)f * G.nc : NULL;
             FORALL(G)
             {
                 long c = CIDX(G);
                 int x = coord(d, i_, j_, k_);
                 int src = -1;
-                if (per) {
+                int ghost = x < 2 || x > n - 3;
+                if (bt == 5) {
+                    if (ghost && qf) q[c] = qf[c];
+                    continue;
+                }
+                if (bt == 0) {
                     if (x == 0 || x == 1) src = n - 4 + x;
                     else if (x == n - 1 || x == n - 2) src = 4 - n + x;
+                } else if (bt == 3) {
+                    if (x == 0 || x == 1) src = 2;
+                    else if (x == n - 1 || x == n - 2) src = n - 3;
                 } else {
                     if (x == 0 || x == 1) src = 4 - x;
                     else if (x == n - 1) src = x - 4;

@@ -22,7 +22,8 @@ extern "C" {
 typedef struct {
     int ndim;            /* 2 or 3 */
     int n[3];            /* cells per dim incl. ghosts; n[2] == 1 when ndim == 2 */
-    int bc[3];           /* per dim: 0 periodic, 4 contcd4 mirror (iosmaugparams boundtype) */
+    int bc[3];           /* per dim: 0 periodic, 3 cont, 4 contcd4 mirror, 5 fixed
+                            (iosmaugparams boundtype) */
     int hyperdif;        /* p->hyperdifmom */
     int driver;          /* 0 none, 1 vertical Gaussian velocity driver,
                             2 torsional driver of usersource.cu (3-D) */
@@ -33,6 +34,8 @@ typedef struct {
                             and for driver 2 the normalising length L */
     const double *dx[3]; /* cell metric per dim, length n[d]  (wd[delx1..3]) */
     const double *pos[3];/* cell centre per dim, length n[d]  (wd[pos1..3])  */
+    const double *wfix;  /* bc 5: the evolved state whose ghost values the fixed
+                            faces keep (NVE x ncell); NULL otherwise */
 } sm_grid;
 
 long sm_ncell(const sm_grid *g);

@@ -67,6 +67,10 @@ struct Ctx {
     double *d_profq[3] = {nullptr, nullptr, nullptr};
     unsigned long long *d_red = nullptr;
     smaug_engine *eng[2] = {nullptr, nullptr};
+    // fixed faces (boundtype 5): the values captured by the cuboundary call
+    // with p->it == -1, NVAR fields of dimp cells; fixed_set per field
+    double *fixed = nullptr;
+    bool fixed_set[NVAR] = {};
     int driver = 0;
     double drv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
@@ -109,9 +113,10 @@ static int check_params(const struct params *p)
     if (!g_ctx.ready) return cfail(SMAUG_EINVAL, "cuinit has not been called");
     for (int d = 0; d < D; d++) {
         int bt = p->boundtype[0][d][0];
-        if (bt != 0 && bt != 4)
-            return cfail(SMAUG_EINVAL, "boundtype %d along dim %d unsupported (0 periodic, 4 contcd4)",
-                         bt, d);
+        if (bt != 0 && bt != 3 && bt != 4 && bt != 5)
+            return cfail(SMAUG_EINVAL,
+                         "boundtype %d along dim %d unsupported (0 periodic, 3 cont, 4 contcd4, "
+                         "5 fixed)", bt, d);
         for (int f = 0; f < NVAR; f++)
             for (int b = 0; b < 2; b++)
                 if (p->boundtype[f][d][b] != bt)
@@ -171,14 +176,24 @@ __global__ void kc_copy(double *__restrict__ dst, const double *__restrict__ src
 // ghost fill of one field along one direction over the whole transverse
 // range (boundary0/1/2_contcd4_parallel, boundary.cu:16-88; periodic as in
 // gradops.cu:1271-1317)
-__global__ void kc_bc(Geo g, double *__restrict__ w, int dir, int per)
+// ghost cells of one field along dir by boundtype bt: 0 periodic, 3 cont
+// (both layers copy the face's edge cell, SAC vacgrid.t "cont"), 4 contcd4,
+// 5 fixed (the values wf captured when the faces were initialised)
+__global__ void kc_bc(Geo g, double *__restrict__ w, int dir, int bt,
+                      const double *__restrict__ wf)
 {
     CELL_ALL();
     const int x = xs[dir], n = g.n[dir];
     int src;
     if (x >= 2 && x <= n - 3) return;
-    if (per)
+    if (bt == 5) {
+        w[c] = wf[c];
+        return;
+    }
+    if (bt == 0)
         src = x < 2 ? n - 4 + x : 4 - n + x;
+    else if (bt == 3)
+        src = x < 2 ? 2 : n - 3;
     else
         src = x < 2 ? 4 - x : (x == n - 1 ? x - 4 : x - 2);
     w[c] = w[c + (idx_t)(src - x) * stride<D>(g, dir)];
@@ -695,9 +710,30 @@ extern "C" int cuboundary(struct params **p, struct bparams **bp, struct params 
     if (idir < 0 || idir >= D || field < 0 || field >= NVAR || order < 0 || order >= g_ctx.nslots)
         return cfail(SMAUG_EINVAL, "bad arguments");
     Geo g = make_geo(*p);
+    const int bt = (*p)->boundtype[field][idir][0];
+    double *wf = slot(order) + (idx_t)field * g.nc;
+    if (bt == 5) {
+        // the reference's driver calls cuboundary once per fixed face with
+        // p->it == -1 before the time loop (iosmaug.cpp:645-659) to record the
+        // face values (bc3_setfixed_dir, gradops.cu:1201-1230); later calls
+        // restore them (bc3_fixed_dir, 1234-1262)
+        if (!g_ctx.fixed) CHK(hipMalloc((void **)&g_ctx.fixed, sizeof(double) * NVAR * g.nc));
+        double *fx = g_ctx.fixed + (idx_t)field * g.nc;
+        if ((*p)->it == -1) {
+            CHK(hipMemcpy(fx, wf, sizeof(double) * g.nc, hipMemcpyDeviceToDevice));
+            g_ctx.fixed_set[field] = true;
+            return 0;
+        }
+        if (!g_ctx.fixed_set[field])
+            return cfail(SMAUG_EINVAL,
+                         "fixed boundary of field %d used before its values were recorded "
+                         "(cuboundary with p->it == -1 records them)", field);
+        dim3 blk(64, 4, 1);
+        hipLaunchKernelGGL(kc_bc, all_grid(blk), blk, 0, 0, g, wf, idir, bt, (const double *)fx);
+        return sync_check();
+    }
     dim3 blk(64, 4, 1);
-    hipLaunchKernelGGL(kc_bc, all_grid(blk), blk, 0, 0, g, slot(order) + (idx_t)field * g.nc, idir,
-                       g.per[idir]);
+    hipLaunchKernelGGL(kc_bc, all_grid(blk), blk, 0, 0, g, wf, idir, bt, (const double *)nullptr);
     return sync_check();
 }
 
@@ -1349,6 +1385,9 @@ extern "C" int cufinish(struct params **p, real **w, real **wnew, struct state *
     (void)hipFree(g_ctx.ds);
     (void)hipFree(g_ctx.dbp);
     (void)hipFree(g_ctx.d_red);
+    if (g_ctx.fixed) (void)hipFree(g_ctx.fixed);
+    g_ctx.fixed = nullptr;
+    for (int f = 0; f < NVAR; f++) g_ctx.fixed_set[f] = false;
     int drv = g_ctx.driver;
     double dv[8];
     memcpy(dv, g_ctx.drv, sizeof(dv));

@@ -174,6 +174,18 @@ int main(int argc, char **argv)
                 &d_state, &d_wtemp, &d_wtemp1, &d_wtemp2));
     CALL(initgrid(&p, &st, &wd, &d_p, &d_dwn1, &d_wd, &d_state, &d_wtemp, &d_wtemp1, &d_wtemp2));
 
+    /* record the values of the fixed faces (boundtype 5) before the time
+     * loop (iosmaug.cpp:645-659) */
+    {
+        int its = p->it;
+        for (int ii = 0; ii < NVAR; ii++)
+            for (int idir = 0; idir < NDIM; idir++)
+                if (p->boundtype[ii][idir][0] == 5) {
+                    p->it = -1;
+                    CALL(cuboundary(&p, &bp, &d_p, &d_bp, &d_state, &d_wmod, 0, idir, ii));
+                }
+        p->it = its;
+    }
     /* ghost fill of the initial state (iosmaug.cpp:783-791) */
     for (int ii = 0; ii <= (b1 + (NDIM - 1)); ii++)
         for (int idir = 0; idir < NDIM; idir++)

@@ -157,3 +157,58 @@ def test_driver_onestep_reference_bw_ini(tmp_path):
     assert np.all(err < 1e-11), err
     # the waves have left the interface
     assert np.abs(ref[0] - s.w[0]).max() > 1e-3
+
+
+@pytest.mark.parametrize("bc", [(3, 5), (5, 3), (3, 4)], ids=["cont_fixed", "fixed_cont",
+                                                          "cont_contcd4"])
+def test_driver_onestep_cont_and_fixed_2d(tmp_path, bc):
+    """cont (3) and fixed (5) faces through the drop-in driver: the driver
+    records the fixed faces with the p->it == -1 call before the loop
+    (iosmaug.cpp:645-659); the oracle keeps the ghost values of the state as
+    read from the file"""
+    s = configs.brio_wu(ncells=60, ny=12)
+    s.bc = bc
+    s.w = perturbed(s, amp=1e-3, seed=40)
+    inp = roundtrip_state(s, tmp_path, "bw_bc")
+    out = str(tmp_path / "bw_bc_out.ini")
+    dt, n = 5e-4, 6
+    run_driver(2, ["-i", inp, "-o", out, "-n", str(n), "-dt", repr(dt),
+                   "-bc", ",".join(str(b) for b in bc)], tmp_path)
+    _, _, res = vacio.read_ascii(out)
+    o = Oracle(s)
+    o.set_fixed(s.w)
+    ref, _ = o.run(s.w, dt, dt, n, rk4=False)
+    err = rel_err(res[: s.nve], ref)
+    print(f"driver onestep bc={bc} rel err", err)
+    assert np.all(err < 1e-11), err
+    if 5 in bc:
+        # the fixed faces still hold the values read from the file (the
+        # corners of dim 0's faces are refilled by dim 1, filled after it)
+        d = bc.index(5)
+        i = np.tile(np.arange(s.n[0]), s.n[1])
+        j = np.repeat(np.arange(s.n[1]), s.n[0])
+        x, y = (i, j) if d == 0 else (j, i)
+        gh = (x < 2) | (x > s.n[d] - 3)
+        if d == 0:
+            gh &= (y >= 2) & (y <= s.n[1] - 3)
+        assert np.array_equal(res[: s.nve][:, gh], s.w[:, gh])
+
+
+def test_driver_onestep_fixed_vertical_stratified(tmp_path):
+    # the stratified atmosphere with its bottom and top faces fixed
+    s = _strat_small()
+    s.bc = (5, 4, 4)
+    o = Oracle(s)
+    s.w = perturbed(s, amp=1e-3, seed=41)
+    inp = roundtrip_state(s, tmp_path, "strat_fix")
+    out = str(tmp_path / "strat_fix_out.ini")
+    n, t0 = 3, 1.5
+    run_driver(3, ["-i", inp, "-o", out, "-n", str(n), "-dt", repr(s.dt), "-t0", str(t0),
+                   "-bc", "5,4,4", "-driver", _drv_arg(s)], tmp_path)
+    _, _, res = vacio.read_ascii(out)
+    o = Oracle(s)
+    o.set_fixed(s.w)
+    ref, _ = o.run(s.w, t0 + s.dt, s.dt, n, rk4=False)
+    err = rel_err(res[: s.nve], ref)
+    print("driver onestep fixed vertical faces rel err", err)
+    assert np.all(err < 1e-11), err

@@ -695,10 +695,37 @@ def check_numax_and_dtvisc(lib_path=None):
     assert o.dtvisc(0.0) == 0.0
 
 
+def check_cont_and_fixed_ghosts(lib_path=None):
+    """cont (3): both ghost layers of a face copy its edge cell (2, n-3);
+    fixed (5): the ghosts take the captured state's values.  Dim 0 is filled
+    first over the whole transverse range, then dim 1, so the corners follow
+    dim 1's rule."""
+    s = grid2d(nx=10, ny=9, bc=(3, 5))
+    i, j, _ = cell_coords(s)
+    n0, n1 = s.n[0], s.n[1]
+    f = np.sin(0.7 * i) + np.cos(0.3 * j) + 2.0
+    w = np.zeros((s.nve, s.ncell))
+    w[0] = np.where(interior_mask(s), f, 1e9)
+    fix = np.zeros_like(w)
+    fix[0] = 1000.0 + i + 100.0 * j
+    o = Oracle(s, lib_path)
+    o.set_fixed(fix)
+    got = o.boundary(w)[0].reshape(n1, n0)
+    F = f.reshape(n1, n0)
+    rows = slice(2, n1 - 2)
+    assert np.array_equal(got[rows, 0], F[rows, 2]) and np.array_equal(got[rows, 1], F[rows, 2])
+    assert np.array_equal(got[rows, n0 - 1], F[rows, n0 - 3])
+    assert np.array_equal(got[rows, n0 - 2], F[rows, n0 - 3])
+    fx = fix[0].reshape(n1, n0)
+    for jj in (0, 1, n1 - 2, n1 - 1):
+        assert np.array_equal(got[jj], fx[jj])
+    assert np.array_equal(got[2:n1 - 2, 2:n0 - 2], F[2:n1 - 2, 2:n0 - 2])
+
+
 SOURCE_CHECKS = [check_rho_and_energy_sources, check_momentum_diagonal_sources,
                  check_momentum_offdiagonal_sources, check_magnetic_sources,
                  check_hyperdiffusion_dissipates, check_maxcourant_stretched,
-                 check_numax_and_dtvisc]
+                 check_numax_and_dtvisc, check_cont_and_fixed_ghosts]
 
 
 @pytest.mark.parametrize("check", SOURCE_CHECKS, ids=lambda c: c.__name__[6:])

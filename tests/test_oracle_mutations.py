@@ -78,6 +78,12 @@ MUTANTS = [
     ("dtvisc_first_cell_only",
      "for (int x = 0; x < G.n[d]; x++) {\n            double t = 0.25",
      "for (int x = 0; x < 1; x++) {\n            double t = 0.25"),
+    ("cont_copies_second_cell",
+     "if (x == 0 || x == 1) src = 2;",
+     "if (x == 0 || x == 1) src = 3;"),
+    ("fixed_faces_not_restored",
+     "if (ghost && qf) q[c] = qf[c];",
+     "(void)qf;"),
     ("maxcourant_edge_dx",
      "cfast_at(&G, g, w, bg, c, d) / g->dx[d][coord(d, i_, j_, k_)];",
      "cfast_at(&G, g, w, bg, c, d) / g->dx[d][0];"),
