@@ -228,16 +228,18 @@ __device__ __forceinline__ T ratio6(const T q[6])
         // 3 (q[f+1]-q[f]) - (q[f+2]-q[f-1])
         m3 = fmax(fabs(fma((T)3, d[f], -(q[f + 2] - q[f - 1]))), m3);
     }
-    if (!(m1 > (T)0)) return (T)0;
     if constexpr (sizeof(T) == 4) {
         // FP32: the hardware reciprocal (1 ulp) and one multiply, within
-        // 2 ulp of m3/m1.  Below 2^-62 both are scaled by 2^64 first
-        // (exactly, as m3 <= 4 m1), so that the reciprocal's argument and
-        // result are normal numbers; m1 above 2^126 does not arise in data
-        // that the FP32 flux arithmetic can carry.
-        const T sc = m1 < 0x1p-62f ? 0x1p64f : 1.0f;
-        return (m3 * sc) * __builtin_amdgcn_rcpf(m1 * sc);
+        // 2 ulp of m3/m1.  The denominator is clamped to the smallest normal
+        // number, so the reciprocal's argument and result stay normal: where
+        // m1 is 0 every difference is 0, so m3 is too and the ratio is 0, as
+        // the reference's guard gives; a denormal m1 (differences below
+        // 1.2e-38, too small for any term of L(w) to carry) gives
+        // m3 / 2^-126, below the exact ratio.  m1 above 2^126 does not arise
+        // in data that the FP32 flux arithmetic can carry.
+        return m3 * __builtin_amdgcn_rcpf(fmax(m1, 0x1p-126f));
     } else {
+        if (!(m1 > (T)0)) return (T)0;
         return m3 / m1;
     }
 }

@@ -190,8 +190,12 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
     const int k0 = 2 + kc * zm.kz;
     const int k1 = min(k0 + zm.kz, n2 - 2);
     const int own = (tj + 2) * ZS_BI + ti + 2;
-    // this thread's halo cells of the box (one or two)
-    int hb[ZS_NHQ], hc[ZS_NHQ];
+    // this thread's halo cells of the box (one or two).  A cell of the two
+    // rows above or below the tile is read only by the differences along j,
+    // one of the two columns either side only by those along i, so each
+    // forms its flux along that axis alone (hax); the corner cells of the
+    // box are read by neither and are skipped.
+    int hb[ZS_NHQ], hc[ZS_NHQ], hax[ZS_NHQ];
     bool hv[ZS_NHQ];
 #pragma unroll
     for (int q = 0; q < ZS_NHQ; q++) {
@@ -200,7 +204,9 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
         int bi = 0, bj = 0;
         if (hv[q]) zs_halo(h, bi, bj);
         const int gi = i0 - 2 + bi, gj = j0 - 2 + bj;
-        hv[q] = hv[q] && gi <= n0 - 1 && gj <= n1 - 1;
+        const bool rowh = bj < 2 || bj >= ZS_BJ - 2, colh = bi < 2 || bi >= ZS_BI - 2;
+        hv[q] = hv[q] && gi <= n0 - 1 && gj <= n1 - 1 && !(rowh && colh);
+        hax[q] = rowh ? 1 : 0;
         hb[q] = bj * ZS_BI + bi;
         hc[q] = gi + (int)s1 * gj;
     }
@@ -313,11 +319,14 @@ __device__ __forceinline__ void zstream_body(const Geo &g, const Phys &ph, const
                     ZCell<T> h;
                     zs_derive<T>(h, rh[q], gam);
                     T F[7];
+                    if (hax[q]) {
+                        zs_flux<T>(1, h, F);
 #pragma unroll
-                    for (int a = 0; a < 2; a++) {
-                        zs_flux<T>(a, h, F);
+                        for (int f = 0; f < 7; f++) sF[1][f][hb[q]] = F[f];
+                    } else {
+                        zs_flux<T>(0, h, F);
 #pragma unroll
-                        for (int f = 0; f < 7; f++) sF[a][f][hb[q]] = F[f];
+                        for (int f = 0; f < 7; f++) sF[0][f][hb[q]] = F[f];
                     }
 #pragma unroll
                     for (int d = 0; d < 3; d++) sV[d][hb[q]] = h.v[d];
