@@ -74,6 +74,8 @@ def parse():
                     help="RK4 steps of the mixed-vs-FP64 accuracy leg (the stratified "
                          "workload's horizon)")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-ens-compare", action="store_true",
+                    help="skip stepping the ensemble's runs one grid at a time for comparison")
     ap.add_argument("--ot-steps", type=int, default=500,
                     help="RK4 steps of the Orszag-Tang leg (500 is the workload's count)")
     return ap.parse_args()
@@ -261,7 +263,7 @@ def cpu_baseline(n):
     }
 
 
-def ensemble_leg(steps, warmup, seq_steps=20):
+def ensemble_leg(steps, warmup, seq_steps=20, compare=True):
     import numpy as np
 
     from smaugamd import configs
@@ -292,7 +294,7 @@ def ensemble_leg(steps, warmup, seq_steps=20):
     ens.close()
     # the same runs one grid at a time (single-grid engine, graph per grid)
     seq_ms = 0.0
-    for s in setups:
+    for s in (setups if compare else []):
         e = Engine(s, MIXED)
         e.set_state()
         e.rk4(0.0, s.dt, 2)
@@ -316,8 +318,8 @@ def ensemble_leg(steps, warmup, seq_steps=20):
         "finite": bool(finite),
         "passes_ms": passes,
         "conservation_max_rel_drift": drift,
-        "one_grid_at_a_time_ms_per_step": seq_ms,
-        "speedup_over_one_grid_at_a_time": seq_ms / ms_step,
+        "one_grid_at_a_time_ms_per_step": seq_ms if compare else None,
+        "speedup_over_one_grid_at_a_time": seq_ms / ms_step if compare else None,
     }
 
 
@@ -436,7 +438,7 @@ def main():
     roof["frac_of_measured_copy"] = achieved / roof["measured_copy_gbs"]
     ens = None
     if not a.no_ensemble:
-        ens = ensemble_leg(a.ens_steps, a.warmup)
+        ens = ensemble_leg(a.ens_steps, a.warmup, compare=not a.no_ens_compare)
         print(f"[bench] ensemble: {ens['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     ot = None
     if not a.no_ot:
