@@ -7,12 +7,14 @@
  * hyperviscous dt control, advection per direction and field, gravity,
  * hyperdiffusion per field/direction with the (ii, ii0) and (jj, mm, sb)
  * patterns, user source and the ghost fill of every field and direction.
- * rkon = 1 advances with classical RK4 on the fused engine (cuadvancerk4).
+ * rkon = 1 advances with classical RK4 on the fused engine (cuadvancerk4),
+ * or, with -rk4calls, through the same per-entry-point calls as the onestep
+ * loop, one stage at a time, with cuadvance applying the RK slot algebra.
  *
  * The parameters the reference sets in iosmaugparams.h come from the
  * configuration file header (grid, gamma, gravity) and the command line:
  *
- *   smaug_main -i in.ini [-o out.ini] [-n steps] [-dt dt] [-rk4] [-mixed]
+ *   smaug_main -i in.ini [-o out.ini] [-n steps] [-dt dt] [-rk4 | -rk4calls] [-mixed]
  *              [-bc b0,b1[,b2]] [-hyperdif 0|1] [-chyp rho,e,mom,b]
  *              [-courant c] [-moddton] [-driver A,P,sh,z0,sz,xc,yc] [-torsional]
  *              [-t0 t] [-tend T] [-dtlog file]
@@ -39,6 +41,124 @@
         }                                                                                  \
     } while (0)
 
+/* the host and device handles of cuinit, passed by address as the entry
+ * points take them */
+struct handles {
+    struct params *p, *d_p;
+    struct bparams *bp, *d_bp;
+    struct state *d_state;
+    real *wd, *d_wmod, *d_dwn1, *d_wd, *d_wtemp, *d_wtemp1, *d_wtemp2;
+};
+
+/* One stage of the reference's call sequence (iosmaug.cpp:913-1160):
+ * velocities, advection per direction and field, gravity, hyperdiffusion per
+ * field and direction with the (ii, ii0) and (jj, mm, sb) patterns and the
+ * user source.  Reads slot `order` and accumulates into slot `ordero`.  Every
+ * term takes p->dt except the energy hyperdiffusion, which takes edt (the
+ * reference hands it the loop's initial dt). */
+static void stage(struct handles *h, int order, int ordero, real edt)
+{
+    struct params **pp = &h->p;
+    struct params *p = h->p;
+    for (int dir = 0; dir < NDIM; dir++) CALL(cucomputevels(pp, &h->d_p, &h->d_wmod, &h->d_wd, order, dir));
+    for (int dir = 0; dir < NDIM; dir++) {
+        for (int fld = rho; fld <= (mom1 + NDIM - 1); fld++) {
+            if (fld == mom1 + dir) {
+                CALL(cucomputept(pp, &h->d_p, &h->d_wmod, &h->d_wd, order, dir));
+                CALL(cucomputepbg(pp, &h->d_p, &h->d_wmod, &h->d_wd, order, dir));
+            }
+            CALL(cucentdiff1(pp, &h->d_p, &h->d_state, &h->d_wmod, &h->d_wmod, &h->d_dwn1, &h->d_wd,
+                             order, ordero, p->dt, fld, dir));
+        }
+        for (int fld = energy; fld <= (b1 + (NDIM - 1)); fld++) {
+            if (fld == energy) {
+                CALL(cucomputevels(pp, &h->d_p, &h->d_wmod, &h->d_wd, order, dir));
+                CALL(cucomputepbg(pp, &h->d_p, &h->d_wmod, &h->d_wd, ordero, dir));
+                CALL(cucomputept(pp, &h->d_p, &h->d_wmod, &h->d_wd, order, dir));
+            }
+            CALL(cucentdiff2(pp, &h->d_p, &h->d_state, &h->d_wmod, &h->d_wmod, &h->d_dwn1, &h->d_wd,
+                             order, ordero, p->dt, fld, dir));
+        }
+    }
+    CALL(cugrav(pp, &h->d_p, &h->d_state, &h->d_wmod, &h->d_wmod, &h->d_dwn1, &h->d_wd, order, ordero, p->dt));
+    if (p->divbon == 1)
+        CALL(cudivb(pp, &h->d_p, &h->d_wmod, &h->d_wmod, &h->d_dwn1, &h->d_wd, order, ordero, p->dt));
+    if (p->hyperdifmom == 1) {
+        real cmax[NDIM];
+        real **wd = &h->wd, **d_wmod = &h->d_wmod, **d_wd = &h->d_wd, **d_dwn1 = &h->d_dwn1;
+        real **t0 = &h->d_wtemp, **t1 = &h->d_wtemp1, **t2 = &h->d_wtemp2;
+        struct params **d_p = &h->d_p;
+        p->maxviscoef = 0.0;
+        for (int dim = 0; dim <= (NDIM - 1); dim++) {
+            CALL(cucomputec(pp, d_p, d_wmod, d_wd, order, dim));
+            CALL(cucomputemaxc(pp, d_p, d_wmod, d_wd, order, dim, wd, t0));
+            cmax[dim] = p->cmax;
+            CALL(cuhyperdifvisc1ir(pp, d_p, d_wmod, d_wd, order, t0, t1, t2, rho, dim));
+            CALL(cuhyperdifvisc1r(pp, d_p, d_wmod, wd, d_wd, order, t0, t1, t2, rho, dim));
+            CALL(cuhyperdifvisc1l(pp, d_p, d_wmod, wd, d_wd, order, t0, t1, t2, rho, dim));
+            CALL(cuhyperdifrhosource1(pp, d_p, d_wmod, d_dwn1, d_wd, order, ordero, t0, rho, dim, p->dt));
+        }
+        for (int dim = 0; dim <= (NDIM - 1); dim++) {
+            p->cmax = cmax[dim];
+            CALL(cuhyperdifvisc1ir(pp, d_p, d_wmod, d_wd, order, t0, t1, t2, energy, dim));
+            CALL(cuhyperdifvisc1r(pp, d_p, d_wmod, wd, d_wd, order, t0, t1, t2, energy, dim));
+            CALL(cuhyperdifvisc1l(pp, d_p, d_wmod, wd, d_wd, order, t0, t1, t2, energy, dim));
+            CALL(cuhyperdifesource1(pp, d_p, d_wmod, d_dwn1, d_wd, order, ordero, t0, energy, dim, edt));
+        }
+        for (int dim = 0; dim <= (NDIM - 1); dim++)
+            for (int fi = 0; fi <= (NDIM - 1); fi++) {
+                p->cmax = cmax[dim];
+                CALL(cuhyperdifvisc1ir(pp, d_p, d_wmod, d_wd, order, t0, t1, t2, mom1 + fi, dim));
+                CALL(cuhyperdifvisc1r(pp, d_p, d_wmod, wd, d_wd, order, t0, t1, t2, mom1 + fi, dim));
+                CALL(cuhyperdifvisc1l(pp, d_p, d_wmod, wd, d_wd, order, t0, t1, t2, mom1 + fi, dim));
+                for (int ii1 = 0; ii1 <= 1; ii1++) {
+                    int ii = ii1 == 0 ? dim : fi;
+                    int ii0 = ii1 == 0 ? fi : dim;
+                    if (ii == dim)
+                        CALL(cuhyperdifmomsource1(pp, d_p, d_wmod, d_dwn1, d_wd, order, ordero, t0, fi, dim, ii, ii0, p->dt));
+                    else
+                        CALL(cuhyperdifmomsourcene1(pp, d_p, d_wmod, d_dwn1, d_wd, order, ordero, t0, fi, dim, ii, ii0, p->dt));
+                }
+            }
+        for (int dim = 0; dim <= (NDIM - 1); dim++)
+            for (int fi = 0; fi <= (NDIM - 1); fi++) {
+                if (fi == dim) continue;
+                p->cmax = cmax[dim];
+                CALL(cuhyperdifvisc1ir(pp, d_p, d_wmod, d_wd, order, t0, t1, t2, b1 + fi, dim));
+                CALL(cuhyperdifvisc1r(pp, d_p, d_wmod, wd, d_wd, order, t0, t1, t2, b1 + fi, dim));
+                CALL(cuhyperdifvisc1l(pp, d_p, d_wmod, wd, d_wd, order, t0, t1, t2, b1 + fi, dim));
+                for (int ii1 = 0; ii1 <= 1; ii1++) {
+                    int jj, mm, ii0;
+                    real sb;
+                    if (ii1 == 0) {
+                        jj = dim;
+                        mm = fi;
+                        sb = -1.0;
+                        ii0 = dim;
+                    } else {
+                        ii0 = fi;
+                        mm = dim;
+                        sb = 1.0;
+                        jj = fi;
+                    }
+                    if (mm == dim)
+                        CALL(cuhyperdifbsource1(pp, d_p, d_wmod, d_dwn1, d_wd, order, ordero, t0, fi, dim, jj, ii0, mm, sb, p->dt));
+                    else
+                        CALL(cuhyperdifbsourcene1(pp, d_p, d_wmod, d_dwn1, d_wd, order, ordero, t0, fi, dim, jj, ii0, mm, sb, p->dt));
+                }
+            }
+    }
+    CALL(cusource(pp, &h->d_p, &h->d_state, &h->d_wmod, &h->d_wmod, &h->d_dwn1, &h->d_wd, order, ordero, p->dt));
+}
+
+/* ghost fill of every field and direction of one slot */
+static void fill_ghosts(struct handles *h, int slot)
+{
+    for (int ii = 0; ii <= (b1 + (NDIM - 1)); ii++)
+        for (int idir = 0; idir < NDIM; idir++)
+            CALL(cuboundary(&h->p, &h->bp, &h->d_p, &h->d_bp, &h->d_state, &h->d_wmod, slot, idir, ii));
+}
+
 static int parse_list(const char *s, double *v, int maxn)
 {
     int k = 0;
@@ -57,7 +177,7 @@ static int parse_list(const char *s, double *v, int maxn)
 int main(int argc, char **argv)
 {
     const char *in = NULL, *out = NULL, *dtlog = NULL;
-    int nt = 1, rk4 = 0, mixed = 0, hyperdif = 1, moddton = 0, bcset = 0, driver = 0, have_drv = 0;
+    int nt = 1, rk4 = 0, rk4calls = 0, mixed = 0, hyperdif = 1, moddton = 0, bcset = 0, driver = 0, have_drv = 0;
     double dtarg = 0, courant = 0.3, t0 = 0, tend = 0;
     int nset = 0;
     double bcv[3] = {4, 4, 4}, chv[4] = {0.02, 0.02, 0.4, 0.02}, drv[7] = {0};
@@ -74,6 +194,7 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[a], "-dt") && a + 1 < argc) dtarg = atof(argv[++a]);
         else if (!strcmp(argv[a], "-t0") && a + 1 < argc) t0 = atof(argv[++a]);
         else if (!strcmp(argv[a], "-rk4")) rk4 = 1;
+        else if (!strcmp(argv[a], "-rk4calls")) rk4 = rk4calls = 1;
         else if (!strcmp(argv[a], "-mixed")) mixed = 1;
         else if (!strcmp(argv[a], "-moddton")) moddton = 1;
         else if (!strcmp(argv[a], "-courant") && a + 1 < argc) courant = atof(argv[++a]);
@@ -92,6 +213,10 @@ int main(int argc, char **argv)
     }
     if (tend > 0 && rk4) {
         fprintf(stderr, "-tend applies to the onestep loop only\n");
+        return 2;
+    }
+    if (rk4calls && (mixed || moddton)) {
+        fprintf(stderr, "-rk4calls runs in FP64 with a fixed dt (no -mixed, no -moddton)\n");
         return 2;
     }
     if (tend > 0 && !nset) nt = 1000000000;
@@ -186,15 +311,46 @@ int main(int argc, char **argv)
                 }
         p->it = its;
     }
+    struct handles h = {p, d_p, bp, d_bp, d_state, wd, d_wmod, d_dwn1, d_wd, d_wtemp, d_wtemp1, d_wtemp2};
     /* ghost fill of the initial state (iosmaug.cpp:783-791) */
-    for (int ii = 0; ii <= (b1 + (NDIM - 1)); ii++)
-        for (int idir = 0; idir < NDIM; idir++)
-            CALL(cuboundary(&p, &bp, &d_p, &d_bp, &d_state, &d_wmod, 1, idir, ii));
+    fill_ghosts(&h, 1);
 
     if (p->rkon == 1) {
         CALL(cucomputedervfields(&p, &d_p, &d_wmod, &d_wd, 0));
-        CALL(cuadvancerk4(&p, &d_p, &d_wmod, nt, mixed));
+        if (!rk4calls) CALL(cuadvancerk4(&p, &d_p, &d_wmod, nt, mixed));
         p->it = nt;
+    }
+    /* rkon = 1 through the entry points (-rk4calls): each step is SAC's
+     * fourstep (dev/sac/src/vac.t:447-464), the scheme the reference's RK
+     * branch (iosmaug.cpp:1182-1395) sets out to run.  Stage `order` reads
+     * slot order and accumulates into slot order + 1 (slot 0 for the last)
+     * with dt/2, dt/2, dt, dt/6 at times t, t + dt/2, t + dt/2, t + dt, and
+     * cuadvance(order) applies the slot algebra after it: slot 2 <- slot 0,
+     * slot 3 <- slot 0, slot 1 <- (slot 1 + 2 slot 2 + slot 3 - 4 slot 0)/3,
+     * slot 0 <- slot 0 + slot 1.  Where the reference's branch departs from
+     * fourstep this loop follows fourstep: every direction is advected, every
+     * term takes the stage's dt (through p->dt), the ghosts of every field are
+     * filled in the slot the next stage reads, and slot 1 starts each step as
+     * a copy of slot 0, refreshed through the host (cuupdate, cuupdatemod)
+     * because the stage-0 copy of cucomputedervfields runs the other way. */
+    for (int n = 1; rk4calls && n <= nt; n++) {
+        const real t = p->qt;
+        const real sdt[4] = {dt / 2, dt / 2, dt, dt / 6};
+        const real stime[4] = {t, t + dt / 2, t + dt / 2, t + dt};
+        p->it = n;
+        CALL(cuupdate(&p, &wmod, &wmod, &d_wtemp2, &st, &d_p, &d_wmod, &d_wmod, &d_wtemp2, &d_state, n));
+        CALL(cuupdatemod(&p, &bp, &wmod, &wnew, &wd, &st, &d_p, &d_bp, &d_wmod, &d_wnew, &d_wmod,
+                         &d_dwn1, &d_wd, &d_state, &d_wtemp, &d_wtemp1, &d_wtemp2));
+        for (int order = 0; order < 4; order++) {
+            const int ordero = order == 3 ? 0 : order + 1;
+            p->dt = sdt[order];
+            p->qt = stime[order];
+            stage(&h, order, ordero, p->dt);
+            CALL(cuadvance(&p, &d_p, &d_wmod, &d_wmod, order));
+            fill_ghosts(&h, ordero);
+        }
+        p->dt = dt;
+        p->qt = t + dt;
     }
     double elapsed = 0.0; /* sum of the steps' dt */
     FILE *flog = NULL;
@@ -221,94 +377,8 @@ int main(int argc, char **argv)
         }
         if (tend > 0 && elapsed + p->dt > tend) p->dt = tend - elapsed;
         p->qt = (p->qt) + dt;
-        for (int dir = 0; dir < NDIM; dir++) CALL(cucomputevels(&p, &d_p, &d_wmod, &d_wd, order, dir));
-        for (int dir = 0; dir < NDIM; dir++) {
-            for (int fld = rho; fld <= (mom1 + NDIM - 1); fld++) {
-                if (fld == mom1 + dir) {
-                    CALL(cucomputept(&p, &d_p, &d_wmod, &d_wd, order, dir));
-                    CALL(cucomputepbg(&p, &d_p, &d_wmod, &d_wd, order, dir));
-                }
-                CALL(cucentdiff1(&p, &d_p, &d_state, &d_wmod, &d_wmod, &d_dwn1, &d_wd, order, ordero,
-                                 p->dt, fld, dir));
-            }
-            for (int fld = energy; fld <= (b1 + (NDIM - 1)); fld++) {
-                if (fld == energy) {
-                    CALL(cucomputevels(&p, &d_p, &d_wmod, &d_wd, order, dir));
-                    CALL(cucomputepbg(&p, &d_p, &d_wmod, &d_wd, ordero, dir));
-                    CALL(cucomputept(&p, &d_p, &d_wmod, &d_wd, order, dir));
-                }
-                CALL(cucentdiff2(&p, &d_p, &d_state, &d_wmod, &d_wmod, &d_dwn1, &d_wd, order, ordero,
-                                 p->dt, fld, dir));
-            }
-        }
-        CALL(cugrav(&p, &d_p, &d_state, &d_wmod, &d_wmod, &d_dwn1, &d_wd, order, ordero, p->dt));
-        if (p->divbon == 1) CALL(cudivb(&p, &d_p, &d_wmod, &d_wmod, &d_dwn1, &d_wd, order, ordero, p->dt));
-        if (p->hyperdifmom == 1) {
-            real cmax[NDIM];
-            p->maxviscoef = 0.0;
-            for (int dim = 0; dim <= (NDIM - 1); dim++) {
-                CALL(cucomputec(&p, &d_p, &d_wmod, &d_wd, order, dim));
-                CALL(cucomputemaxc(&p, &d_p, &d_wmod, &d_wd, order, dim, &wd, &d_wtemp));
-                cmax[dim] = p->cmax;
-                CALL(cuhyperdifvisc1ir(&p, &d_p, &d_wmod, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, rho, dim));
-                CALL(cuhyperdifvisc1r(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, rho, dim));
-                CALL(cuhyperdifvisc1l(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, rho, dim));
-                CALL(cuhyperdifrhosource1(&p, &d_p, &d_wmod, &d_dwn1, &d_wd, order, ordero, &d_wtemp, rho, dim, p->dt));
-            }
-            for (int dim = 0; dim <= (NDIM - 1); dim++) {
-                p->cmax = cmax[dim];
-                CALL(cuhyperdifvisc1ir(&p, &d_p, &d_wmod, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, energy, dim));
-                CALL(cuhyperdifvisc1r(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, energy, dim));
-                CALL(cuhyperdifvisc1l(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, energy, dim));
-                CALL(cuhyperdifesource1(&p, &d_p, &d_wmod, &d_dwn1, &d_wd, order, ordero, &d_wtemp, energy, dim, dt));
-            }
-            for (int dim = 0; dim <= (NDIM - 1); dim++)
-                for (int fi = 0; fi <= (NDIM - 1); fi++) {
-                    p->cmax = cmax[dim];
-                    CALL(cuhyperdifvisc1ir(&p, &d_p, &d_wmod, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, mom1 + fi, dim));
-                    CALL(cuhyperdifvisc1r(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, mom1 + fi, dim));
-                    CALL(cuhyperdifvisc1l(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, mom1 + fi, dim));
-                    for (int ii1 = 0; ii1 <= 1; ii1++) {
-                        int ii = ii1 == 0 ? dim : fi;
-                        int ii0 = ii1 == 0 ? fi : dim;
-                        if (ii == dim)
-                            CALL(cuhyperdifmomsource1(&p, &d_p, &d_wmod, &d_dwn1, &d_wd, order, ordero, &d_wtemp, fi, dim, ii, ii0, p->dt));
-                        else
-                            CALL(cuhyperdifmomsourcene1(&p, &d_p, &d_wmod, &d_dwn1, &d_wd, order, ordero, &d_wtemp, fi, dim, ii, ii0, p->dt));
-                    }
-                }
-            for (int dim = 0; dim <= (NDIM - 1); dim++)
-                for (int fi = 0; fi <= (NDIM - 1); fi++) {
-                    if (fi == dim) continue;
-                    p->cmax = cmax[dim];
-                    CALL(cuhyperdifvisc1ir(&p, &d_p, &d_wmod, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, b1 + fi, dim));
-                    CALL(cuhyperdifvisc1r(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, b1 + fi, dim));
-                    CALL(cuhyperdifvisc1l(&p, &d_p, &d_wmod, &wd, &d_wd, order, &d_wtemp, &d_wtemp1, &d_wtemp2, b1 + fi, dim));
-                    for (int ii1 = 0; ii1 <= 1; ii1++) {
-                        int jj, mm, ii0;
-                        real sb;
-                        if (ii1 == 0) {
-                            jj = dim;
-                            mm = fi;
-                            sb = -1.0;
-                            ii0 = dim;
-                        } else {
-                            ii0 = fi;
-                            mm = dim;
-                            sb = 1.0;
-                            jj = fi;
-                        }
-                        if (mm == dim)
-                            CALL(cuhyperdifbsource1(&p, &d_p, &d_wmod, &d_dwn1, &d_wd, order, ordero, &d_wtemp, fi, dim, jj, ii0, mm, sb, p->dt));
-                        else
-                            CALL(cuhyperdifbsourcene1(&p, &d_p, &d_wmod, &d_dwn1, &d_wd, order, ordero, &d_wtemp, fi, dim, jj, ii0, mm, sb, p->dt));
-                    }
-                }
-        }
-        CALL(cusource(&p, &d_p, &d_state, &d_wmod, &d_wmod, &d_dwn1, &d_wd, order, ordero, p->dt));
-        for (int ii = 0; ii <= (b1 + (NDIM - 1)); ii++)
-            for (int idir = 0; idir < NDIM; idir++)
-                CALL(cuboundary(&p, &bp, &d_p, &d_bp, &d_state, &d_wmod, ordero, idir, ii));
+        stage(&h, order, ordero, dt);
+        fill_ghosts(&h, ordero);
         elapsed += p->dt;
         nrun = n;
         if (flog) fprintf(flog, "%d %.17g\n", n, p->dt);

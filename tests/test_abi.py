@@ -77,3 +77,14 @@ def test_sums_noalloc_rejects_bad_sizes_before_device_use():
     assert eng.sums_noalloc(4, (8, 8, 8), eng.FP64, 1, 1) == 1
     assert eng.sums_noalloc(3, (8, 8, 8), 7, 1, 1) == 1
     assert eng.sums_noalloc(3, (8, 8, 8), eng.FP64, 0, 1) == 1
+
+
+@pytest.mark.parametrize("extra", [["-mixed"], ["-moddton"]])
+def test_driver_rk4calls_rejects_mixed_and_moddton(extra):
+    # the per-entry-point RK4 loop runs in FP64 with a fixed dt; the option
+    # check comes before the configuration is read or a device is touched
+    exe = os.path.join(LIB, "smaug_main2d")
+    r = subprocess.run([exe, "-i", "/nonexistent.ini", "-rk4calls"] + extra, capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 2
+    assert "-rk4calls runs in FP64 with a fixed dt" in r.stderr

@@ -2,7 +2,8 @@
 reference's onestep time loop through the cu* entry points of
 libsmaug_hip2d/3d.so, reading and writing VAC ascii configuration files;
 its result is compared with the oracle's forward-Euler (rkon=0) run, and its
-fused RK4 branch (cuadvancerk4) with the oracle's RK4."""
+RK4 branches, fused (cuadvancerk4) and through the per-entry-point calls
+with cuadvance's slot algebra (-rk4calls), with the oracle's RK4."""
 import os
 import subprocess
 
@@ -96,6 +97,49 @@ def test_driver_rk4_fused_stratified(tmp_path):
     err = rel_err(data[: s.nve], ref)
     print("driver rk4 stratified rel err", err)
     assert np.all(err < 1e-11), err
+
+
+def test_driver_rk4calls_ot(tmp_path):
+    # RK4 through the entry points: four stages of the onestep call sequence
+    # (dt/2, dt/2, dt, dt/6) and cuadvance(order) after each
+    s = configs.orszag_tang(nx=32)
+    s.w = perturbed(s, amp=1e-3, seed=11)
+    s.w = Oracle(s).boundary(s.w)
+    inp = roundtrip_state(s, tmp_path, "ot_rkc")
+    out = str(tmp_path / "ot_rkc_out.ini")
+    dt, n = 1e-3, 4
+    run_driver(2, ["-i", inp, "-o", out, "-n", str(n), "-dt", str(dt), "-bc", "0,0",
+                   "-chyp", "0.045,0.02,0.4,0.02", "-rk4calls"], tmp_path)
+    _, _, data = vacio.read_ascii(out)
+    ref, _ = Oracle(s).run(s.w, 0.0, dt, n, rk4=True)
+    err = rel_err(data[: s.nve], ref)
+    print("driver rk4calls OT rel err", err)
+    assert np.all(err < 1e-11), err
+    # one Euler step differs from RK4 by O(dt^2): the slot algebra is not a
+    # single stage in disguise
+    eul, _ = Oracle(s).run(s.w, 0.0, dt, n, rk4=False)
+    assert np.max(rel_err(data[: s.nve], eul)) > 1e-8
+
+
+def test_driver_rk4calls_stratified_matches_fused(tmp_path):
+    # gravity, the vertical driver at the stage times and contcd4 ghosts of
+    # every slot; the fused branch on the same input gives the same state
+    s = _strat_small()
+    s.w = Oracle(s).boundary(perturbed(s, amp=1e-3, seed=12))
+    inp = roundtrip_state(s, tmp_path, "strat_rkc")
+    n, t0 = 3, 1.0
+    outs = {}
+    for flag in ("-rk4calls", "-rk4"):
+        outs[flag] = str(tmp_path / f"strat_rkc_out{flag}.ini")
+        run_driver(3, ["-i", inp, "-o", outs[flag], "-n", str(n), "-dt", repr(s.dt), "-t0",
+                       str(t0), "-bc", "4,4,4", "-driver", _drv_arg(s), flag], tmp_path)
+    _, _, calls = vacio.read_ascii(outs["-rk4calls"])
+    _, _, fused = vacio.read_ascii(outs["-rk4"])
+    ref, _ = Oracle(s).run(s.w, t0, s.dt, n, rk4=True)
+    err = rel_err(calls[: s.nve], ref)
+    print("driver rk4calls stratified rel err", err)
+    assert np.all(err < 1e-11), err
+    assert np.all(rel_err(calls[: s.nve], fused[: s.nve]) < 1e-11)
 
 
 def test_driver_torsional_onestep_and_rk4(tmp_path):
