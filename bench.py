@@ -66,9 +66,11 @@ def parse():
     ap.add_argument("--no-ot", action="store_true")
     ap.add_argument("--shape", default=None,
                     help="fixed 3-D update form, z-stream depth and (optionally) ratio z-stream "
-                         "depth 'FORM,KZ[,KZR]' instead of the autotuner (FORM as in "
-                         "smaug_set_zstream: 0 per-cell, 1 z-streamed, 3 z-streamed with "
-                         "LDS-plane sources), e.g. for profiling runs")
+                         "depth and ratio form 'FORM,KZ[,KZR[,RFORM]]' instead of the autotuner "
+                         "(FORM as in smaug_set_zstream: 0 per-cell, 1 z-streamed, 3 z-streamed "
+                         "with LDS-plane sources; RFORM as in smaug_set_ratio_form: 1 every pair "
+                         "in the ratio pass, 2 the in-plane pairs in the source kernel), e.g. for "
+                         "profiling runs")
     ap.add_argument("--ens-steps", type=int, default=200)
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="RK4 steps of the mixed-vs-FP64 accuracy leg (the stratified "
@@ -96,13 +98,15 @@ def run_path(setup, precision, warmup, steps, dt_scale=1.0, tune=True, shape=Non
         form, kz = vals[0], vals[1]
         e.set_zstream(form, kz)
         if len(vals) > 2:
-            e.set_ratio_form(True, vals[2])
+            e.set_ratio_form(vals[3] if len(vals) > 3 else 1, vals[2])
         info["tuned"] = {"update_form": FORMS[form], "kz": kz,
+                         "ratio_form": e.get_ratio_form()[0],
                          "ratio_kz": e.get_ratio_form()[1], "fixed": True}
     elif tune:
         best, ms = e.autotune(3)
         form, kz = e.get_zstream()
         info["tuned"] = {"update_form": FORMS[form], "kz": kz,
+                         "ratio_form": e.get_ratio_form()[0],
                          "ratio_kz": e.get_ratio_form()[1],
                          "panel_ratio": list(best[0]), "panel_update": list(best[1])}
         e.set_state()  # the tuning launches leave the state untouched; re-fill anyway
@@ -111,6 +115,7 @@ def run_path(setup, precision, warmup, steps, dt_scale=1.0, tune=True, shape=Non
     tot_ms, stage_ms = e.time_rk4(t0, dt, steps)
     info["passes_ms"] = e.time_passes(t0 + (steps + 1) * dt, dt)
     info["form"] = e.get_zstream()[0]
+    info["ratio_form"] = e.get_ratio_form()[0]
     info["bg_plane"] = e.get_bg_plane()
     w = e.get_state()
     e.close()
@@ -165,19 +170,23 @@ def accuracy_leg(setup, nsteps):
             "fp64_rel_l2_under_1e-9_dt_perturbation": sens}
 
 
-def pass_kernels(form, bg_plane, fp32=True):
-    """the kernels of each pass of a 3-D RK4 step for the update form and
-    background layout in use: (name as rocprofv3 prints it, namespace and
-    arguments dropped; launches per step).  With the k-plane background and
-    the z-streamed update, the update of every stage also forms the next
-    stage's aux quantities and cfast maxima (k_ghost_prep, the ghost fill,
-    does the ghost cells'), stage 3 those of the next step's stage 0: the
-    step has no aux pass."""
+def pass_kernels(form, bg_plane, fp32=True, rform=1):
+    """the kernels of each pass of a 3-D RK4 step for the update form, ratio
+    form and background layout in use: (name as rocprofv3 prints it,
+    namespace and arguments dropped; launches per step).  With the k-plane
+    background and the z-streamed update, the update of every stage also
+    forms the next stage's aux quantities and cfast maxima (k_ghost_prep, the
+    ghost fill, does the ghost cells'), stage 3 those of the next step's stage
+    0: the step has no aux pass.  Ratio form 2 with the z-streamed sources:
+    the ratio pass forms the pairs along k (k_ratio_kz) and the source kernel
+    the in-plane pairs (k_hyper_zsr)."""
     t = "float" if fp32 else "double"
     s = "f" if fp32 else "d"
     bp = "true" if bg_plane else "false"
     fold = bool(form) and bg_plane
-    src = {1: f"k_hyper_zs_{s}<{bp}>", 2: f"k_hyper<3, {t}>", 3: f"k_hyper_lds_{s}"}
+    rin = rform == 2 and form == 1
+    src = {1: f"k_hyper_zsr_{s}<{bp}>" if rin else f"k_hyper_zs_{s}<{bp}>",
+           2: f"k_hyper<3, {t}>", 3: f"k_hyper_lds_{s}"}
     if fold:
         upd = [(f"k_zstream_{s}<true, true>", 4)]
     elif form:
@@ -187,7 +196,7 @@ def pass_kernels(form, bg_plane, fp32=True):
     return {
         "aux+cmax": [] if fold else [(f"k_prep_col<{t}>", 4)] if bg_plane
         else [(f"k_prep<3, {t}>", 4)],
-        "ratio": [(f"k_ratio_zs_{s}", 4)],
+        "ratio": [(f"k_ratio_kz_{s}" if rin else f"k_ratio_zs_{s}", 4)],
         "hyperdiffusive sources": [(src.get(form, ""), 4)],
         "stage update": upd,
         "ghost fill": [(f"k_ghost_prep<{t}>", 4)] if fold
@@ -409,7 +418,7 @@ def main():
     # launch are the step figure / 4 stages x cells.
     bytes_launch = BYTES_PER_CELL_STEP["mixed"] / STAGES_PER_STEP * cells
     pms = info["passes_ms"]
-    pk = pass_kernels(info["form"], info["bg_plane"])
+    pk = pass_kernels(info["form"], info["bg_plane"], rform=info["ratio_form"])
     single = [p for p in pms if len(pk[p]) == 1 and pk[p][0][1] == STAGES_PER_STEP]
     dom = max(single or pms, key=pms.get)
     kname = " + ".join(k for k, _ in pk[dom])

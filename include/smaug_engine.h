@@ -92,7 +92,9 @@ int smaug_time_rk4(smaug_engine *e, double t0, double dt, int nsteps,
  * tpx in {1,2,4,8} x tpy in {1,..,32} and, for 3-D grids, the z-streamed
  * ratio kernel at depths 8..128 and the z-streamed update at depths 8..128
  * with either form of the hyperdiffusive-source kernel, and keeps the
- * fastest form per pass. */
+ * fastest form per pass; then ratio form 2 (smaug_set_ratio_form) at depths
+ * 8..32, kept if the ratio and update passes take less time together with
+ * it.  best_ms[p] is the mean time of pass p in the forms kept. */
 int smaug_set_panel(smaug_engine *e, int pass, int tpx, int tpy);
 int smaug_get_panel(smaug_engine *e, int pass, int *tpx, int *tpy);
 int smaug_time_pass(smaug_engine *e, int pass, int reps, double *ms);
@@ -105,10 +107,16 @@ int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double best_ms[2])
  * (enable = 2) or the LDS-plane (enable = 3) source kernel, or per cell
  * (enable = 0; the update pass then uses the XCD panel) */
 int smaug_set_zstream(smaug_engine *e, int enable, int kz);
-/* 3-D grids: the hyperviscosity ratios z-streamed (1, the default; kz output
- * planes per block, 0 picks 32) or per cell (0) */
+/* 3-D grids: the hyperviscosity ratios z-streamed (1; kz output planes per
+ * block, 0 picks 32) or per cell (0).  zstreamed = 2 (the default): the ratio
+ * pass forms the seven pairs along k alone, z-streamed, and the z-streamed
+ * hyperdiffusive-source kernel forms the fourteen in-plane pairs itself, in
+ * LDS, from the state around its tile -- the same ratios without their trip
+ * through HBM.  Form 2 takes effect with the z-streamed update and source
+ * kernel (smaug_set_zstream enable = 1); with any other form of the update
+ * the ratio pass forms every pair, as form 1. */
 int smaug_set_ratio_form(smaug_engine *e, int zstreamed, int kz);
-/* the form of the ratio pass in use (1 z-streamed) and its depth (0 per cell) */
+/* the form of the ratio pass in use (0, 1 or 2) and its depth (0 per cell) */
 int smaug_get_ratio_form(smaug_engine *e, int *zstreamed, int *kz);
 /* the form of the 3-D update pass in use (enable as above) and its depth */
 int smaug_get_zstream(smaug_engine *e, int *enable, int *kz);

@@ -143,7 +143,8 @@ struct smaug_engine {
     int zstream;             // 3-D: 1 = z-streamed stage update, 0 = per-cell stage update
     int kz;                  // z-stream depth (output planes per block)
     int hyper_lds;           // 3-D: hyperdiffusive sources with the plane staged in LDS
-    int ratio_zs;            // 3-D: z-streamed hyperviscosity ratios
+    int ratio_zs;            // 3-D: z-streamed hyperviscosity ratios (2: the pairs along k
+                             // alone, the in-plane pairs formed by the source kernel)
     int kzr;                 // their z-stream depth (0: 32)
     int kzh;                 // z-stream depth of the z-streamed hyperdiffusive sources (0: 32)
 };
@@ -242,6 +243,14 @@ static ZMap make_rmap(const smaug_engine *e, unsigned *nblocks)
     return m;
 }
 
+// 3-D, ratio form 2 with the z-streamed update and source kernel: the ratio
+// pass forms the pairs along k alone and the source kernel the in-plane
+// pairs (the per-cell forms read every pair from R)
+static bool ratio_in_sources(const smaug_engine *e)
+{
+    return e->D == 3 && e->ratio_zs == 2 && e->zstream && e->hyper_lds == 2;
+}
+
 template <int D, typename T>
 static int launch_ratio(smaug_engine *e, const void *u)
 {
@@ -249,12 +258,15 @@ static int launch_ratio(smaug_engine *e, const void *u)
         if (e->ratio_zs) {
             unsigned nb;
             ZMap m = make_rmap(e, &nb);
+            const bool kz = ratio_in_sources(e);
             if constexpr (sizeof(T) == 4)
-                hipLaunchKernelGGL(k_ratio_zs_f, dim3(nb), dim3(RZ_NT), 0, e->stream, e->geo, m,
-                                   (const T *)u, (const T *)e->aux_in, (T *)e->R);
+                hipLaunchKernelGGL(kz ? k_ratio_kz_f : k_ratio_zs_f, dim3(nb), dim3(RZ_NT), 0,
+                                   e->stream, e->geo, m, (const T *)u, (const T *)e->aux_in,
+                                   (T *)e->R);
             else
-                hipLaunchKernelGGL(k_ratio_zs_d, dim3(nb), dim3(RZ_NT), 0, e->stream, e->geo, m,
-                                   (const T *)u, (const T *)e->aux_in, (T *)e->R);
+                hipLaunchKernelGGL(kz ? k_ratio_kz_d : k_ratio_zs_d, dim3(nb), dim3(RZ_NT), 0,
+                                   e->stream, e->geo, m, (const T *)u, (const T *)e->aux_in,
+                                   (T *)e->R);
             HIPCHK(hipGetLastError());
             return 0;
         }
@@ -347,16 +359,20 @@ static int launch_hyper_sources(smaug_engine *e, const void *uin)
         ZMap m = make_hmap(e, &nb);
         const bool bp = bg_planar(e);
         const double *bg = bp ? e->bgp : e->bg;
-        if constexpr (sizeof(T) == 4)
-            hipLaunchKernelGGL(bp ? k_hyper_zs_f<true> : k_hyper_zs_f<false>, dim3(nb), dim3(256),
-                               0, e->stream, e->geo, m, e->phys, (const T *)uin, (const T *)e->R,
-                               (const T *)e->aux_in, bg, (const unsigned long long *)e->d_cmax,
-                               (T *)e->HYP);
-        else
-            hipLaunchKernelGGL(bp ? k_hyper_zs_d<true> : k_hyper_zs_d<false>, dim3(nb), dim3(256),
-                               0, e->stream, e->geo, m, e->phys, (const T *)uin, (const T *)e->R,
-                               (const T *)e->aux_in, bg, (const unsigned long long *)e->d_cmax,
-                               (T *)e->HYP);
+        const bool rin = ratio_in_sources(e);
+        if constexpr (sizeof(T) == 4) {
+            auto kern = rin ? (bp ? k_hyper_zsr_f<true> : k_hyper_zsr_f<false>)
+                            : (bp ? k_hyper_zs_f<true> : k_hyper_zs_f<false>);
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, e->stream, e->geo, m, e->phys,
+                               (const T *)uin, (const T *)e->R, (const T *)e->aux_in, bg,
+                               (const unsigned long long *)e->d_cmax, (T *)e->HYP);
+        } else {
+            auto kern = rin ? (bp ? k_hyper_zsr_d<true> : k_hyper_zsr_d<false>)
+                            : (bp ? k_hyper_zs_d<true> : k_hyper_zs_d<false>);
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, e->stream, e->geo, m, e->phys,
+                               (const T *)uin, (const T *)e->R, (const T *)e->aux_in, bg,
+                               (const unsigned long long *)e->d_cmax, (T *)e->HYP);
+        }
         HIPCHK(hipGetLastError());
         return 0;
     }
@@ -786,7 +802,7 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
     e->kz = 0;
     e->hyper_lds = 2;
     e->kzh = 0;
-    e->ratio_zs = D == 3;
+    e->ratio_zs = D == 3 ? 2 : 0;
     e->kzr = 0;
     e->bgp_on = 1;
     *out = e;
@@ -1214,9 +1230,11 @@ extern "C" int smaug_time_pass(smaug_engine *e, int pass, int reps, double *ms)
 
 // picks, for the ratio and update passes, the fastest XCD panel among
 // tpx in {1, 2, 4, 8} and tpy in {1, 2, 4, 8, 16, 32}; for 3-D grids the
-// ratio pass also tries the z-streamed ratio kernel at depths 8..64 and
-// the update pass the z-streamed update at depths 4..64, and each keeps
-// whichever form is fastest
+// ratio pass also tries the z-streamed ratio kernel at depths 8..128 and
+// the update pass the z-streamed update at depths 8..128 with either
+// z-streamed source kernel, and each keeps whichever form is fastest; then
+// ratio form 2 at depths 8..32 is kept if the two passes together take less
+// time with it
 extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double best_ms[2])
 {
     if (!e || reps < 1) return fail(SMAUG_EINVAL, "bad argument");
@@ -1224,6 +1242,7 @@ extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double 
     const int zs0 = e->zstream, rz0 = e->ratio_zs;
     e->zstream = 0;
     e->ratio_zs = 0;
+    double bms[2] = {0, 0};
     for (int pass = 0; pass < 2; pass++) {
         if (pass == 0 && !e->phys.hyperdif) {
             if (best) best[0][0] = best[0][1] = 0;
@@ -1296,7 +1315,43 @@ extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double 
             best[pass][0] = bx;
             best[pass][1] = by;
         }
-        if (best_ms) best_ms[pass] = bm;
+        bms[pass] = bm;
+    }
+    // ratio form 2 (the pairs along k in the ratio pass, the in-plane pairs
+    // formed by the z-streamed source kernel) against the tuned form: kept
+    // where the ratio pass and the update pass with its sources take less
+    // time together
+    if (e->D == 3 && e->phys.hyperdif && e->zstream && e->hyper_lds == 2) {
+        static const int KZ[] = {8, 16, 32};
+        const int rz = e->ratio_zs, kzr = e->kzr;
+        double bt = bms[0] + bms[1], b0 = 0, b1 = 0;
+        int bk = -1;
+        e->ratio_zs = 2;
+        for (int kz : KZ) {
+            e->kzr = kz;
+            double m0, m1;
+            int rc = DISPATCH(e, time_pass, e, 0, reps, &m0);
+            if (!rc) rc = DISPATCH(e, time_pass, e, 1, reps, &m1);
+            if (rc) return rc;
+            if (m0 + m1 < bt) {
+                bt = m0 + m1;
+                bk = kz;
+                b0 = m0;
+                b1 = m1;
+            }
+        }
+        if (bk > 0) {
+            e->kzr = bk;
+            bms[0] = b0;
+            bms[1] = b1;
+        } else {
+            e->ratio_zs = rz;
+            e->kzr = kzr;
+        }
+    }
+    if (best_ms) {
+        best_ms[0] = bms[0];
+        best_ms[1] = bms[1];
     }
     if (e->D != 3) {
         e->zstream = zs0;
@@ -1308,9 +1363,9 @@ extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double 
 
 extern "C" int smaug_set_ratio_form(smaug_engine *e, int zstreamed, int kz)
 {
-    if (!e || kz < 0) return fail(SMAUG_EINVAL, "bad argument");
+    if (!e || kz < 0 || zstreamed < 0 || zstreamed > 2) return fail(SMAUG_EINVAL, "bad argument");
     if (zstreamed && e->D != 3) return fail(SMAUG_EINVAL, "the z-streamed ratios are for 3-D grids");
-    e->ratio_zs = zstreamed ? 1 : 0;
+    e->ratio_zs = zstreamed;
     e->kzr = kz;
     drop_graphs(e);
     return 0;
@@ -1320,7 +1375,7 @@ extern "C" int smaug_get_ratio_form(smaug_engine *e, int *zstreamed, int *kz)
 {
     if (!e || !zstreamed || !kz) return fail(SMAUG_EINVAL, "null argument");
     unsigned nb;
-    *zstreamed = e->D == 3 && e->ratio_zs ? 1 : 0;
+    *zstreamed = e->D == 3 ? e->ratio_zs : 0;
     *kz = *zstreamed ? make_rmap(e, &nb).kz : 0;
     return 0;
 }

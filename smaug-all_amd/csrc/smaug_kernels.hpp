@@ -208,26 +208,10 @@ __global__ __launch_bounds__(256) void k_prep(Geo g, Phys ph, const T *__restric
 // (hyperdifvisc1r.cu:426-1300, hyperdifvisc1l.cu:180-1450).
 // Computed on the box [1, n-3] in every dim; R is stored as T.
 // ---------------------------------------------------------------------------
-// R of one window.  D3 is formed with one fused multiply-add from the two
-// first differences, so it carries a single rounding even where it is a
-// small remainder of nearly equal terms (smooth data); in FP32 the
-// differences of the FP32 state are then exact to one rounding, as in FP64.
+// R from the two maxima m1 = max D1, m3 = max D3 of a window
 template <typename T>
-__device__ __forceinline__ T ratio6(const T q[6])
+__device__ __forceinline__ T ratio_fin(T m1, T m3)
 {
-    T d[5];
-#pragma unroll
-    for (int f = 0; f < 5; f++) d[f] = q[f + 1] - q[f];
-    // running maxima of non-negative values: fmax(v, m) equals
-    // (v > m ? v : m) here, a NaN v included (both keep m)
-    T m1 = 0, m3 = 0;
-#pragma unroll
-    for (int f = 0; f < 5; f++) m1 = fmax(fabs(d[f]), m1);
-#pragma unroll
-    for (int f = 1; f < 4; f++) {
-        // 3 (q[f+1]-q[f]) - (q[f+2]-q[f-1])
-        m3 = fmax(fabs(fma((T)3, d[f], -(q[f + 2] - q[f - 1]))), m3);
-    }
     if constexpr (sizeof(T) == 4) {
         // FP32: the hardware reciprocal (1 ulp) and one multiply, within
         // 2 ulp of m3/m1.  The denominator is clamped to the smallest normal
@@ -241,6 +225,55 @@ __device__ __forceinline__ T ratio6(const T q[6])
     } else {
         if (!(m1 > (T)0)) return (T)0;
         return m3 / m1;
+    }
+}
+
+// D3 at the centre difference d = q[f+1]-q[f] of a window: formed with one
+// fused multiply-add from the two first differences, so it carries a
+// single rounding even where it is a small remainder of nearly equal terms
+// (smooth data); in FP32 the differences of the FP32 state are then exact
+// to one rounding, as in FP64.  qa = q[f+2], qb = q[f-1].
+template <typename T>
+__device__ __forceinline__ T d3abs(T d, T qa, T qb)
+{
+    return fabs(fma((T)3, d, -(qa - qb)));
+}
+
+// R of one window
+template <typename T>
+__device__ __forceinline__ T ratio6(const T q[6])
+{
+    T d[5];
+#pragma unroll
+    for (int f = 0; f < 5; f++) d[f] = q[f + 1] - q[f];
+    // running maxima of non-negative values: fmax(v, m) equals
+    // (v > m ? v : m) here, a NaN v included (both keep m)
+    T m1 = 0, m3 = 0;
+#pragma unroll
+    for (int f = 0; f < 5; f++) m1 = fmax(fabs(d[f]), m1);
+#pragma unroll
+    for (int f = 1; f < 4; f++) m3 = fmax(d3abs(d[f], q[f + 2], q[f - 1]), m3);
+    return ratio_fin(m1, m3);
+}
+
+// R of the five windows q[c..c+5], c = 0..4, of ten values, sharing their
+// differences; each equals ratio6 of its window
+template <typename T>
+__device__ __forceinline__ void ratio6x5(const T q[10], T r[5])
+{
+    T d[9], t[9];
+#pragma unroll
+    for (int x = 0; x < 9; x++) d[x] = q[x + 1] - q[x];
+#pragma unroll
+    for (int x = 1; x < 8; x++) t[x] = d3abs(d[x], q[x + 2], q[x - 1]);
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+        T m1 = 0, m3 = 0;
+#pragma unroll
+        for (int f = 0; f < 5; f++) m1 = fmax(fabs(d[c + f]), m1);
+#pragma unroll
+        for (int f = 1; f < 4; f++) m3 = fmax(t[c + f], m3);
+        r[c] = ratio_fin(m1, m3);
     }
 }
 

@@ -712,6 +712,96 @@ __global__ __launch_bounds__(RZ_NT) void k_ratio_zs_d(Geo g, ZMap zm, const doub
 }
 
 // ---------------------------------------------------------------------------
+// The seven ratio pairs along k alone (prho, pe, pv(0..2), pb(0..1) of
+// dim 2), for the source kernel that forms the in-plane pairs itself.  The
+// cells, blocks and arithmetic are those of k_ratio_zs: each thread streams
+// its own column through planes k0-2 .. k1+2 (mirrored pads along k) with
+// the last six planes in registers, and writes the ratios of the plane three
+// behind the front.  No LDS: the column is the only neighbourhood.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void ratio_kz_body(const Geo &g, const ZMap &zm,
+                                              const T *__restrict__ u,
+                                              const T *__restrict__ aux, T *__restrict__ R)
+{
+    typedef Lay<3> LY;
+    typedef Aux<3> AX;
+    int tx, ty, kc;
+    if (!zmap_block(zm, tx, ty, kc)) return;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    const size_t nc = (size_t)g.nc;
+    const ci_t s1 = (ci_t)g.s1, s2 = (ci_t)g.s2;
+    const int i = tx * RZ_TI + threadIdx.x % RZ_TI, j = 1 + ty * RZ_TJ + threadIdx.x / RZ_TI;
+    if (i > n0 - 1 || j > n1 - 3) return;
+    const int k0 = 1 + kc * zm.kz;
+    const int k1 = min(k0 + zm.kz, n2 - 2);
+    const ci_t cell = (ci_t)i + s1 * (ci_t)j;
+    // rho', e_int, v_0..2, b'_0, b'_1 of one plane
+    auto load = [&](T v[7], int P) {
+        const ci_t c = cell + s2 * (ci_t)pad_map(P, n2);
+        v[0] = ld(u, c);
+        v[1] = ld(aux + AX::EINT * nc, c);
+#pragma unroll
+        for (int d = 0; d < 3; d++) v[2 + d] = ld(aux + (AX::V0 + d) * nc, c);
+        v[5] = ld(u + LY::B0 * nc, c);
+        v[6] = ld(u + (LY::B0 + 1) * nc, c);
+    };
+    T zw[6][7];
+#pragma unroll
+    for (int p = 0; p < 6; p++)
+#pragma unroll
+        for (int q = 0; q < 7; q++) zw[p][q] = 0;
+    T nx[7];
+    load(nx, k0 - 2);
+    for (int P = k0 - 2; P <= k1 + 2; P++) {
+#pragma unroll
+        for (int p = 0; p < 5; p++)
+#pragma unroll
+            for (int q = 0; q < 7; q++) zw[p][q] = zw[p + 1][q];
+#pragma unroll
+        for (int q = 0; q < 7; q++) zw[5][q] = nx[q];
+        if (P + 1 <= k1 + 2) load(nx, P + 1);
+        const int O = P - 3; // the window planes O-2..O+3 are in zw
+        if (O >= k0 && O < k1) {
+            const ci_t c = cell + s2 * (ci_t)O;
+            T w[6];
+            auto zwin = [&](int q) {
+#pragma unroll
+                for (int p = 0; p < 6; p++) w[p] = zw[p][q];
+            };
+            zwin(0);
+            st(R + LY::prho(2) * nc, c, ratio6<T>(w));
+            zwin(1);
+            st(R + LY::pe(2) * nc, c, ratio6<T>(w));
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                zwin(2 + f);
+                st(R + LY::pv(f, 2) * nc, c, ratio6<T>(w));
+            }
+#pragma unroll
+            for (int f = 0; f < 2; f++) {
+                zwin(5 + f);
+                st(R + LY::pb(f, 2) * nc, c, ratio6<T>(w));
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(RZ_NT) void k_ratio_kz_f(Geo g, ZMap zm, const float *__restrict__ u,
+                                                       const float *__restrict__ aux,
+                                                       float *__restrict__ R)
+{
+    ratio_kz_body<float>(g, zm, u, aux, R);
+}
+
+__global__ __launch_bounds__(RZ_NT) void k_ratio_kz_d(Geo g, ZMap zm, const double *__restrict__ u,
+                                                       const double *__restrict__ aux,
+                                                       double *__restrict__ R)
+{
+    ratio_kz_body<double>(g, zm, u, aux, R);
+}
+
+// ---------------------------------------------------------------------------
 // Hyperdiffusive sources with the plane staged in LDS.  A block takes a
 // 64 x 4 tile of one plane; the state, aux quantities, rho_b and 18 of the
 // 21 ratio pairs of the tile and a 1-cell halo are loaded once, coalesced,
@@ -905,50 +995,113 @@ struct HzCell {
     T R[21];
 };
 
+// In-plane ratios formed in the source kernel (RIN): the state that ratio6
+// reads -- rho', e_int, v_0..2, b'_0..2 -- staged on the 70 x 10 box of
+// cells i0-3 .. i0+66, j0-3 .. j0+6 (pads mirrored as in k_ratio_zs), which
+// holds the 66 x 6 box at offset (2, 2).  The pairs along i are formed on the
+// 65 x 6 box cells that read them (box columns 0..64), those along j on the
+// 66 x 5 cells (box rows 0..4).
+constexpr int QB_BI = HB_BI + 4, QB_BJ = HB_BJ + 4, QB_NB = QB_BI * QB_BJ;
+constexpr int QB_NX = QB_NB - HB_NB;       // ratio-box cells outside the 66 x 6 box
+constexpr int QB_PX = (QB_NX + 255) / 256; // of them per thread
+
+// ratio-box coordinates of the cells outside the 66 x 6 box: rows 0, 1, 8, 9
+// in full, then columns 0, 1, 68, 69 of rows 2..7
+__device__ __forceinline__ void qb_extra(int e, int &qi, int &qj)
+{
+    if (e < 4 * QB_BI) {
+        const int r = e / QB_BI;
+        qj = r < 2 ? r : QB_BJ - 4 + r;
+        qi = e - r * QB_BI;
+    } else {
+        const int x = e - 4 * QB_BI;
+        qj = 2 + (x >> 2);
+        qi = (x & 3) < 2 ? (x & 3) : QB_BI - 4 + (x & 3);
+    }
+}
+
+// pair a (0..6) of the in-plane pairs along d: rho', e_int, v_0..2, then the
+// two b'_f with f != d -- its ratio-box field and its sR slot (hz_islot)
+__device__ __forceinline__ int rin_field(int d, int a)
+{
+    return a < 5 ? a : (a == 6 ? 7 : (d == 0 ? 6 : 5));
+}
+__device__ __forceinline__ int rin_slot(int d, int a)
+{
+    return a < 5 ? 2 * a + d : (d == 0 ? 6 + a : (a == 5 ? 10 : 13));
+}
+
 // all 21 ratio pairs for the own column (full = true); a halo column needs
 // only the in-plane pairs and pv/pb(f, 2), f in-plane.  BGP: the background
-// does not vary along k, and h.rb was set once for the column.
-template <bool BGP, typename T>
+// does not vary along k, and h.rb was set once for the column.  RIN: the
+// in-plane pairs are formed in the kernel and not loaded; rho', e_int, v and
+// b' come from cq (the cell with pads mirrored in-plane, which is c itself
+// for a cell on the grid), the rest from c.
+template <bool BGP, bool RIN, typename T>
 __device__ __forceinline__ void hz_load(HzCell<T> &h, const T *__restrict__ u,
                                         const T *__restrict__ aux, const double *__restrict__ bg,
-                                        const T *__restrict__ R, size_t nc, ci_t c, bool full)
+                                        const T *__restrict__ R, size_t nc, ci_t c, bool full,
+                                        ci_t cq = 0)
 {
     typedef Lay<3> LY;
     typedef Aux<3> AX;
-    h.r = ld(u, c);
-    h.e = ld(aux + AX::EINT * nc, c);
+    if constexpr (!RIN) cq = c;
+    h.r = ld(u, cq);
+    h.e = ld(aux + AX::EINT * nc, cq);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         h.m[d] = ld(u + (LY::M0 + d) * nc, c);
-        h.b[d] = ld(u + (LY::B0 + d) * nc, c);
-        h.v[d] = ld(aux + (AX::V0 + d) * nc, c);
+        h.b[d] = ld(u + (LY::B0 + d) * nc, cq);
+        h.v[d] = ld(aux + (AX::V0 + d) * nc, cq);
     }
     if constexpr (!BGP) h.rb = ld(bg + nc, c);
 #pragma unroll
-    for (int p = 0; p < LY::NPAIR; p++)
-        h.R[p] = (full || hz_islot(p) >= 0 || p == LY::pv(0, 2) || p == LY::pv(1, 2) ||
-                  p == LY::pb(0, 2) || p == LY::pb(1, 2))
-                     ? ld(R + p * nc, c)
-                     : (T)0;
+    for (int p = 0; p < LY::NPAIR; p++) {
+        const bool kh = p == LY::pv(0, 2) || p == LY::pv(1, 2) || p == LY::pb(0, 2) ||
+                        p == LY::pb(1, 2);
+        const bool need = RIN ? (kh || (full && hz_islot(p) < 0))
+                              : (full || hz_islot(p) >= 0 || kh);
+        h.R[p] = need ? ld(R + p * nc, c) : (T)0;
+    }
 }
 
-// LDS view of the centre plane for the in-plane terms (accesses with dk = 0)
-template <typename T>
+// LDS view of the centre plane for the in-plane terms (accesses with dk = 0).
+// RIN: rho', e_int, v and b' are read from the ratio box sQ, m from sU.
+template <typename T, bool RIN>
 struct HzView {
-    const T (*sU)[HB_NB]; // rho', m_0..2, b'_0..2
-    const T (*sA)[HB_NB]; // e_int, v_0..2
+    const T (*sU)[HB_NB]; // rho', m_0..2, b'_0..2 (RIN: m_0..2)
+    const T (*sA)[HB_NB]; // e_int, v_0..2 (not with RIN)
+    const T (*sQ)[QB_NB]; // RIN: rho', e_int, v_0..2, b'_0..2
     const T *sRf; // full density rho' + rho_b (formed in double, stored in T)
     const T (*sR)[HB_NB];
-    int own;
+    int own, ownq;
     __device__ __forceinline__ int bx(int di, int dj) const { return own + di + HB_BI * dj; }
-    __device__ __forceinline__ T U(int f, int di, int dj) const { return sU[hb_uslot(f)][bx(di, dj)]; }
-    __device__ __forceinline__ T A(int a, int di, int dj) const { return sA[a - Aux<3>::EINT][bx(di, dj)]; }
+    __device__ __forceinline__ int qx(int di, int dj) const { return ownq + di + QB_BI * dj; }
+    __device__ __forceinline__ T U(int f, int di, int dj) const
+    {
+        if constexpr (RIN) {
+            if (f == Lay<3>::RHO) return sQ[0][qx(di, dj)];
+            if (f >= Lay<3>::B0) return sQ[5 + f - Lay<3>::B0][qx(di, dj)];
+            return sU[f - Lay<3>::M0][bx(di, dj)];
+        } else {
+            return sU[hb_uslot(f)][bx(di, dj)];
+        }
+    }
+    __device__ __forceinline__ T A(int a, int di, int dj) const
+    {
+        if constexpr (RIN)
+            return sQ[1 + a - Aux<3>::EINT][qx(di, dj)];
+        else
+            return sA[a - Aux<3>::EINT][bx(di, dj)];
+    }
     __device__ __forceinline__ T rhof(int di, int dj) const { return sRf[bx(di, dj)]; }
     __device__ __forceinline__ T Rp(int p, int di, int dj) const { return sR[hz_islot(p)][bx(di, dj)]; }
 };
 
-// BGP: bg is a background stored as one k-plane (field stride s2)
-template <bool BGP, typename T>
+// BGP: bg is a background stored as one k-plane (field stride s2).  RIN:
+// the in-plane ratio pairs are formed here from the ratio box sQ (R holds
+// the pairs along k only); sU then holds m_0..2 and sA is not used.
+template <bool BGP, bool RIN, typename T>
 __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, const Phys &ph,
                                               const T *__restrict__ uin,
                                               const T *__restrict__ R,
@@ -959,7 +1112,8 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                                               T (*sA)[HB_NB], T *sRf, T (*sR)[HB_NB],
                                               T (*sX)[HB_NB], T *sIz, T (*sCx)[HB_TI],
                                               T (*sCy)[HB_TJ], T (*sCz)[ZS_KZMAX],
-                                              T (*sKx)[HB_TI], T (*sKy)[HB_TJ], T (*sKz)[ZS_KZMAX])
+                                              T (*sKx)[HB_TI], T (*sKy)[HB_TJ], T (*sKz)[ZS_KZMAX],
+                                              T (*sQ)[QB_NB])
 {
     typedef Lay<3> LY;
     typedef Aux<3> AX;
@@ -1050,6 +1204,11 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
     int bb[2], bi_[2], bj_[2];
     bool bv[2];
     ci_t bc[2];
+    // RIN: whether the box position exists (on the grid or not), its index
+    // in the ratio box and its cell with pads mirrored in-plane
+    bool pq[2];
+    int bq[2];
+    ci_t cqa[2];
 #pragma unroll
     for (int q = 0; q < 2; q++) {
         int bi, bj;
@@ -1062,20 +1221,56 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
             bi = bj = 0;
             if (bv[q]) hb_halo(threadIdx.x, bi, bj);
         }
+        pq[q] = bv[q];
         const int gi = i0 - 1 + bi, gj = j0 - 1 + bj;
         bv[q] = bv[q] && gi <= n0 - 1 && gj <= n1 - 1;
         bi_[q] = gi;
         bj_[q] = gj;
         bb[q] = bj * HB_BI + bi;
         bc[q] = (ci_t)gi + s1 * (ci_t)gj;
+        bq[q] = (bj + 2) * QB_BI + bi + 2;
+        cqa[q] = (ci_t)pad_map(gi, n0) + s1 * (ci_t)pad_map(gj, n1);
     }
+    // RIN: the ratio-box cells outside the 66 x 6 box that this thread loads
+    bool ev[QB_PX];
+    int eq[QB_PX];
+    ci_t ce[QB_PX];
+    T xq[QB_PX][8];
+#pragma unroll
+    for (int x = 0; x < QB_PX; x++) {
+        const int e = threadIdx.x + 256 * x;
+        ev[x] = RIN && e < QB_NX;
+        int qi = 0, qj = 0;
+        if (ev[x]) qb_extra(e, qi, qj);
+        eq[x] = qj * QB_BI + qi;
+        ce[x] = (ci_t)pad_map(i0 - 3 + qi, n0) + s1 * (ci_t)pad_map(j0 - 3 + qj, n1);
+#pragma unroll
+        for (int f = 0; f < 8; f++) xq[x][f] = 0;
+    }
+    auto xload = [&](int P) {
+        typedef Aux<3> AX;
+#pragma unroll
+        for (int x = 0; x < QB_PX; x++)
+            if (ev[x]) {
+                const ci_t c = ce[x] + s2 * (ci_t)P;
+                xq[x][0] = ld(uin, c);
+                xq[x][1] = ld(aux + AX::EINT * nc, c);
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    xq[x][2 + d] = ld(aux + (AX::V0 + d) * nc, c);
+                    xq[x][5 + d] = ld(uin + (LY::B0 + d) * nc, c);
+                }
+            }
+    };
     const int own = bb[0];
-    HzView<T> vw;
+    HzView<T, RIN> vw;
     vw.sU = sU;
     vw.sA = sA;
+    vw.sQ = sQ;
     vw.sRf = sRf;
     vw.sR = sR;
     vw.own = own;
+    vw.ownq = bq[0];
 
     // per box column, planes P-1 and P-2 (index 0 and 1): v_0, v_1, b_0, b_1,
     // and the k-pairs pv(0,2), pv(1,2), pb(0,2), pb(1,2); rho, m_2, b_2 of P-1
@@ -1111,9 +1306,21 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
 #pragma unroll
         for (int q = 0; q < 2; q++) cur[q].rb = ld(bg + (size_t)s2, bv[q] ? bc[q] : 0u);
     }
+    if constexpr (RIN) {
+        xload(k0 - 1);
 #pragma unroll
-    for (int q = 0; q < 2; q++)
-        if (bv[q]) hz_load<BGP, T>(cur[q], uin, aux, bg, R, nc, bc[q] + s2 * (ci_t)(k0 - 1), q == 0);
+        for (int q = 0; q < 2; q++)
+            if (pq[q])
+                hz_load<BGP, RIN, T>(cur[q], uin, aux, bg, R, nc,
+                                     (bv[q] ? bc[q] : 0u) + s2 * (ci_t)(k0 - 1), q == 0,
+                                     cqa[q] + s2 * (ci_t)(k0 - 1));
+    } else {
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+            if (bv[q])
+                hz_load<BGP, RIN, T>(cur[q], uin, aux, bg, R, nc, bc[q] + s2 * (ci_t)(k0 - 1),
+                                     q == 0);
+    }
 
     // The output completed in one iteration is stored at the head of the
     // next one, behind the loads of the own column's plane, which the head
@@ -1128,23 +1335,50 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
 #pragma unroll
         for (int f = 0; f < 8; f++) st(dUh + f * nc, hco, hs[f]);
         __syncthreads(); // the previous plane's LDS reads are done
+        if constexpr (RIN) {
 #pragma unroll
-        for (int q = 0; q < 2; q++)
-            if (bv[q]) {
-                const int b = bb[q];
-                sU[0][b] = cur[q].r;
+            for (int q = 0; q < 2; q++) {
+                if (pq[q]) {
+                    const int b = bq[q];
+                    sQ[0][b] = cur[q].r;
+                    sQ[1][b] = cur[q].e;
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    sU[1 + d][b] = cur[q].m[d];
-                    sU[4 + d][b] = cur[q].b[d];
-                    sA[1 + d][b] = cur[q].v[d];
+                    for (int d = 0; d < 3; d++) {
+                        sQ[2 + d][b] = cur[q].v[d];
+                        sQ[5 + d][b] = cur[q].b[d];
+                    }
                 }
-                sA[0][b] = cur[q].e;
-                sRf[b] = (T)((double)cur[q].r + cur[q].rb);
+                if (bv[q]) {
+                    const int b = bb[q];
 #pragma unroll
-                for (int p = 0; p < LY::NPAIR; p++)
-                    if (hz_islot(p) >= 0) sR[hz_islot(p)][b] = cur[q].R[p];
+                    for (int d = 0; d < 3; d++) sU[d][b] = cur[q].m[d];
+                    sRf[b] = (T)((double)cur[q].r + cur[q].rb);
+                }
             }
+#pragma unroll
+            for (int x = 0; x < QB_PX; x++)
+                if (ev[x])
+#pragma unroll
+                    for (int f = 0; f < 8; f++) sQ[f][eq[x]] = xq[x][f];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 2; q++)
+                if (bv[q]) {
+                    const int b = bb[q];
+                    sU[0][b] = cur[q].r;
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        sU[1 + d][b] = cur[q].m[d];
+                        sU[4 + d][b] = cur[q].b[d];
+                        sA[1 + d][b] = cur[q].v[d];
+                    }
+                    sA[0][b] = cur[q].e;
+                    sRf[b] = (T)((double)cur[q].r + cur[q].rb);
+#pragma unroll
+                    for (int p = 0; p < LY::NPAIR; p++)
+                        if (hz_islot(p) >= 0) sR[hz_islot(p)][b] = cur[q].R[p];
+                }
+        }
         // X_{f,k}(P-1) of this thread's box columns, from planes P-2, P-1
         // (registers) and P (cur)
         const int O = P - 1;
@@ -1286,11 +1520,68 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
         // The own column's come last and are issued on every path (the last
         // iteration reloads its plane, a column off the grid reads cell 0),
         // so that the compiler's waits for earlier loads count past them.
-        if (P + 1 <= k1 && bv[1])
-            hz_load<BGP, T>(cur[1], uin, aux, bg, R, nc, bc[1] + s2 * (ci_t)(P + 1), false);
-        hz_load<BGP, T>(cur[0], uin, aux, bg, R, nc, bv[0] ? bc[0] + s2 * (ci_t)min(P + 1, k1) : 0u,
-                   true);
+        if constexpr (RIN) {
+            if (P + 1 <= k1) {
+                xload(P + 1);
+                if (pq[1])
+                    hz_load<BGP, RIN, T>(cur[1], uin, aux, bg, R, nc,
+                                         (bv[1] ? bc[1] : 0u) + s2 * (ci_t)(P + 1), false,
+                                         cqa[1] + s2 * (ci_t)(P + 1));
+            }
+            hz_load<BGP, RIN, T>(cur[0], uin, aux, bg, R, nc,
+                                 bv[0] ? bc[0] + s2 * (ci_t)min(P + 1, k1) : 0u, true,
+                                 cqa[0] + s2 * (ci_t)min(P + 1, k1));
+        } else {
+            if (P + 1 <= k1 && bv[1])
+                hz_load<BGP, RIN, T>(cur[1], uin, aux, bg, R, nc, bc[1] + s2 * (ci_t)(P + 1),
+                                     false);
+            hz_load<BGP, RIN, T>(cur[0], uin, aux, bg, R, nc,
+                                 bv[0] ? bc[0] + s2 * (ci_t)min(P + 1, k1) : 0u, true);
+        }
         __syncthreads();
+        if constexpr (RIN) {
+            // the in-plane ratio pairs of plane P on the box cells that read
+            // them, from the ratio box, five cells of a row or column per
+            // item; 0 where k_ratio_zs writes none (rows above n1-3, planes
+            // above n2-3, cells off the grid).  Items: the 66 box columns
+            // along j (cells in rows 0..4), then the 13 runs of five cells
+            // of each box row along i (columns 0..64), pair by pair.
+            const bool zk = P > n2 - 3;
+            constexpr int NRUN = (HB_BI - 1) / 5;
+            constexpr int NI1 = 7 * HB_BI, NI0 = 7 * HB_BJ * NRUN;
+            for (int u = threadIdx.x; u < NI1 + NI0; u += 256) {
+                int d, a, bi, bj;
+                if (u < NI1) {
+                    d = 1;
+                    a = u / HB_BI;
+                    bi = u - a * HB_BI;
+                    bj = 0;
+                } else {
+                    d = 0;
+                    const int x = u - NI1;
+                    a = x / (HB_BJ * NRUN);
+                    const int y = x - a * (HB_BJ * NRUN);
+                    bj = y / NRUN;
+                    bi = 5 * (y - bj * NRUN);
+                }
+                // first window value: row 0 of column bi + 2 (along j), or
+                // column bi of row bj + 2 (along i)
+                const int qs = d ? QB_BI : 1, rs = d ? HB_BI : 1;
+                const T *q = sQ[rin_field(d, a)] + (d ? bi + 2 : (bj + 2) * QB_BI + bi);
+                T v[10], r[5];
+#pragma unroll
+                for (int x = 0; x < 10; x++) v[x] = q[x * qs];
+                ratio6x5<T>(v, r);
+                T *o = sR[rin_slot(d, a)] + bj * HB_BI + bi;
+                const int gi = i0 - 1 + bi, gj = j0 - 1 + bj;
+#pragma unroll
+                for (int c = 0; c < 5; c++) {
+                    const bool z = zk || gi + (d ? 0 : c) > n0 - 1 || gj + (d ? c : 0) > n1 - 3;
+                    o[c * rs] = z ? (T)0 : r[c];
+                }
+            }
+            __syncthreads();
+        }
         if (act) {
             const int xs[2] = {i, j};
             // ---- in-plane terms of output P
@@ -1465,8 +1756,25 @@ k_hyper_zs_f(Geo g, ZMap zm, Phys ph, const float *__restrict__ uin, const float
     __shared__ float sIz[ZS_KZMAX]; // 1/dz of the output planes
     __shared__ float sCx[8][HB_TI], sCy[8][HB_TJ], sCz[7][ZS_KZMAX];
     __shared__ float sKx[4][HB_TI], sKy[4][HB_TJ], sKz[4][ZS_KZMAX];
-    hyper_zs_body<BGP, float>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRf, sR, sX, sIz, sCx,
-                              sCy, sCz, sKx, sKy, sKz);
+    hyper_zs_body<BGP, false, float>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRf, sR, sX,
+                                     sIz, sCx, sCy, sCz, sKx, sKy, sKz, nullptr);
+}
+
+// the same sources with the in-plane ratio pairs formed in the kernel from
+// the ratio box; R holds the pairs along k (k_ratio_kz_f)
+template <bool BGP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void
+k_hyper_zsr_f(Geo g, ZMap zm, Phys ph, const float *__restrict__ uin, const float *__restrict__ R,
+              const float *__restrict__ aux, const double *__restrict__ bg,
+              const unsigned long long *__restrict__ cmb, float *__restrict__ dUh)
+{
+    __shared__ float sM[3][HB_NB], sR[HZ_NIP][HB_NB], sX[8][HB_NB], sQ[8][QB_NB];
+    __shared__ float sRf[HB_NB];
+    __shared__ float sIz[ZS_KZMAX];
+    __shared__ float sCx[8][HB_TI], sCy[8][HB_TJ], sCz[7][ZS_KZMAX];
+    __shared__ float sKx[4][HB_TI], sKy[4][HB_TJ], sKz[4][ZS_KZMAX];
+    hyper_zs_body<BGP, true, float>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sM, nullptr, sRf, sR,
+                                    sX, sIz, sCx, sCy, sCz, sKx, sKy, sKz, sQ);
 }
 
 template <bool BGP>
@@ -1481,8 +1789,24 @@ k_hyper_zs_d(Geo g, ZMap zm, Phys ph, const double *__restrict__ uin,
     __shared__ double sIz[ZS_KZMAX];
     __shared__ double sCx[8][HB_TI], sCy[8][HB_TJ], sCz[7][ZS_KZMAX];
     __shared__ double sKx[4][HB_TI], sKy[4][HB_TJ], sKz[4][ZS_KZMAX];
-    hyper_zs_body<BGP, double>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRf, sR, sX, sIz, sCx,
-                               sCy, sCz, sKx, sKy, sKz);
+    hyper_zs_body<BGP, false, double>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRf, sR, sX,
+                                      sIz, sCx, sCy, sCz, sKx, sKy, sKz, nullptr);
+}
+
+template <bool BGP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1))) void
+k_hyper_zsr_d(Geo g, ZMap zm, Phys ph, const double *__restrict__ uin,
+              const double *__restrict__ R, const double *__restrict__ aux,
+              const double *__restrict__ bg, const unsigned long long *__restrict__ cmb,
+              double *__restrict__ dUh)
+{
+    __shared__ double sM[3][HB_NB], sR[HZ_NIP][HB_NB], sX[8][HB_NB], sQ[8][QB_NB];
+    __shared__ double sRf[HB_NB];
+    __shared__ double sIz[ZS_KZMAX];
+    __shared__ double sCx[8][HB_TI], sCy[8][HB_TJ], sCz[7][ZS_KZMAX];
+    __shared__ double sKx[4][HB_TI], sKy[4][HB_TJ], sKz[4][ZS_KZMAX];
+    hyper_zs_body<BGP, true, double>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sM, nullptr, sRf, sR,
+                                     sX, sIz, sCx, sCy, sCz, sKx, sKy, sKz, sQ);
 }
 
 // the hyperdiffusive sources of every interior cell into dUh (the stage

@@ -254,11 +254,15 @@ class Engine:
         return dict(zip(self.PASSES, list(ms)))
 
     def set_ratio_form(self, zstreamed, kz=0):
-        """3-D: z-streamed hyperviscosity ratios (default) or per cell"""
+        """3-D: hyperviscosity ratios z-streamed (1), per cell (0), or
+        z-streamed along k with the in-plane pairs formed by the z-streamed
+        source kernel (2, the default)"""
         self._check(self.lib.smaug_set_ratio_form(self.h, int(zstreamed), kz))
 
     def get_ratio_form(self):
-        """(1 if the 3-D ratio pass is z-streamed, else 0; its z-stream depth)"""
+        """(form of the 3-D ratio pass: 0 per cell, 1 z-streamed, 2 z-streamed
+        pairs along k with the in-plane pairs formed by the source kernel; its
+        z-stream depth)"""
         zs, kz = ctypes.c_int(), ctypes.c_int()
         self._check(self.lib.smaug_get_ratio_form(self.h, ctypes.byref(zs), ctypes.byref(kz)))
         return zs.value, kz.value

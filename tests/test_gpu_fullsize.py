@@ -64,7 +64,8 @@ def test_large_grid_zstream_matches_per_cell_mixed():
     """the large stratified atmosphere at its own size, 768 x 512 x 512
     interior cells over 12 x 8 x 8 Mm (201 M cells, 62 GB per engine in
     mixed precision), with the launch shape its timed runs use (update and
-    sources at z-stream depth 64, ratios at 8): two mixed RK4 steps of the
+    sources at z-stream depth 64, the ratios along k at depth 8 and the
+    in-plane ones formed by the source kernel): two mixed RK4 steps of the
     z-streamed kernels against the per-cell kernels"""
     s = configs.stratified(nz=768, nx=512, ny=512, lz=12.0e6, lx=8.0e6, ly=8.0e6)
     print(f"[case] large grid {s.n} mixed, z-streamed vs per-cell", flush=True)
@@ -85,7 +86,7 @@ def test_large_grid_zstream_matches_per_cell_mixed():
             e.set_bg_plane(False)
         else:
             e.set_zstream(True, 64)
-            e.set_ratio_form(True, 8)
+            e.set_ratio_form(2, 8)
         e.set_state(w)
         e.rk4(1.0, s.dt, 2)
         out[form] = e.get_state()
