@@ -1573,12 +1573,13 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                 for (int x = 0; x < 10; x++) v[x] = q[x * qs];
                 ratio6x5<T>(v, r);
                 T *o = sR[rin_slot(d, a)] + bj * HB_BI + bi;
-                const int gi = i0 - 1 + bi, gj = j0 - 1 + bj;
+                // cells c of the item with a ratio: gi = gi0 + c (along i) or
+                // gj = gj0 + c (along j) within the limits
+                const int gi0 = i0 - 1 + bi, gj0 = j0 - 1 + bj;
+                const int lim = d ? n1 - 3 - gj0 : n0 - 1 - gi0;
+                const bool ok = !zk && (d ? gi0 <= n0 - 1 : gj0 <= n1 - 3);
 #pragma unroll
-                for (int c = 0; c < 5; c++) {
-                    const bool z = zk || gi + (d ? 0 : c) > n0 - 1 || gj + (d ? c : 0) > n1 - 3;
-                    o[c * rs] = z ? (T)0 : r[c];
-                }
+                for (int c = 0; c < 5; c++) o[c * rs] = (ok && c <= lim) ? r[c] : (T)0;
             }
             __syncthreads();
         }
