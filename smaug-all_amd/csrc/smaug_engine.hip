@@ -229,12 +229,14 @@ static TileMap make_map(int e0, int e1, int e2, int tpx, int tpy, unsigned *nblo
     return m;
 }
 
-// tiles and z-chunks of the ratio box [1, n-3] for the z-streamed ratio kernel
-static ZMap make_rmap(const smaug_engine *e, unsigned *nblocks)
+// tiles and z-chunks of the ratio box [1, n-3] for the z-streamed ratio
+// kernels: tiles of RZ_TI x tj columns (RZ_TJ for k_ratio_zs, KZ_TJ for
+// k_ratio_kz)
+static ZMap make_rmap(const smaug_engine *e, unsigned *nblocks, int tj = RZ_TJ)
 {
     ZMap m;
     m.ntx = (e->n[0] + RZ_TI - 1) / RZ_TI;
-    m.nty = (e->n[1] - 3 + RZ_TJ - 1) / RZ_TJ;
+    m.nty = (e->n[1] - 3 + tj - 1) / tj;
     const int nk = e->n[2] - 3;
     m.kz = e->kzr > 0 ? (e->kzr < nk ? e->kzr : nk) : (nk < 32 ? nk : 32);
     m.nkz = (nk + m.kz - 1) / m.kz;
@@ -257,14 +259,15 @@ static int launch_ratio(smaug_engine *e, const void *u)
     if constexpr (D == 3) {
         if (e->ratio_zs) {
             unsigned nb;
-            ZMap m = make_rmap(e, &nb);
             const bool kz = ratio_in_sources(e);
+            ZMap m = make_rmap(e, &nb, kz ? KZ_TJ : RZ_TJ);
+            const int nt = kz ? KZ_NT : RZ_NT;
             if constexpr (sizeof(T) == 4)
-                hipLaunchKernelGGL(kz ? k_ratio_kz_f : k_ratio_zs_f, dim3(nb), dim3(RZ_NT), 0,
+                hipLaunchKernelGGL(kz ? k_ratio_kz_f : k_ratio_zs_f, dim3(nb), dim3(nt), 0,
                                    e->stream, e->geo, m, (const T *)u, (const T *)e->aux_in,
                                    (T *)e->R);
             else
-                hipLaunchKernelGGL(kz ? k_ratio_kz_d : k_ratio_zs_d, dim3(nb), dim3(RZ_NT), 0,
+                hipLaunchKernelGGL(kz ? k_ratio_kz_d : k_ratio_zs_d, dim3(nb), dim3(nt), 0,
                                    e->stream, e->geo, m, (const T *)u, (const T *)e->aux_in,
                                    (T *)e->R);
             HIPCHK(hipGetLastError());

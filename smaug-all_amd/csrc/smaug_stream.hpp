@@ -717,8 +717,11 @@ __global__ __launch_bounds__(RZ_NT) void k_ratio_zs_d(Geo g, ZMap zm, const doub
 // cells, blocks and arithmetic are those of k_ratio_zs: each thread streams
 // its own column through planes k0-2 .. k1+2 (mirrored pads along k) with
 // the last six planes in registers, and writes the ratios of the plane three
-// behind the front.  No LDS: the column is the only neighbourhood.
+// behind the front.  No LDS: the column is the only neighbourhood, so the
+// blocks are 64 x KZ_TJ columns, smaller than k_ratio_zs's, for more blocks
+// at a given depth.
 // ---------------------------------------------------------------------------
+constexpr int KZ_TJ = 4, KZ_NT = RZ_TI * KZ_TJ;
 template <typename T>
 __device__ __forceinline__ void ratio_kz_body(const Geo &g, const ZMap &zm,
                                               const T *__restrict__ u,
@@ -731,7 +734,7 @@ __device__ __forceinline__ void ratio_kz_body(const Geo &g, const ZMap &zm,
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     const size_t nc = (size_t)g.nc;
     const ci_t s1 = (ci_t)g.s1, s2 = (ci_t)g.s2;
-    const int i = tx * RZ_TI + threadIdx.x % RZ_TI, j = 1 + ty * RZ_TJ + threadIdx.x / RZ_TI;
+    const int i = tx * RZ_TI + threadIdx.x % RZ_TI, j = 1 + ty * KZ_TJ + threadIdx.x / RZ_TI;
     if (i > n0 - 1 || j > n1 - 3) return;
     const int k0 = 1 + kc * zm.kz;
     const int k1 = min(k0 + zm.kz, n2 - 2);
@@ -787,14 +790,14 @@ __device__ __forceinline__ void ratio_kz_body(const Geo &g, const ZMap &zm,
     }
 }
 
-__global__ __launch_bounds__(RZ_NT) void k_ratio_kz_f(Geo g, ZMap zm, const float *__restrict__ u,
+__global__ __launch_bounds__(KZ_NT) void k_ratio_kz_f(Geo g, ZMap zm, const float *__restrict__ u,
                                                        const float *__restrict__ aux,
                                                        float *__restrict__ R)
 {
     ratio_kz_body<float>(g, zm, u, aux, R);
 }
 
-__global__ __launch_bounds__(RZ_NT) void k_ratio_kz_d(Geo g, ZMap zm, const double *__restrict__ u,
+__global__ __launch_bounds__(KZ_NT) void k_ratio_kz_d(Geo g, ZMap zm, const double *__restrict__ u,
                                                        const double *__restrict__ aux,
                                                        double *__restrict__ R)
 {
