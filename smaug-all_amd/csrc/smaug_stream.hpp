@@ -979,7 +979,10 @@ __global__ __launch_bounds__(256) void k_hyper_lds_d(Geo g, TileMap tm, Phys ph,
 // along the stream, so every history is private to its thread.  The next
 // plane's loads are issued before the current plane is processed.  The
 // arithmetic of each term is that of hyper_terms; only the summation order
-// differs.
+// differs.  k_hyper_zsr forms the 14 in-plane ratio pairs of each plane in
+// the kernel, from the state of a wider box staged in LDS, instead of
+// loading them from R (which then holds the 7 pairs along k); the ratios
+// are bit for bit those of the ratio kernels.
 // ---------------------------------------------------------------------------
 constexpr int HZ_NIP = 14; // in-plane ratio pairs staged in LDS (every pair with d < 2)
 
