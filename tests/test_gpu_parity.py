@@ -224,3 +224,41 @@ def test_mixed_rk4_200_steps_vs_oracle_stratified():
     assert np.all(np.isfinite(got))
     bound = np.array([1.5e-6, 1.0e-6, 6e-6, 6e-6, 1.5e-6, 6e-6, 6e-6, 6e-6])
     assert np.all(l2 < bound), l2
+
+
+@pytest.fixture(scope="module")
+def strat64_100_steps():
+    """a 64 x 32 x 32 stratified grid (the workload's cell sizes and dt,
+    driver on, a seeded perturbation of every field) and the FP64 oracle's
+    state after 100 RK4 steps"""
+    full = configs.stratified(nz=256, nx=8, ny=8, lx=4e6 * 8 / 256, ly=4e6 * 8 / 256)
+    s = configs.stratified(nz=64, nx=32, ny=32, lz=6e6 * 64 / 256, lx=4e6 * 32 / 256,
+                           ly=4e6 * 32 / 256)
+    s.dt = full.dt
+    o = Oracle(s)
+    w0 = o.boundary(perturbed(s, amp=1e-3, seed=41))
+    ref, _ = o.run(w0, 0.0, s.dt, 100, rk4=True)
+    return s, w0, ref
+
+
+@pytest.mark.parametrize("shape", ["default", "kz16_r8"])
+def test_fp64_rk4_100_steps_vs_oracle_stratified(strat64_100_steps, shape):
+    """the FP64 path after 100 RK4 steps against the FP64 oracle, in the
+    default launch shapes (the in-plane ratios formed by the source kernel)
+    and with two z-chunks for the update and sources and ratio chunks of 8
+    planes.  Measured on MI355X: 3.3e-15 of each field's amplitude at most
+    (L-infinity, every cell); the bound is 1e-12."""
+    s, w0, ref = strat64_100_steps
+    print(f"[case] fp64 vs oracle, 100 RK4 steps, n={s.n}, shape={shape}", flush=True)
+    e = Engine(s, FP64)
+    if shape != "default":
+        e.set_zstream(1, 16)
+        e.set_ratio_form(2, 8)
+    assert e.get_ratio_form()[0] == 2
+    e.set_state(w0)
+    e.rk4(0.0, s.dt, 100)
+    got = e.get_state()
+    e.close()
+    err = rel_err(got, ref)
+    print("fp64 vs oracle after 100 steps, rel Linf per field:", err, flush=True)
+    assert np.all(err < 1e-12), err
