@@ -262,3 +262,51 @@ def test_fp64_rk4_100_steps_vs_oracle_stratified(strat64_100_steps, shape):
     err = rel_err(got, ref)
     print("fp64 vs oracle after 100 steps, rel Linf per field:", err, flush=True)
     assert np.all(err < 1e-12), err
+
+
+@pytest.fixture(scope="module")
+def ot128_runs():
+    """the Orszag-Tang vortex on 128 x 128 cells (periodic, dt = 1e-3): the
+    FP64 oracle's states after 20 and 200 RK4 steps, and its own spread after
+    200 steps -- the change a seeded 1e-15 relative perturbation of the
+    initial state makes.  The vortex amplifies round-off about 1e9-fold over
+    200 steps (measured spread: 8.8e-13 at 20 steps, 3.4e-6 at 200)."""
+    s = configs.orszag_tang(nx=128)
+    o = Oracle(s)
+    w0 = o.boundary(s.w)
+    ref20, _ = o.run(w0, 0.0, 1e-3, 20, rk4=True)
+    ref200, _ = o.run(w0, 0.0, 1e-3, 200, rk4=True)
+    rng = np.random.default_rng(1)
+    w1 = o.boundary(w0 * (1 + 1e-15 * rng.standard_normal(w0.shape)))
+    alt200, _ = o.run(w1, 0.0, 1e-3, 200, rk4=True)
+    return s, w0, ref20, ref200, rel_err(alt200, ref200)
+
+
+@pytest.mark.parametrize("prec", [FP64, MIXED], ids=["fp64", "mixed"])
+def test_rk4_long_vs_oracle_orszag_tang(ot128_runs, prec):
+    """the 2-D per-cell path after 20 and 200 RK4 steps of the Orszag-Tang
+    vortex against the FP64 oracle, per field, normalised by the field's
+    amplitude (L-infinity, every cell).  FP64: to 1e-11 after 20 steps, and
+    within the oracle's own round-off spread after 200.  Mixed: the same
+    amplification acts on FP32's rounding (6e-8 per stored value); measured
+    on MI355X 2.3e-6 after 20 steps and 4.1e-4 after 200, bounded at 1e-5
+    and 2e-3."""
+    s, w0, ref20, ref200, spread = ot128_runs
+    print(f"[case] orszag-tang 128^2, 20 and 200 RK4 steps, prec={prec}", flush=True)
+    e = Engine(s, prec)
+    e.set_state(w0)
+    e.rk4(0.0, 1e-3, 20)
+    got20 = e.get_state()
+    e.rk4(20e-3, 1e-3, 180)
+    got200 = e.get_state()
+    e.close()
+    err20, err200 = rel_err(got20, ref20), rel_err(got200, ref200)
+    print("orszag-tang rel Linf per field after 20 steps:", err20, flush=True)
+    print("after 200 steps:", err200, "oracle spread:", spread, flush=True)
+    assert np.all(np.isfinite(got200))
+    if prec == FP64:
+        assert np.all(err20 < 1e-11), err20
+        assert np.all(err200 <= spread.max()), (err200, spread)
+    else:
+        assert np.all(err20 < 1e-5), err20
+        assert np.all(err200 < 2e-3), err200
