@@ -343,8 +343,8 @@ static ZMap make_zmap(const smaug_engine *e, unsigned *nblocks)
 static ZMap make_hmap(const smaug_engine *e, unsigned *nblocks)
 {
     ZMap m;
-    m.ntx = (e->n[0] - 4 + HB_TI - 1) / HB_TI;
-    m.nty = (e->n[1] - 4 + HB_TJ - 1) / HB_TJ;
+    m.ntx = (e->n[0] - 4 + HZ_TI - 1) / HZ_TI;
+    m.nty = (e->n[1] - 4 + HZ_TJ - 1) / HZ_TJ;
     const int nk = e->n[2] - 4;
     m.kz = e->kzh > 0 ? (e->kzh < nk ? e->kzh : nk) : (nk < 32 ? nk : 32);
     if (m.kz > ZS_KZMAX) m.kz = ZS_KZMAX;

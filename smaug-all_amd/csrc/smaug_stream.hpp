@@ -963,7 +963,7 @@ __global__ __launch_bounds__(256) void k_hyper_lds_d(Geo g, TileMap tm, Phys ph,
 }
 
 // ---------------------------------------------------------------------------
-// Hyperdiffusive sources, z-streamed.  A block owns 64 x 4 columns and walks
+// Hyperdiffusive sources, z-streamed.  A block owns 32 x 8 columns and walks
 // a chunk of output planes.  Per plane P (the "centre") the state, AUX,
 // rho_b and the 14 in-plane ratio pairs of the tile + 1-cell halo are staged
 // in LDS, and:
@@ -986,6 +986,26 @@ __global__ __launch_bounds__(256) void k_hyper_lds_d(Geo g, TileMap tm, Phys ph,
 // ---------------------------------------------------------------------------
 constexpr int HZ_NIP = 14; // in-plane ratio pairs staged in LDS (every pair with d < 2)
 
+// tiles of 32 x 8 columns; the box of the tile and its 1-cell halo is 34 x 10
+constexpr int HZ_TI = 32, HZ_TJ = 8;
+constexpr int HZ_BI = HZ_TI + 2, HZ_BJ = HZ_TJ + 2, HZ_NB = HZ_BI * HZ_BJ;
+constexpr int HZ_NH = HZ_NB - HZ_TI * HZ_TJ;
+static_assert(HZ_TI * HZ_TJ == 256 && HZ_NH <= 256, "one own column and at most one halo column per thread");
+
+// box coordinates of halo cell h of the 34 x 10 box: rows 0 and 9 in full,
+// then columns 0 and 33 of rows 1..8
+__device__ __forceinline__ void hz_halo(int h, int &bi, int &bj)
+{
+    if (h < 2 * HZ_BI) {
+        bj = h < HZ_BI ? 0 : HZ_BJ - 1;
+        bi = h < HZ_BI ? h : h - HZ_BI;
+    } else {
+        const int q = h - 2 * HZ_BI;
+        bj = 1 + (q >> 1);
+        bi = (q & 1) ? HZ_BI - 1 : 0;
+    }
+}
+
 __host__ __device__ constexpr int hz_islot(int p)
 {
     // pairs along d = 2 (3-D numbering): 2, 5, 8, 11, 14, 16, 18
@@ -1002,17 +1022,17 @@ struct HzCell {
 };
 
 // In-plane ratios formed in the source kernel (RIN): the state that ratio6
-// reads -- rho', e_int, v_0..2, b'_0..2 -- staged on the 70 x 10 box of
-// cells i0-3 .. i0+66, j0-3 .. j0+6 (pads mirrored as in k_ratio_zs), which
-// holds the 66 x 6 box at offset (2, 2).  The pairs along i are formed on the
-// 65 x 6 box cells that read them (box columns 0..64), those along j on the
-// 66 x 5 cells (box rows 0..4).
-constexpr int QB_BI = HB_BI + 4, QB_BJ = HB_BJ + 4, QB_NB = QB_BI * QB_BJ;
-constexpr int QB_NX = QB_NB - HB_NB;       // ratio-box cells outside the 66 x 6 box
+// reads -- rho', e_int, v_0..2, b'_0..2 -- staged on the 38 x 14 box of
+// cells i0-3 .. i0+34, j0-3 .. j0+10 (pads mirrored as in k_ratio_zs), which
+// holds the 34 x 10 box at offset (2, 2).  The pairs along i are formed on the
+// 33 x 10 box cells that read them (box columns 0..32), those along j on the
+// 34 x 9 cells (box rows 0..8).
+constexpr int QB_BI = HZ_BI + 4, QB_BJ = HZ_BJ + 4, QB_NB = QB_BI * QB_BJ;
+constexpr int QB_NX = QB_NB - HZ_NB;       // ratio-box cells outside the 34 x 10 box
 constexpr int QB_PX = (QB_NX + 255) / 256; // of them per thread
 
-// ratio-box coordinates of the cells outside the 66 x 6 box: rows 0, 1, 8, 9
-// in full, then columns 0, 1, 68, 69 of rows 2..7
+// ratio-box coordinates of the cells outside the 34 x 10 box: rows 0, 1, 12,
+// 13 in full, then columns 0, 1, 36, 37 of rows 2..11
 __device__ __forceinline__ void qb_extra(int e, int &qi, int &qj)
 {
     if (e < 4 * QB_BI) {
@@ -1035,6 +1055,52 @@ __device__ __forceinline__ int rin_field(int d, int a)
 __device__ __forceinline__ int rin_slot(int d, int a)
 {
     return a < 5 ? 2 * a + d : (d == 0 ? 6 + a : (a == 5 ? 10 : 13));
+}
+
+// runs of the items along i: the 33 box cells of a row in 3 runs of 11
+constexpr int HZ_RL = 11, HZ_NRUN = (HZ_BI - 1) / HZ_RL;
+static_assert(HZ_RL * HZ_NRUN == HZ_BI - 1, "the runs cover box columns 0..HZ_BI-2");
+
+// R of the L windows q[c..c+5] (c = 0..L-1) of the L + 5 values q[x st],
+// sharing their differences and partial maxima; each equals ratio6 of its
+// window.  The maxima are over the same values as ratio6's, in another
+// order: fmax ignores a NaN operand and the values are non-negative, so the
+// order does not change the result.  The D3 maxima keep ratio6's starting
+// 0 (as fmax(t, 0) per difference); for the D1 maxima ratio_fin's clamp
+// (FP32) or its test m1 > 0 (FP64) gives what the starting 0 does.
+template <int L, typename T>
+__device__ __forceinline__ void ratio_run(const T *q, int st, T r[L])
+{
+    T v[L + 5], d[L + 4], t[L + 3], a[L + 2];
+#pragma unroll
+    for (int x = 0; x < L + 5; x++) v[x] = q[x * st];
+#pragma unroll
+    for (int x = 0; x < L + 4; x++) d[x] = v[x + 1] - v[x];
+    // D3 at the centre differences 1..L+2
+#pragma unroll
+    for (int x = 1; x < L + 3; x++) t[x] = fmax(d3abs(d[x], v[x + 2], v[x - 1]), (T)0);
+    // max |d[x..x+2]|
+#pragma unroll
+    for (int x = 0; x < L + 2; x++) a[x] = fmax(fmax(fabs(d[x]), fabs(d[x + 1])), fabs(d[x + 2]));
+#pragma unroll
+    for (int c = 0; c < L; c++) {
+        const T m1 = fmax(fmax(a[c], fabs(d[c + 3])), fabs(d[c + 4]));
+        const T m3 = fmax(fmax(t[c + 1], t[c + 2]), t[c + 3]);
+        r[c] = ratio_fin(m1, m3);
+    }
+}
+
+// r[c] to o[c st] for c <= lim, 0 beyond
+template <int L, typename T>
+__device__ __forceinline__ void ratio_store(T *o, int st, const T r[L], int lim)
+{
+    if (lim >= L - 1) {
+#pragma unroll
+        for (int c = 0; c < L; c++) o[c * st] = r[c];
+    } else {
+#pragma unroll
+        for (int c = 0; c < L; c++) o[c * st] = c <= lim ? r[c] : (T)0;
+    }
 }
 
 // all 21 ratio pairs for the own column (full = true); a halo column needs
@@ -1075,13 +1141,13 @@ __device__ __forceinline__ void hz_load(HzCell<T> &h, const T *__restrict__ u,
 // RIN: rho', e_int, v and b' are read from the ratio box sQ, m from sU.
 template <typename T, bool RIN>
 struct HzView {
-    const T (*sU)[HB_NB]; // rho', m_0..2, b'_0..2 (RIN: m_0..2)
-    const T (*sA)[HB_NB]; // e_int, v_0..2 (not with RIN)
+    const T (*sU)[HZ_NB]; // rho', m_0..2, b'_0..2 (RIN: m_0..2)
+    const T (*sA)[HZ_NB]; // e_int, v_0..2 (not with RIN)
     const T (*sQ)[QB_NB]; // RIN: rho', e_int, v_0..2, b'_0..2
     const T *sRf; // full density rho' + rho_b (formed in double, stored in T)
-    const T (*sR)[HB_NB];
+    const T (*sR)[HZ_NB];
     int own, ownq;
-    __device__ __forceinline__ int bx(int di, int dj) const { return own + di + HB_BI * dj; }
+    __device__ __forceinline__ int bx(int di, int dj) const { return own + di + HZ_BI * dj; }
     __device__ __forceinline__ int qx(int di, int dj) const { return ownq + di + QB_BI * dj; }
     __device__ __forceinline__ T U(int f, int di, int dj) const
     {
@@ -1114,11 +1180,11 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
                                               const T *__restrict__ aux,
                                               const double *__restrict__ bg,
                                               const unsigned long long *__restrict__ cmb,
-                                              T *__restrict__ dUh, T (*sU)[HB_NB],
-                                              T (*sA)[HB_NB], T *sRf, T (*sR)[HB_NB],
-                                              T (*sX)[HB_NB], T *sIz, T (*sCx)[HB_TI],
-                                              T (*sCy)[HB_TJ], T (*sCz)[ZS_KZMAX],
-                                              T (*sKx)[HB_TI], T (*sKy)[HB_TJ], T (*sKz)[ZS_KZMAX],
+                                              T *__restrict__ dUh, T (*sU)[HZ_NB],
+                                              T (*sA)[HZ_NB], T *sRf, T (*sR)[HZ_NB],
+                                              T (*sX)[HZ_NB], T *sIz, T (*sCx)[HZ_TI],
+                                              T (*sCy)[HZ_TJ], T (*sCz)[ZS_KZMAX],
+                                              T (*sKx)[HZ_TI], T (*sKy)[HZ_TJ], T (*sKz)[ZS_KZMAX],
                                               T (*sQ)[QB_NB])
 {
     typedef Lay<3> LY;
@@ -1128,8 +1194,8 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     const size_t nc = (size_t)g.nc;
     const ci_t s1 = (ci_t)g.s1, s2 = (ci_t)g.s2;
-    const int ti = threadIdx.x % HB_TI, tj = threadIdx.x / HB_TI;
-    const int i0 = 2 + tx * HB_TI, j0 = 2 + ty * HB_TJ;
+    const int ti = threadIdx.x % HZ_TI, tj = threadIdx.x / HZ_TI;
+    const int i0 = 2 + tx * HZ_TI, j0 = 2 + ty * HZ_TJ;
     const int i = i0 + ti, j = j0 + tj;
     const bool act = i <= n0 - 3 && j <= n1 - 3;
     const int k0 = 2 + kc * zm.kz;
@@ -1178,10 +1244,10 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
             sKz[2 + f][t] = (T)(dz * ph.chyp[LY::B0 + f]);
         }
     }
-    if (threadIdx.x < HB_TI + HB_TJ) {
-        const bool alongi = threadIdx.x < HB_TI;
+    if (threadIdx.x < HZ_TI + HZ_TJ) {
+        const bool alongi = threadIdx.x < HZ_TI;
         const int a = alongi ? 0 : 1;
-        const int x = alongi ? i0 + threadIdx.x : j0 + (threadIdx.x - HB_TI);
+        const int x = alongi ? i0 + threadIdx.x : j0 + (threadIdx.x - HZ_TI);
         const bool in = x <= g.n[a] - 3;
         const double dd = in ? g.dx[a][x] * cmax[a] : 0.0;
         const T ix = in ? (T)g.idx[a][x] : (T)0;
@@ -1192,7 +1258,7 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
             if (alongi)
                 sCx[q][threadIdx.x] = c;
             else
-                sCy[q][threadIdx.x - HB_TI] = c;
+                sCy[q][threadIdx.x - HZ_TI] = c;
         }
         const int o = 1 - a; // the other in-plane axis
         const int KF[4] = {LY::M0 + o, LY::B0 + o, LY::M0 + 2, LY::B0 + 2};
@@ -1202,7 +1268,7 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
             if (alongi)
                 sKx[q][threadIdx.x] = kq;
             else
-                sKy[q][threadIdx.x - HB_TI] = kq;
+                sKy[q][threadIdx.x - HZ_TI] = kq;
         }
     }
     __syncthreads();
@@ -1223,21 +1289,21 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
             bj = tj + 1;
             bv[q] = true;
         } else {
-            bv[q] = threadIdx.x < HB_NH;
+            bv[q] = threadIdx.x < HZ_NH;
             bi = bj = 0;
-            if (bv[q]) hb_halo(threadIdx.x, bi, bj);
+            if (bv[q]) hz_halo(threadIdx.x, bi, bj);
         }
         pq[q] = bv[q];
         const int gi = i0 - 1 + bi, gj = j0 - 1 + bj;
         bv[q] = bv[q] && gi <= n0 - 1 && gj <= n1 - 1;
         bi_[q] = gi;
         bj_[q] = gj;
-        bb[q] = bj * HB_BI + bi;
+        bb[q] = bj * HZ_BI + bi;
         bc[q] = (ci_t)gi + s1 * (ci_t)gj;
         bq[q] = (bj + 2) * QB_BI + bi + 2;
         cqa[q] = (ci_t)pad_map(gi, n0) + s1 * (ci_t)pad_map(gj, n1);
     }
-    // RIN: the ratio-box cells outside the 66 x 6 box that this thread loads
+    // RIN: the ratio-box cells outside the 34 x 10 box that this thread loads
     bool ev[QB_PX];
     int eq[QB_PX];
     ci_t ce[QB_PX];
@@ -1547,45 +1613,35 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
         __syncthreads();
         if constexpr (RIN) {
             // the in-plane ratio pairs of plane P on the box cells that read
-            // them, from the ratio box, five cells of a row or column per
-            // item; 0 where k_ratio_zs writes none (rows above n1-3, planes
-            // above n2-3, cells off the grid).  Items: the 66 box columns
-            // along j (cells in rows 0..4), then the 13 runs of five cells
-            // of each box row along i (columns 0..64), pair by pair.
+            // them, from the ratio box; 0 where k_ratio_zs writes none (rows
+            // above n1-3, planes above n2-3, cells off the grid).  Items:
+            // pair a of each of the 34 box columns along j (the column's 14
+            // values give the pairs of its cells in box rows 0..8), then
+            // pair a of each of the 10 box rows along i, in runs of HZ_RL
+            // cells (box columns 0..32).  Iteration 0 of the 256 threads
+            // takes the 238 items along j and the first 18 along i, so one
+            // wave runs both kinds; iteration 1 the other 192.
             const bool zk = P > n2 - 3;
-            constexpr int NRUN = (HB_BI - 1) / 5;
-            constexpr int NI1 = 7 * HB_BI, NI0 = 7 * HB_BJ * NRUN;
-            for (int u = threadIdx.x; u < NI1 + NI0; u += 256) {
-                int d, a, bi, bj;
-                if (u < NI1) {
-                    d = 1;
-                    a = u / HB_BI;
-                    bi = u - a * HB_BI;
-                    bj = 0;
+            constexpr int NJ = 7 * HZ_BI, NI = 7 * HZ_BJ * HZ_NRUN;
+            for (int u = threadIdx.x; u < NJ + NI; u += 256) {
+                if (u < NJ) {
+                    const int a = u / HZ_BI, bi = u - a * HZ_BI;
+                    T r[HZ_BJ - 1];
+                    ratio_run<HZ_BJ - 1, T>(sQ[rin_field(1, a)] + bi + 2, QB_BI, r);
+                    // cell c at gj = j0 - 1 + c of the column gi = i0 - 1 + bi
+                    const int lim = zk || i0 - 1 + bi > n0 - 1 ? -1 : n1 - 2 - j0;
+                    ratio_store<HZ_BJ - 1, T>(sR[rin_slot(1, a)] + bi, HZ_BI, r, lim);
                 } else {
-                    d = 0;
-                    const int x = u - NI1;
-                    a = x / (HB_BJ * NRUN);
-                    const int y = x - a * (HB_BJ * NRUN);
-                    bj = y / NRUN;
-                    bi = 5 * (y - bj * NRUN);
+                    const int x = u - NJ;
+                    const int a = x / (HZ_BJ * HZ_NRUN);
+                    const int y = x - a * (HZ_BJ * HZ_NRUN);
+                    const int bj = y / HZ_NRUN, bi = HZ_RL * (y - bj * HZ_NRUN);
+                    T r[HZ_RL];
+                    ratio_run<HZ_RL, T>(sQ[rin_field(0, a)] + (bj + 2) * QB_BI + bi, 1, r);
+                    // cell c at gi = i0 - 1 + bi + c of the row gj = j0 - 1 + bj
+                    const int lim = zk || j0 - 1 + bj > n1 - 3 ? -1 : n0 - i0 - bi;
+                    ratio_store<HZ_RL, T>(sR[rin_slot(0, a)] + bj * HZ_BI + bi, 1, r, lim);
                 }
-                // first window value: row 0 of column bi + 2 (along j), or
-                // column bi of row bj + 2 (along i)
-                const int qs = d ? QB_BI : 1, rs = d ? HB_BI : 1;
-                const T *q = sQ[rin_field(d, a)] + (d ? bi + 2 : (bj + 2) * QB_BI + bi);
-                T v[10], r[5];
-#pragma unroll
-                for (int x = 0; x < 10; x++) v[x] = q[x * qs];
-                ratio6x5<T>(v, r);
-                T *o = sR[rin_slot(d, a)] + bj * HB_BI + bi;
-                // cells c of the item with a ratio: gi = gi0 + c (along i) or
-                // gj = gj0 + c (along j) within the limits
-                const int gi0 = i0 - 1 + bi, gj0 = j0 - 1 + bj;
-                const int lim = d ? n1 - 3 - gj0 : n0 - 1 - gi0;
-                const bool ok = !zk && (d ? gi0 <= n0 - 1 : gj0 <= n1 - 3);
-#pragma unroll
-                for (int c = 0; c < 5; c++) o[c * rs] = (ok && c <= lim) ? r[c] : (T)0;
             }
             __syncthreads();
         }
@@ -1724,7 +1780,7 @@ __device__ __forceinline__ void hyper_zs_body(const Geo &g, const ZMap &zm, cons
             // neighbours of an active column lie on the grid
 #pragma unroll
             for (int f = 0; f < 2; f++) {
-                const int sf = f == 0 ? 1 : HB_BI;
+                const int sf = f == 0 ? 1 : HZ_BI;
                 const T h = hf[f];
                 const T Xr = sX[4 * f + 0][own + sf] - sX[4 * f + 0][own - sf];
                 const T Xm = sX[4 * f + 1][own + sf] - sX[4 * f + 1][own - sf];
@@ -1758,11 +1814,11 @@ k_hyper_zs_f(Geo g, ZMap zm, Phys ph, const float *__restrict__ uin, const float
              const float *__restrict__ aux, const double *__restrict__ bg,
              const unsigned long long *__restrict__ cmb, float *__restrict__ dUh)
 {
-    __shared__ float sU[7][HB_NB], sA[4][HB_NB], sR[HZ_NIP][HB_NB], sX[8][HB_NB];
-    __shared__ float sRf[HB_NB];
+    __shared__ float sU[7][HZ_NB], sA[4][HZ_NB], sR[HZ_NIP][HZ_NB], sX[8][HZ_NB];
+    __shared__ float sRf[HZ_NB];
     __shared__ float sIz[ZS_KZMAX]; // 1/dz of the output planes
-    __shared__ float sCx[8][HB_TI], sCy[8][HB_TJ], sCz[7][ZS_KZMAX];
-    __shared__ float sKx[4][HB_TI], sKy[4][HB_TJ], sKz[4][ZS_KZMAX];
+    __shared__ float sCx[8][HZ_TI], sCy[8][HZ_TJ], sCz[7][ZS_KZMAX];
+    __shared__ float sKx[4][HZ_TI], sKy[4][HZ_TJ], sKz[4][ZS_KZMAX];
     hyper_zs_body<BGP, false, float>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRf, sR, sX,
                                      sIz, sCx, sCy, sCz, sKx, sKy, sKz, nullptr);
 }
@@ -1775,11 +1831,11 @@ k_hyper_zsr_f(Geo g, ZMap zm, Phys ph, const float *__restrict__ uin, const floa
               const float *__restrict__ aux, const double *__restrict__ bg,
               const unsigned long long *__restrict__ cmb, float *__restrict__ dUh)
 {
-    __shared__ float sM[3][HB_NB], sR[HZ_NIP][HB_NB], sX[8][HB_NB], sQ[8][QB_NB];
-    __shared__ float sRf[HB_NB];
+    __shared__ float sM[3][HZ_NB], sR[HZ_NIP][HZ_NB], sX[8][HZ_NB], sQ[8][QB_NB];
+    __shared__ float sRf[HZ_NB];
     __shared__ float sIz[ZS_KZMAX];
-    __shared__ float sCx[8][HB_TI], sCy[8][HB_TJ], sCz[7][ZS_KZMAX];
-    __shared__ float sKx[4][HB_TI], sKy[4][HB_TJ], sKz[4][ZS_KZMAX];
+    __shared__ float sCx[8][HZ_TI], sCy[8][HZ_TJ], sCz[7][ZS_KZMAX];
+    __shared__ float sKx[4][HZ_TI], sKy[4][HZ_TJ], sKz[4][ZS_KZMAX];
     hyper_zs_body<BGP, true, float>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sM, nullptr, sRf, sR,
                                     sX, sIz, sCx, sCy, sCz, sKx, sKy, sKz, sQ);
 }
@@ -1791,11 +1847,11 @@ k_hyper_zs_d(Geo g, ZMap zm, Phys ph, const double *__restrict__ uin,
              const double *__restrict__ bg, const unsigned long long *__restrict__ cmb,
              double *__restrict__ dUh)
 {
-    __shared__ double sU[7][HB_NB], sA[4][HB_NB], sR[HZ_NIP][HB_NB], sX[8][HB_NB];
-    __shared__ double sRf[HB_NB];
+    __shared__ double sU[7][HZ_NB], sA[4][HZ_NB], sR[HZ_NIP][HZ_NB], sX[8][HZ_NB];
+    __shared__ double sRf[HZ_NB];
     __shared__ double sIz[ZS_KZMAX];
-    __shared__ double sCx[8][HB_TI], sCy[8][HB_TJ], sCz[7][ZS_KZMAX];
-    __shared__ double sKx[4][HB_TI], sKy[4][HB_TJ], sKz[4][ZS_KZMAX];
+    __shared__ double sCx[8][HZ_TI], sCy[8][HZ_TJ], sCz[7][ZS_KZMAX];
+    __shared__ double sKx[4][HZ_TI], sKy[4][HZ_TJ], sKz[4][ZS_KZMAX];
     hyper_zs_body<BGP, false, double>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRf, sR, sX,
                                       sIz, sCx, sCy, sCz, sKx, sKy, sKz, nullptr);
 }
@@ -1807,11 +1863,11 @@ k_hyper_zsr_d(Geo g, ZMap zm, Phys ph, const double *__restrict__ uin,
               const double *__restrict__ bg, const unsigned long long *__restrict__ cmb,
               double *__restrict__ dUh)
 {
-    __shared__ double sM[3][HB_NB], sR[HZ_NIP][HB_NB], sX[8][HB_NB], sQ[8][QB_NB];
-    __shared__ double sRf[HB_NB];
+    __shared__ double sM[3][HZ_NB], sR[HZ_NIP][HZ_NB], sX[8][HZ_NB], sQ[8][QB_NB];
+    __shared__ double sRf[HZ_NB];
     __shared__ double sIz[ZS_KZMAX];
-    __shared__ double sCx[8][HB_TI], sCy[8][HB_TJ], sCz[7][ZS_KZMAX];
-    __shared__ double sKx[4][HB_TI], sKy[4][HB_TJ], sKz[4][ZS_KZMAX];
+    __shared__ double sCx[8][HZ_TI], sCy[8][HZ_TJ], sCz[7][ZS_KZMAX];
+    __shared__ double sKx[4][HZ_TI], sKy[4][HZ_TJ], sKz[4][ZS_KZMAX];
     hyper_zs_body<BGP, true, double>(g, zm, ph, uin, R, aux, bg, cmb, dUh, sM, nullptr, sRf, sR,
                                      sX, sIz, sCx, sCy, sCz, sKx, sKy, sKz, sQ);
 }
