@@ -454,10 +454,14 @@ def main():
         "step_level_frac": (BYTES_PER_CELL_STEP["mixed"] * cells / (ms_step / 1e3) / 1e9)
         / HBM_PEAK_GBS,
     }
-    from smaugamd.engine import copy_probe
+    from smaugamd.engine import copy_probe, stream_probe
 
     roof["measured_copy_gbs"] = copy_probe()
     roof["frac_of_measured_copy"] = achieved / roof["measured_copy_gbs"]
+    # the streams of a stage update (32 read, 20 written, one FP32 value per
+    # cell each), for the measured traffic of the update pass to be read
+    # against: what HBM delivers to that access pattern
+    roof["measured_stream_gbs"] = stream_probe(cells)
     ens = None
     if not a.no_ensemble:
         ens = ensemble_leg(a.ens_steps, a.warmup, compare=not a.no_ens_compare)

@@ -163,6 +163,12 @@ int smaug_time_passes(smaug_engine *e, double t0, double dt, double ms[5]);
  * copy of nbytes, mean of reps launches, the best over 1, 4 or 8 16-byte
  * loads in flight per thread and 8, 16 or 32 workgroups per CU */
 int smaug_copy_probe(long long nbytes, int reps, double *gbs);
+/* measured many-stream bandwidth: GB/s (read + write bytes) of a kernel that
+ * reads one FP32 value of each of 32 arrays of ncell values and writes one
+ * of each of 20 arrays per cell (the streams of a mixed 3-D stage update),
+ * mean of reps launches, the best over 8, 16 or 32 workgroups per CU.  It
+ * allocates 208 bytes per cell. */
+int smaug_stream_probe(long long ncell, int reps, double *gbs);
 
 /* sizes */
 long long smaug_ncell(const smaug_engine *e);

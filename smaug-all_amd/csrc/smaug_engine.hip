@@ -1578,6 +1578,72 @@ __global__ __launch_bounds__(256) void k_copy_probe(const float4 *__restrict__ a
     }
 }
 
+// many-stream bandwidth: every thread reads one 4-byte value of each of NR
+// arrays at its cell and writes one of each of NW arrays there -- the
+// access pattern of a stage update that reads the 8 fields of four arrays
+// (state, u_n, accumulator, sources) and writes those of two (stage
+// result, accumulator) and 4 aux quantities, one cell per thread
+template <int NR, int NW>
+__global__ __launch_bounds__(256) void k_stream_probe(const float *__restrict__ a,
+                                                      float *__restrict__ b, idx_t n)
+{
+    const idx_t stride = (idx_t)gridDim.x * 256;
+    for (idx_t c = (idx_t)blockIdx.x * 256 + threadIdx.x; c < n; c += stride) {
+        float v[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) v[r] = a[(idx_t)r * n + c];
+        float sum = 0.0f;
+#pragma unroll
+        for (int r = 0; r < NR; r++) sum += v[r];
+#pragma unroll
+        for (int w = 0; w < NW; w++) b[(idx_t)w * n + c] = sum + (float)w;
+    }
+}
+
+constexpr int SP_NR = 32, SP_NW = 20;
+
+extern "C" int smaug_stream_probe(long long ncell, int reps, double *gbs)
+{
+    if (ncell < 256 || reps < 1 || !gbs) return fail(SMAUG_EINVAL, "bad argument");
+    const idx_t n = ncell;
+    float *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc((void **)&a, (size_t)SP_NR * n * sizeof(float)));
+    hipError_t he = hipMalloc((void **)&b, (size_t)SP_NW * n * sizeof(float));
+    if (he != hipSuccess) {
+        hipFree(a);
+        return fail(SMAUG_ENOMEM, "hipMalloc: %s", hipGetErrorString(he));
+    }
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIPCHK(hipMemsetAsync(a, 0, (size_t)SP_NR * n * sizeof(float), st));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double best = 0.0;
+    const unsigned grids[3] = {256 * 8, 256 * 16, 256 * 32}; // workgroups: 8..32 per CU
+    for (int gi = 0; gi < 3; gi++) {
+        const unsigned nb = grids[gi];
+        hipLaunchKernelGGL((k_stream_probe<SP_NR, SP_NW>), dim3(nb), dim3(256), 0, st, a, b, n);
+        HIPCHK(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; r++)
+            hipLaunchKernelGGL((k_stream_probe<SP_NR, SP_NW>), dim3(nb), dim3(256), 0, st, a, b, n);
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipGetLastError());
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        const double g = (double)(SP_NR + SP_NW) * n * sizeof(float) * reps / (ms / 1e3) / 1e9;
+        if (g > best) best = g;
+    }
+    *gbs = best;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipStreamDestroy(st);
+    hipFree(a);
+    hipFree(b);
+    return 0;
+}
+
 extern "C" int smaug_copy_probe(long long nbytes, int reps, double *gbs)
 {
     if (nbytes < 16 || reps < 1 || !gbs) return fail(SMAUG_EINVAL, "bad argument");

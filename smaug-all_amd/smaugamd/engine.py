@@ -23,7 +23,7 @@ MIXED = 1
 EXPORTS = [
     "smaug_last_error", "smaug_create", "smaug_destroy", "smaug_set_state", "smaug_get_state",
     "smaug_rhs", "smaug_cmax", "smaug_maxcourant", "smaug_rk4", "smaug_euler",
-    "smaug_time_rk4", "smaug_ncell", "smaug_ninterior", "smaug_copy_probe",
+    "smaug_time_rk4", "smaug_ncell", "smaug_ninterior", "smaug_copy_probe", "smaug_stream_probe",
     "smaug_set_panel", "smaug_get_panel", "smaug_time_pass", "smaug_autotune",
     "smaug_set_zstream", "smaug_get_zstream", "smaug_time_passes", "smaug_set_ratio_form",
     "smaug_get_ratio_form", "smaug_set_bg_plane", "smaug_get_bg_plane",
@@ -81,6 +81,7 @@ def load_library(path: str = LIB_PATH):
         lib.smaug_ninterior.argtypes = [ctypes.c_void_p]
         lib.smaug_ninterior.restype = ctypes.c_longlong
         lib.smaug_copy_probe.argtypes = [ctypes.c_longlong, ctypes.c_int, dp]
+        lib.smaug_stream_probe.argtypes = [ctypes.c_longlong, ctypes.c_int, dp]
         vp = ctypes.c_void_p
         ip = ctypes.POINTER(ctypes.c_int)
         lib.smaug_set_panel.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -379,6 +380,17 @@ class Ensemble:
         self._check(self.lib.smaug_ens_time_rk4(self.h, _dptr(t0), _dptr(dt), nsteps,
                                                 ctypes.byref(ms)))
         return ms.value
+
+
+def stream_probe(ncell=1 << 24, reps=10):
+    """measured bandwidth (GB/s, read + write) of 32 read and 20 write FP32
+    streams, one value of each per cell: a mixed 3-D stage update's streams"""
+    lib = load_library()
+    g = ctypes.c_double()
+    rc = lib.smaug_stream_probe(ncell, reps, ctypes.byref(g))
+    if rc:
+        raise SmaugError(lib.smaug_last_error().decode())
+    return g.value
 
 
 def copy_probe(nbytes=2 << 30, reps=10):
