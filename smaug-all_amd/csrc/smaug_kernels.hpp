@@ -256,38 +256,6 @@ __device__ __forceinline__ T ratio6(const T q[6])
     return ratio_fin(m1, m3);
 }
 
-// R of the five windows q[c..c+5], c = 0..4, of ten values, sharing their
-// differences and partial maxima; each equals ratio6 of its window.  The
-// maxima are over the same values as ratio6's (with its starting 0), in
-// another order: fmax ignores a NaN operand and the values are non-negative,
-// so the order does not change the result.  Window c takes |d[c..c+4]|,
-// which all hold |d[4]|: m1(c) = max(L[c], R[c]) with L[c] = max|d[c..4]|,
-// R[c] = max|d[4..c+4]|; m3(c) = max(t[c+1..c+3]) = max(P[c+1], t[c+3]) with
-// P[x] = max(t[x], t[x+1]).
-template <typename T>
-__device__ __forceinline__ void ratio6x5(const T q[10], T r[5])
-{
-    T d[9], t[9];
-#pragma unroll
-    for (int x = 0; x < 9; x++) d[x] = q[x + 1] - q[x];
-#pragma unroll
-    for (int x = 1; x < 8; x++) t[x] = d3abs(d[x], q[x + 2], q[x - 1]);
-    T L[5], R[5], Pm[7];
-    L[4] = R[0] = fmax(fabs(d[4]), (T)0);
-#pragma unroll
-    for (int c = 3; c >= 0; c--) L[c] = fmax(fabs(d[c]), L[c + 1]);
-#pragma unroll
-    for (int c = 1; c < 5; c++) R[c] = fmax(fabs(d[4 + c]), R[c - 1]);
-#pragma unroll
-    for (int x = 1; x < 7; x++) Pm[x] = fmax(t[x], t[x + 1]);
-#pragma unroll
-    for (int c = 0; c < 5; c++) {
-        const T m1 = c == 0 ? L[0] : (c == 4 ? R[4] : fmax(L[c], R[c]));
-        const T m3 = fmax(fmax(Pm[c + 1], t[c + 3]), (T)0);
-        r[c] = ratio_fin(m1, m3);
-    }
-}
-
 template <int D, typename T>
 __device__ __forceinline__ void ratio_cell(const Geo &g, const T *__restrict__ u,
                                            const T *__restrict__ aux, T *__restrict__ R,
