@@ -454,14 +454,16 @@ def main():
         "step_level_frac": (BYTES_PER_CELL_STEP["mixed"] * cells / (ms_step / 1e3) / 1e9)
         / HBM_PEAK_GBS,
     }
-    from smaugamd.engine import copy_probe, stream_probe
+    from smaugamd.engine import copy_probe, stream_probe, walk_probe
 
     roof["measured_copy_gbs"] = copy_probe()
     roof["frac_of_measured_copy"] = achieved / roof["measured_copy_gbs"]
     # the streams of a stage update (32 read, 20 written, one FP32 value per
     # cell each), for the measured traffic of the update pass to be read
-    # against: what HBM delivers to that access pattern
+    # against: what HBM delivers to them read linearly, and walked plane by
+    # plane in the update's 64 x 4-column tiles at its tuned depth
     roof["measured_stream_gbs"] = stream_probe(cells)
+    roof["measured_walk_gbs"] = walk_probe(setup.n, kz=(info.get("tuned") or {}).get("kz", 128))
     ens = None
     if not a.no_ensemble:
         ens = ensemble_leg(a.ens_steps, a.warmup, compare=not a.no_ens_compare)

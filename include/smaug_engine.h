@@ -169,6 +169,12 @@ int smaug_copy_probe(long long nbytes, int reps, double *gbs);
  * mean of reps launches, the best over 8, 16 or 32 workgroups per CU.  It
  * allocates 208 bytes per cell. */
 int smaug_stream_probe(long long ncell, int reps, double *gbs);
+/* the same streams walked as the z-streamed stage update walks them: GB/s
+ * (read + write bytes of the interior) of blocks of 64 x 4 columns of an
+ * n0 x n1 x n2 grid, each walking a chunk of kz planes with the next
+ * plane's loads issued before the current plane's stores, the tiles of an
+ * XCD contiguous; mean of reps launches.  Allocates 208 bytes per cell. */
+int smaug_walk_probe(int n0, int n1, int n2, int kz, int reps, double *gbs);
 
 /* sizes */
 long long smaug_ncell(const smaug_engine *e);

@@ -1644,6 +1644,81 @@ extern "C" int smaug_stream_probe(long long ncell, int reps, double *gbs)
     return 0;
 }
 
+// the same 32 read and 20 written FP32 streams walked as the z-streamed
+// stage update walks them: blocks of 64 x 4 columns of an n0 x n1 x n2 grid
+// (the interior, 2 cells in from each side), each walking a chunk of kz
+// planes with the next plane's loads issued before the current plane's
+// stores, the tiles of an XCD contiguous (as ZMap's)
+template <int NR, int NW>
+__global__ __launch_bounds__(256) void k_walk_probe(const float *__restrict__ a,
+                                                    float *__restrict__ b, ZMap m, int n0, int n1,
+                                                    int n2, idx_t nc)
+{
+    int tx, ty, kc;
+    if (!zmap_block(m, tx, ty, kc)) return;
+    const int i = 2 + tx * 64 + (threadIdx.x & 63), j = 2 + ty * 4 + (threadIdx.x >> 6);
+    const int k0 = 2 + kc * m.kz, k1 = min(k0 + m.kz, n2 - 2);
+    if (i > n0 - 3 || j > n1 - 3 || k0 >= k1) return;
+    const idx_t col = (idx_t)i + (idx_t)n0 * j, s2 = (idx_t)n0 * n1;
+    float v[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) v[r] = a[(idx_t)r * nc + col + s2 * k0];
+    for (int k = k0; k < k1; k++) {
+        float sum = 0.0f;
+#pragma unroll
+        for (int r = 0; r < NR; r++) sum += v[r];
+        const int kn = min(k + 1, k1 - 1);
+#pragma unroll
+        for (int r = 0; r < NR; r++) v[r] = a[(idx_t)r * nc + col + s2 * kn];
+#pragma unroll
+        for (int w = 0; w < NW; w++) b[(idx_t)w * nc + col + s2 * k] = sum + (float)w;
+    }
+}
+
+extern "C" int smaug_walk_probe(int n0, int n1, int n2, int kz, int reps, double *gbs)
+{
+    if (n0 < 8 || n1 < 8 || n2 < 8 || kz < 1 || reps < 1 || !gbs) return fail(SMAUG_EINVAL, "bad argument");
+    const idx_t n = (idx_t)n0 * n1 * n2;
+    float *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc((void **)&a, (size_t)SP_NR * n * sizeof(float)));
+    hipError_t he = hipMalloc((void **)&b, (size_t)SP_NW * n * sizeof(float));
+    if (he != hipSuccess) {
+        hipFree(a);
+        return fail(SMAUG_ENOMEM, "hipMalloc: %s", hipGetErrorString(he));
+    }
+    ZMap m;
+    m.ntx = (n0 - 4 + 63) / 64;
+    m.nty = (n1 - 4 + 3) / 4;
+    m.kz = kz < n2 - 4 ? kz : n2 - 4;
+    m.nkz = (n2 - 4 + m.kz - 1) / m.kz;
+    m.per_xcd = (m.ntx * m.nty + 7) / 8;
+    const unsigned nb = (unsigned)(8 * m.per_xcd * m.nkz);
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIPCHK(hipMemsetAsync(a, 0, (size_t)SP_NR * n * sizeof(float), st));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    hipLaunchKernelGGL((k_walk_probe<SP_NR, SP_NW>), dim3(nb), dim3(256), 0, st, a, b, m, n0, n1, n2, n);
+    HIPCHK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL((k_walk_probe<SP_NR, SP_NW>), dim3(nb), dim3(256), 0, st, a, b, m, n0, n1,
+                           n2, n);
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipGetLastError());
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    const double interior = (double)(n0 - 4) * (n1 - 4) * (n2 - 4);
+    *gbs = (double)(SP_NR + SP_NW) * interior * sizeof(float) * reps / (ms / 1e3) / 1e9;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipStreamDestroy(st);
+    hipFree(a);
+    hipFree(b);
+    return 0;
+}
+
 extern "C" int smaug_copy_probe(long long nbytes, int reps, double *gbs)
 {
     if (nbytes < 16 || reps < 1 || !gbs) return fail(SMAUG_EINVAL, "bad argument");

@@ -24,6 +24,7 @@ EXPORTS = [
     "smaug_last_error", "smaug_create", "smaug_destroy", "smaug_set_state", "smaug_get_state",
     "smaug_rhs", "smaug_cmax", "smaug_maxcourant", "smaug_rk4", "smaug_euler",
     "smaug_time_rk4", "smaug_ncell", "smaug_ninterior", "smaug_copy_probe", "smaug_stream_probe",
+    "smaug_walk_probe",
     "smaug_set_panel", "smaug_get_panel", "smaug_time_pass", "smaug_autotune",
     "smaug_set_zstream", "smaug_get_zstream", "smaug_time_passes", "smaug_set_ratio_form",
     "smaug_get_ratio_form", "smaug_set_bg_plane", "smaug_get_bg_plane",
@@ -82,6 +83,8 @@ def load_library(path: str = LIB_PATH):
         lib.smaug_ninterior.restype = ctypes.c_longlong
         lib.smaug_copy_probe.argtypes = [ctypes.c_longlong, ctypes.c_int, dp]
         lib.smaug_stream_probe.argtypes = [ctypes.c_longlong, ctypes.c_int, dp]
+        lib.smaug_walk_probe.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, dp]
         vp = ctypes.c_void_p
         ip = ctypes.POINTER(ctypes.c_int)
         lib.smaug_set_panel.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -388,6 +391,18 @@ def stream_probe(ncell=1 << 24, reps=10):
     lib = load_library()
     g = ctypes.c_double()
     rc = lib.smaug_stream_probe(ncell, reps, ctypes.byref(g))
+    if rc:
+        raise SmaugError(lib.smaug_last_error().decode())
+    return g.value
+
+
+def walk_probe(n, kz=128, reps=10):
+    """measured bandwidth (GB/s, read + write of the interior) of the same
+    streams walked as the z-streamed stage update walks them: 64 x 4-column
+    tiles of a grid of n = (n0, n1, n2) cells, chunks of kz planes"""
+    lib = load_library()
+    g = ctypes.c_double()
+    rc = lib.smaug_walk_probe(int(n[0]), int(n[1]), int(n[2]), int(kz), reps, ctypes.byref(g))
     if rc:
         raise SmaugError(lib.smaug_last_error().decode())
     return g.value
