@@ -93,7 +93,7 @@ int smaug_time_rk4(smaug_engine *e, double t0, double dt, int nsteps,
  * ratio kernel at depths 8..128 and the z-streamed update at depths 8..128
  * with either form of the hyperdiffusive-source kernel, and keeps the
  * fastest form per pass; then ratio form 2 (smaug_set_ratio_form) at depths
- * 8..32, kept if the ratio and update passes take less time together with
+ * 4..32, kept if the ratio and update passes take less time together with
  * it.  best_ms[p] is the mean time of pass p in the forms kept. */
 int smaug_set_panel(smaug_engine *e, int pass, int tpx, int tpy);
 int smaug_get_panel(smaug_engine *e, int pass, int *tpx, int *tpy);

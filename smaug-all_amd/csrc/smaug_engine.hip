@@ -1236,7 +1236,7 @@ extern "C" int smaug_time_pass(smaug_engine *e, int pass, int reps, double *ms)
 // ratio pass also tries the z-streamed ratio kernel at depths 8..128 and
 // the update pass the z-streamed update at depths 8..128 with either
 // z-streamed source kernel, and each keeps whichever form is fastest; then
-// ratio form 2 at depths 8..32 is kept if the two passes together take less
+// ratio form 2 at depths 4..32 is kept if the two passes together take less
 // time with it
 extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double best_ms[2])
 {
@@ -1325,7 +1325,7 @@ extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double 
     // where the ratio pass and the update pass with its sources take less
     // time together
     if (e->D == 3 && e->phys.hyperdif && e->zstream && e->hyper_lds == 2) {
-        static const int KZ[] = {8, 16, 32};
+        static const int KZ[] = {4, 6, 8, 16, 32};
         const int rz = e->ratio_zs, kzr = e->kzr;
         double bt = bms[0] + bms[1], b0 = 0, b1 = 0;
         int bk = -1;
