@@ -364,8 +364,10 @@ def orszag_tang_leg(nsteps=500):
         e.set_state()
         tot, _ = e.time_rk4(0.0, s.dt, nsteps)
         res[name] = e.get_state()
+        rform = e.get_ratio_form()[0]
         e.close()
-        out[name] = {"ms_per_step": tot / nsteps, "value": s.ninterior * nsteps / (tot / 1e3) / 1e6}
+        out[name] = {"ms_per_step": tot / nsteps, "value": s.ninterior * nsteps / (tot / 1e3) / 1e6,
+                     "ratio_form": rform}
     m = np.zeros((s.n[1], s.n[0]), dtype=bool)
     m[2:-2, 2:-2] = True
     m = m.reshape(-1)

@@ -114,7 +114,11 @@ int smaug_set_zstream(smaug_engine *e, int enable, int kz);
  * LDS, from the state around its tile -- the same ratios without their trip
  * through HBM.  Form 2 takes effect with the z-streamed update and source
  * kernel (smaug_set_zstream enable = 1); with any other form of the update
- * the ratio pass forms every pair, as form 1. */
+ * the ratio pass forms every pair, as form 1.  2-D grids: 2 (the default)
+ * the update kernel forms the ten pairs of its tile and a 1-cell halo in
+ * LDS and the stage has no ratio pass, 0 the ratio pass writes them to HBM
+ * and the update reads them there; same ratios either way (form 1 is
+ * rejected). */
 int smaug_set_ratio_form(smaug_engine *e, int zstreamed, int kz);
 /* the form of the ratio pass in use (0, 1 or 2) and its depth (0 per cell) */
 int smaug_get_ratio_form(smaug_engine *e, int *zstreamed, int *kz);
@@ -211,6 +215,12 @@ int smaug_ens_time_passes(smaug_ensemble *e, const double *t0, const double *dt,
 /* as smaug_ens_rk4, and the milliseconds of the step loop (HIP events) */
 int smaug_ens_time_rk4(smaug_ensemble *e, const double *t0, const double *dt, int nsteps,
                        double *total_ms);
+/* 2-D runs with hyperdiffusion: 2 (the default) the stage update pulls
+ * 64 x 4-cell tiles and forms their ratio pairs itself, in LDS, and the
+ * stage has no ratio pass; 0 the ratio pass writes them to HBM.  Same
+ * ratios either way.  smaug_ens_get_ratio_form reports the form in use. */
+int smaug_ens_set_ratio_form(smaug_ensemble *e, int form);
+int smaug_ens_get_ratio_form(smaug_ensemble *e, int *form);
 
 #ifdef __cplusplus
 }

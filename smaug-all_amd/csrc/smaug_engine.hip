@@ -253,6 +253,13 @@ static bool ratio_in_sources(const smaug_engine *e)
     return e->D == 3 && e->ratio_zs == 2 && e->zstream && e->hyper_lds == 2;
 }
 
+// 2-D, ratio form 2: the update kernel forms the ratios itself
+// (k_update2r) and the stage has no ratio pass
+static bool ratio_in_update(const smaug_engine *e)
+{
+    return e->D == 2 && e->ratio_zs == 2 && e->phys.hyperdif;
+}
+
 template <int D, typename T>
 static int launch_ratio(smaug_engine *e, const void *u)
 {
@@ -298,6 +305,30 @@ static int launch_update(smaug_engine *e, StageArgs sa, const void *uin, const v
     const Geo g = pc_geo(e, &bg);
     // a background that is the same in every cell (bgm == 0) is read once
     const bool uni = g.bgm == 0u;
+    if constexpr (D == 2) {
+        if (ratio_in_update(e)) {
+            if (aux_out) {
+                const idx_t ng = ghost_count2(e->n[0], e->n[1]);
+                const unsigned gb = (unsigned)((ng + 255) / 256);
+                if ((idx_t)nb + gb > (idx_t)e->ncpart)
+                    return fail(SMAUG_EINVAL, "cfast partials overflow");
+                auto kern = uni ? k_update2r<T, true, true> : k_update2r<T, false, true>;
+                hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, e->stream, g, m, e->phys, sa,
+                                   (const T *)uin, (const T *)un, (T *)uout, (T *)e->ACC,
+                                   (const T *)e->aux_in, bg,
+                                   (const unsigned long long *)e->d_cmax, (T *)aux_out,
+                                   e->d_cpart);
+            } else {
+                auto kern = uni ? k_update2r<T, true> : k_update2r<T, false>;
+                hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, e->stream, g, m, e->phys, sa,
+                                   (const T *)uin, (const T *)un, (T *)uout, (T *)e->ACC,
+                                   (const T *)e->aux_in, bg,
+                                   (const unsigned long long *)e->d_cmax, nullptr, nullptr);
+            }
+            HIPCHK(hipGetLastError());
+            return 0;
+        }
+    }
     if (aux_out) {
         if constexpr (D == 2) {
             const idx_t ng = ghost_count2(e->n[0], e->n[1]);
@@ -549,7 +580,7 @@ static int stage_launches(smaug_engine *e, StageArgs sa, const void *uin, const 
     if (!fold && !(fold2 && sa.stage > 0))
         if ((rc = launch_cmax<D, T>(e, uin, defer && sa.stage > 0))) return rc;
     if (evk) HIPCHK(hipEventRecord(evk[1], e->stream));
-    if (e->phys.hyperdif)
+    if (e->phys.hyperdif && !ratio_in_update(e))
         if ((rc = launch_ratio<D, T>(e, uin))) return rc;
     if (evk) HIPCHK(hipEventRecord(evk[2], e->stream));
     if ((rc = launch_pre_update<D, T>(e, uin))) return rc;
@@ -805,7 +836,7 @@ extern "C" int smaug_create(const smaug_cfg *cfg, smaug_engine **out)
     e->kz = 0;
     e->hyper_lds = 2;
     e->kzh = 0;
-    e->ratio_zs = D == 3 ? 2 : 0;
+    e->ratio_zs = 2; // 3-D: in-plane pairs in the source kernel; 2-D: every pair in the update
     e->kzr = 0;
     e->bgp_on = 1;
     *out = e;
@@ -1242,7 +1273,7 @@ extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double 
 {
     if (!e || reps < 1) return fail(SMAUG_EINVAL, "bad argument");
     static const int TX[] = {1, 2, 4, 8}, TY[] = {1, 2, 4, 8, 16, 32};
-    const int zs0 = e->zstream, rz0 = e->ratio_zs;
+    const int zs0 = e->zstream;
     e->zstream = 0;
     e->ratio_zs = 0;
     double bms[2] = {0, 0};
@@ -1352,13 +1383,27 @@ extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double 
             e->kzr = kzr;
         }
     }
+    // 2-D: the ratios formed by the update kernel (form 2) against the ratio
+    // pass and the update, each at its tuned panel
+    if (e->D != 3) {
+        e->zstream = zs0;
+        e->ratio_zs = 0;
+        if (e->phys.hyperdif) {
+            e->ratio_zs = 2;
+            double m1;
+            int rc = DISPATCH(e, time_pass, e, 1, reps, &m1);
+            if (rc) return rc;
+            if (m1 < bms[0] + bms[1]) {
+                bms[0] = 0;
+                bms[1] = m1;
+            } else {
+                e->ratio_zs = 0;
+            }
+        }
+    }
     if (best_ms) {
         best_ms[0] = bms[0];
         best_ms[1] = bms[1];
-    }
-    if (e->D != 3) {
-        e->zstream = zs0;
-        e->ratio_zs = rz0;
     }
     drop_graphs(e);
     return 0;
@@ -1367,7 +1412,7 @@ extern "C" int smaug_autotune(smaug_engine *e, int reps, int best[2][2], double 
 extern "C" int smaug_set_ratio_form(smaug_engine *e, int zstreamed, int kz)
 {
     if (!e || kz < 0 || zstreamed < 0 || zstreamed > 2) return fail(SMAUG_EINVAL, "bad argument");
-    if (zstreamed && e->D != 3) return fail(SMAUG_EINVAL, "the z-streamed ratios are for 3-D grids");
+    if (zstreamed == 1 && e->D != 3) return fail(SMAUG_EINVAL, "the z-streamed ratios are for 3-D grids");
     e->ratio_zs = zstreamed;
     e->kzr = kz;
     drop_graphs(e);
@@ -1378,8 +1423,8 @@ extern "C" int smaug_get_ratio_form(smaug_engine *e, int *zstreamed, int *kz)
 {
     if (!e || !zstreamed || !kz) return fail(SMAUG_EINVAL, "null argument");
     unsigned nb;
-    *zstreamed = e->D == 3 ? e->ratio_zs : 0;
-    *kz = *zstreamed ? make_rmap(e, &nb).kz : 0;
+    *zstreamed = e->ratio_zs;
+    *kz = e->D == 3 && *zstreamed ? make_rmap(e, &nb).kz : 0;
     return 0;
 }
 

@@ -58,6 +58,8 @@ __device__ __forceinline__ unsigned fdiv(unsigned t, const FastDiv &f)
     return (unsigned)(((unsigned long long)t * f.m) >> f.s);
 }
 constexpr int NPASS = 4;         // prep, ratio, update, ghost fill
+constexpr int NLIST = NPASS + 1; // their item lists, and the 64 x 4 tiles of the 2-D
+                                 // update that forms the ratios itself (ke_update2r)
 constexpr int NQ = 8;            // queue slices, one per XCD
 // slice counters lie 4 KiB apart: atomics on one cache line (or one memory
 // channel) are served one after another, and every pull of a pass, as well
@@ -250,6 +252,45 @@ __global__ __launch_bounds__(256) void ke_update(const RunDesc<T> *__restrict__ 
     }
 }
 
+// 2-D ratio form 2: the update of the 64 x 4 tile items of every run, each
+// forming the ratio pairs of its tile and a 1-cell halo in LDS as
+// k_update2r does (the stage then has no ratio pass)
+template <typename T>
+__global__ __launch_bounds__(256) void ke_update2r(const RunDesc<T> *__restrict__ runs, Queue q,
+                                                    Phys ph, int stage, double ct, int src,
+                                                    int dst)
+{
+    __shared__ int s_it[4];
+    __shared__ T sQ[6][U2_QN];
+    __shared__ T sR[Lay<2>::NPAIR * U2_RN];
+    int v = 0;
+    if (threadIdx.x == 0) s_it[1] = -2;
+    for (;;) {
+        int r;
+        const int it = block_pull(q, v, s_it, r);
+        if (it < 0) break;
+        const RunDesc<T> &rd = runs[r];
+        const int ntx = (rd.geo.n[0] - 4 + U2_TI - 1) / U2_TI;
+        const int t = it - q.base[r];
+        const int ty = t / ntx, tx = t - ty * ntx;
+        StageArgs sa;
+        sa.mode = 0;
+        sa.stage = stage;
+        sa.ct = ct;
+        sa.tm = rd.tm;
+        // the run's background layout is block-uniform: read once when it
+        // is the same in every cell
+        if (rd.geo.bgm == 0u)
+            update2r_tile<T, true, false>(rd.geo, ph, sa, rd.U[src], rd.U[0], rd.U[dst], rd.ACC,
+                                          rd.AUX, rd.bg, rd.cm + 4 * stage, tx, ty, sQ, sR,
+                                          nullptr, nullptr);
+        else
+            update2r_tile<T, false, false>(rd.geo, ph, sa, rd.U[src], rd.U[0], rd.U[dst], rd.ACC,
+                                           rd.AUX, rd.bg, rd.cm + 4 * stage, tx, ty, sQ, sR,
+                                           nullptr, nullptr);
+    }
+}
+
 template <int D, typename T>
 __global__ __launch_bounds__(256) void ke_bc(const RunDesc<T> *__restrict__ runs, Queue q, int dst)
 {
@@ -287,10 +328,11 @@ struct smaug_ensemble {
     unsigned long long *cm;        // nrun x 16
     unsigned *ctr;                 // 16 launches x NQ
     void *descs;                   // RunDesc<T>[nrun]
-    int *base[NPASS];              // per-pass item prefix sums (device)
-    int *irun[NPASS];              // per-pass run of every item (device)
-    int nitems[NPASS];
-    unsigned grid[NPASS];          // persistent grid per pass
+    int *base[NLIST];              // per-list item prefix sums (device)
+    int *irun[NLIST];              // per-list run of every item (device)
+    int nitems[NLIST];
+    unsigned grid[NLIST];          // persistent grid per list's kernel
+    int rform;                     // 2-D: 2 the update forms the ratios (ke_update2r), 0 not
     double *d_part;                // conservation-diagnostic partial sums
     Phys phys;
     hipGraphExec_t gexec;
@@ -305,7 +347,7 @@ static void free_ens(smaug_ensemble *e)
                     e->d_part, e->bgu};
     for (void *p : ptrs)
         if (p) hipFree(p);
-    for (int k = 0; k < NPASS; k++) {
+    for (int k = 0; k < NLIST; k++) {
         if (e->base[k]) hipFree(e->base[k]);
         if (e->irun[k]) hipFree(e->irun[k]);
     }
@@ -347,8 +389,8 @@ template <int D, typename T>
 static int setup_typed(smaug_ensemble *e)
 {
     std::vector<RunDesc<T>> d(e->nrun);
-    std::vector<int> base[NPASS];
-    for (int k = 0; k < NPASS; k++) base[k].assign(e->nrun + 1, 0);
+    std::vector<int> base[NLIST];
+    for (int k = 0; k < NLIST; k++) base[k].assign(e->nrun + 1, 0);
     const int nve = 2 + 2 * D;
     for (int r = 0; r < e->nrun; r++) {
         RunDesc<T> &rd = d[r];
@@ -373,12 +415,14 @@ static int setup_typed(smaug_ensemble *e)
         const long long nbox[2] = {(long long)rd.ext[0][0] * rd.ext[0][1] * rd.ext[0][2],
                                    (long long)rd.ext[1][0] * rd.ext[1][1] * rd.ext[1][2]};
         if (g.nc >= (1ll << 31)) return fail(SMAUG_EINVAL, "run %d: too many cells for a member", r);
-        long long items[NPASS] = {
+        long long items[NLIST] = {
             (long long)((g.nc + PREP_CELLS - 1) / PREP_CELLS),
             e->phys.hyperdif ? (nbox[0] + RATIO_CELLS - 1) / RATIO_CELLS : 0,
             (nbox[1] + UPDATE_CELLS - 1) / UPDATE_CELLS,
-            (long long)((bc_count<D>(g.n[0], g.n[1], n2) + BC_CELLS - 1) / BC_CELLS)};
-        for (int k = 0; k < NPASS; k++) {
+            (long long)((bc_count<D>(g.n[0], g.n[1], n2) + BC_CELLS - 1) / BC_CELLS),
+            D == 2 ? (long long)((g.n[0] - 4 + U2_TI - 1) / U2_TI) * ((g.n[1] - 4 + U2_TJ - 1) / U2_TJ)
+                   : 0};
+        for (int k = 0; k < NLIST; k++) {
             long long nb = (long long)base[k][r] + items[k];
             if (nb > 0x7fffffffLL) return fail(SMAUG_EINVAL, "ensemble too large for the work queue");
             base[k][r + 1] = (int)nb;
@@ -386,7 +430,7 @@ static int setup_typed(smaug_ensemble *e)
     }
     HIPCHK(hipMalloc(&e->descs, sizeof(RunDesc<T>) * e->nrun));
     HIPCHK(hipMemcpy(e->descs, d.data(), sizeof(RunDesc<T>) * e->nrun, hipMemcpyHostToDevice));
-    for (int k = 0; k < NPASS; k++) {
+    for (int k = 0; k < NLIST; k++) {
         e->nitems[k] = base[k][e->nrun];
         HIPCHK(hipMalloc((void **)&e->base[k], sizeof(int) * (e->nrun + 1)));
         HIPCHK(hipMemcpy(e->base[k], base[k].data(), sizeof(int) * (e->nrun + 1),
@@ -402,8 +446,11 @@ static int setup_typed(smaug_ensemble *e)
     if ((rc = pass_grid(ke_ratio<D, T>, &e->grid[1]))) return rc;
     if ((rc = pass_grid(ke_update<D, T>, &e->grid[2]))) return rc;
     if ((rc = pass_grid(ke_bc<D, T>, &e->grid[3]))) return rc;
+    e->grid[4] = 0;
+    if constexpr (D == 2)
+        if ((rc = pass_grid(ke_update2r<T>, &e->grid[4]))) return rc;
     // no more blocks than items (a multiple of 8 so every XCD slice has blocks)
-    for (int k = 0; k < NPASS; k++) {
+    for (int k = 0; k < NLIST; k++) {
         const unsigned need = ((unsigned)e->nitems[k] + 7) & ~7u;
         if (need > 0 && need < e->grid[k]) e->grid[k] = need;
     }
@@ -442,7 +489,9 @@ extern "C" int smaug_ens_create(const smaug_cfg *cfgs, int nrun, smaug_ensemble 
     e->cm = nullptr;
     e->ctr = nullptr;
     e->descs = nullptr;
-    for (int k = 0; k < NPASS; k++) e->base[k] = e->irun[k] = nullptr;
+    for (int k = 0; k < NLIST; k++) e->base[k] = e->irun[k] = nullptr;
+    // 2-D with hyperdiffusion: the update forms the ratios (no ratio pass)
+    e->rform = e->D == 2 && cfgs[0].hyperdif ? 2 : 0;
     e->stream = nullptr;
     e->d_part = nullptr;
     e->gvalid = false;
@@ -665,12 +714,19 @@ static int ens_step(smaug_ensemble *e, const hipEvent_t *evk = nullptr)
         hipLaunchKernelGGL((ke_prep<D, T>), dim3(e->grid[0]), dim3(256), 0, e->stream, rd,
                            queue(e, 0, 4 * s + 0), e->phys, s, in[s]);
         if (evk) HIPCHK(hipEventRecord(evk[5 * s + 1], e->stream));
-        if (e->phys.hyperdif)
+        const bool r2 = D == 2 && e->rform == 2 && e->phys.hyperdif;
+        if (e->phys.hyperdif && !r2)
             hipLaunchKernelGGL((ke_ratio<D, T>), dim3(e->grid[1]), dim3(256), 0, e->stream, rd,
                                queue(e, 1, 4 * s + 1), in[s]);
         if (evk) HIPCHK(hipEventRecord(evk[5 * s + 2], e->stream));
-        hipLaunchKernelGGL((ke_update<D, T>), dim3(e->grid[2]), dim3(256), 0, e->stream, rd,
-                           queue(e, 2, 4 * s + 2), e->phys, s, ct[s], in[s], out[s]);
+        if constexpr (D == 2) {
+            if (r2)
+                hipLaunchKernelGGL(ke_update2r<T>, dim3(e->grid[4]), dim3(256), 0, e->stream, rd,
+                                   queue(e, 4, 4 * s + 2), e->phys, s, ct[s], in[s], out[s]);
+        }
+        if (!r2)
+            hipLaunchKernelGGL((ke_update<D, T>), dim3(e->grid[2]), dim3(256), 0, e->stream, rd,
+                               queue(e, 2, 4 * s + 2), e->phys, s, ct[s], in[s], out[s]);
         if (evk) HIPCHK(hipEventRecord(evk[5 * s + 3], e->stream));
         hipLaunchKernelGGL((ke_bc<D, T>), dim3(e->grid[3]), dim3(256), 0, e->stream, rd,
                            queue(e, 3, 4 * s + 3), out[s]);
@@ -778,4 +834,24 @@ extern "C" int smaug_ens_time_rk4(smaug_ensemble *e, const double *t0, const dou
 {
     if (!e || !dt || nsteps < 1 || !total_ms) return fail(SMAUG_EINVAL, "bad argument");
     return EDISPATCH(e, ens_run, e, t0, dt, nsteps, total_ms);
+}
+
+extern "C" int smaug_ens_set_ratio_form(smaug_ensemble *e, int form)
+{
+    if (!e || (form != 0 && form != 2)) return fail(SMAUG_EINVAL, "bad argument");
+    if (form == 2 && e->D != 2)
+        return fail(SMAUG_EINVAL, "the ratios formed by the update are for 2-D runs");
+    e->rform = form;
+    if (e->gvalid) {
+        hipGraphExecDestroy(e->gexec);
+        e->gvalid = false;
+    }
+    return 0;
+}
+
+extern "C" int smaug_ens_get_ratio_form(smaug_ensemble *e, int *form)
+{
+    if (!e || !form) return fail(SMAUG_EINVAL, "null argument");
+    *form = e->D == 2 && e->phys.hyperdif ? e->rform : 0;
+    return 0;
 }

@@ -256,6 +256,52 @@ __device__ __forceinline__ T ratio6(const T q[6])
     return ratio_fin(m1, m3);
 }
 
+// index of window position x along an axis of n cells: the pads q[-1] =
+// q[5] and q[n] = q[n-6] of the ratio windows
+__device__ __forceinline__ int pad_map(int x, int n) { return x < 0 ? 5 : (x >= n ? n - 6 : x); }
+
+// R of the L windows q[c..c+5] (c = 0..L-1) of the L + 5 values q[x st],
+// sharing their differences and partial maxima; each equals ratio6 of its
+// window.  The maxima are over the same values as ratio6's, in another
+// order: fmax ignores a NaN operand and the values are non-negative, so the
+// order does not change the result.  The D3 maxima keep ratio6's starting
+// 0 (as fmax(t, 0) per difference); for the D1 maxima ratio_fin's clamp
+// (FP32) or its test m1 > 0 (FP64) gives what the starting 0 does.
+template <int L, typename T>
+__device__ __forceinline__ void ratio_run(const T *q, int st, T r[L])
+{
+    T v[L + 5], d[L + 4], t[L + 3], a[L + 2];
+#pragma unroll
+    for (int x = 0; x < L + 5; x++) v[x] = q[x * st];
+#pragma unroll
+    for (int x = 0; x < L + 4; x++) d[x] = v[x + 1] - v[x];
+    // D3 at the centre differences 1..L+2
+#pragma unroll
+    for (int x = 1; x < L + 3; x++) t[x] = fmax(d3abs(d[x], v[x + 2], v[x - 1]), (T)0);
+    // max |d[x..x+2]|
+#pragma unroll
+    for (int x = 0; x < L + 2; x++) a[x] = fmax(fmax(fabs(d[x]), fabs(d[x + 1])), fabs(d[x + 2]));
+#pragma unroll
+    for (int c = 0; c < L; c++) {
+        const T m1 = fmax(fmax(a[c], fabs(d[c + 3])), fabs(d[c + 4]));
+        const T m3 = fmax(fmax(t[c + 1], t[c + 2]), t[c + 3]);
+        r[c] = ratio_fin(m1, m3);
+    }
+}
+
+// r[c] to o[c st] for c <= lim, 0 beyond
+template <int L, typename T>
+__device__ __forceinline__ void ratio_store(T *o, int st, const T r[L], int lim)
+{
+    if (lim >= L - 1) {
+#pragma unroll
+        for (int c = 0; c < L; c++) o[c * st] = r[c];
+    } else {
+#pragma unroll
+        for (int c = 0; c < L; c++) o[c * st] = c <= lim ? r[c] : (T)0;
+    }
+}
+
 template <int D, typename T>
 __device__ __forceinline__ void ratio_cell(const Geo &g, const T *__restrict__ u,
                                            const T *__restrict__ aux, T *__restrict__ R,
@@ -810,14 +856,15 @@ __device__ __forceinline__ void hyper_terms(const Geo &g, const Phys &ph, const 
     }
 }
 
-template <int D, typename T, bool UNI = false>
-__device__ __forceinline__ void hyper_cell(const Geo &g, const Phys &ph, const T *__restrict__ uin,
-                                           const T *__restrict__ R, const T *__restrict__ aux,
-                                           const double *__restrict__ bg,
-                                           const unsigned long long *__restrict__ cmb,
-                                           const int x0, const int x1, const int x2, T *dU)
+// the global-memory view of cell (x0, x1, x2)
+template <int D, typename T, bool UNI>
+__device__ __forceinline__ void hy_global_init(HyGlobal<D, T, UNI> &vw, const Geo &g,
+                                               const T *__restrict__ uin,
+                                               const T *__restrict__ R,
+                                               const T *__restrict__ aux,
+                                               const double *__restrict__ bg, const int x0,
+                                               const int x1, const int x2)
 {
-    HyGlobal<D, T, UNI> vw;
     vw.rb0 = UNI ? ld(bg + (size_t)g.bgf, 0u) : 0.0;
     vw.uin = uin;
     vw.aux = aux;
@@ -829,8 +876,37 @@ __device__ __forceinline__ void hyper_cell(const Geo &g, const Phys &ph, const T
     vw.s1 = (ci_t)g.s1;
     vw.s2 = (ci_t)g.s2;
     vw.c = (ci_t)x0 + vw.s1 * (ci_t)x1 + vw.s2 * (ci_t)x2;
+}
+
+template <int D, typename T, bool UNI = false>
+__device__ __forceinline__ void hyper_cell(const Geo &g, const Phys &ph, const T *__restrict__ uin,
+                                           const T *__restrict__ R, const T *__restrict__ aux,
+                                           const double *__restrict__ bg,
+                                           const unsigned long long *__restrict__ cmb,
+                                           const int x0, const int x1, const int x2, T *dU)
+{
+    HyGlobal<D, T, UNI> vw;
+    hy_global_init<D, T, UNI>(vw, g, uin, R, aux, bg, x0, x1, x2);
     hyper_terms<D, T>(g, ph, vw, cmb, x0, x1, x2, dU);
 }
+
+// HyGlobal with the ratio pairs read from a box in LDS: pair p of the cell
+// at (di, dj) from the box cell own is sR[p nb + own + di + bi dj] (2-D)
+template <int D, typename T, bool UNI = false>
+struct HyLdsR : HyGlobal<D, T, UNI> {
+    const T *sR;
+    int nb, own, bi;
+    __device__ __forceinline__ T Rp(int p, int di, int dj, int dk) const
+    {
+        return sR[p * nb + own + di + bi * dj];
+    }
+};
+
+// LR (update_cell): the hyperdiffusive sources with the ratio pairs from LDS
+struct RBox {
+    const void *sR; // T [NPAIR][nb]
+    int nb, own, bi;
+};
 
 // the driver's time factor drv_0 sin(2 pi (t + ct dt) / drv_1) at the stage
 // time (usersource.cu:4-92)
@@ -945,7 +1021,7 @@ __device__ __forceinline__ void rk_write(const StageArgs &sa, const T *__restric
 // aux quantities of the stage result (all of them, as the aux pass forms
 // them) go to aux_out and its cfast into the running maxima mx, so that the
 // next stage needs no aux/cmax pass over the interior.
-template <int D, typename T, bool UNI = false, bool FOLD = false>
+template <int D, typename T, bool UNI = false, bool FOLD = false, bool LR = false>
 __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const StageArgs &sa,
                                             const T *__restrict__ uin,
                                             const T *__restrict__ un,
@@ -956,7 +1032,7 @@ __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const 
                                             const unsigned long long *__restrict__ cmb,
                                             const int x0, const int x1, const int x2,
                                             T *__restrict__ aux_out = nullptr,
-                                            double *mx = nullptr)
+                                            double *mx = nullptr, RBox rbox = RBox())
 {
     typedef Lay<D> LY;
     typedef Aux<D> AX;
@@ -1062,7 +1138,19 @@ __device__ __forceinline__ void update_cell(const Geo &g, const Phys &ph, const 
         dU[LY::EN] += rc * ge * A(AX::IRHO, c);
     }
 
-    if (ph.hyperdif) hyper_cell<D, T, UNI>(g, ph, uin, R, aux, bg, cmb, x0, x1, x2, dU);
+    if (ph.hyperdif) {
+        if constexpr (LR) {
+            HyLdsR<D, T, UNI> vw;
+            hy_global_init<D, T, UNI>(vw, g, uin, nullptr, aux, bg, x0, x1, x2);
+            vw.sR = (const T *)rbox.sR;
+            vw.nb = rbox.nb;
+            vw.own = rbox.own;
+            vw.bi = rbox.bi;
+            hyper_terms<D, T>(g, ph, vw, cmb, x0, x1, x2, dU);
+        } else {
+            hyper_cell<D, T, UNI>(g, ph, uin, R, aux, bg, cmb, x0, x1, x2, dU);
+        }
+    }
 
     driver_cell<D, T>(g, ph, sa, rc, rbc, x0, x1, x2, dU);
     if constexpr (FOLD) {
@@ -1116,6 +1204,132 @@ __global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, Stag
     } else {
         if (x0 > g.n[0] - 3 || x1 > g.n[1] - 3) return;
         update_cell<D, T, UNI>(g, ph, sa, uin, un, uout, acc, R, aux, bg, cmb, x0, x1, x2);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2-D stage update with the hyperviscosity ratios formed in the kernel
+// (ratio form 2 of 2-D grids).  A block stages rho', e_int, v_0, v_1, b'_0
+// and b'_1 of its 64 x 4 tile and a 3-cell halo in LDS (pads mirrored as in
+// ratio_cell), forms the ten ratio pairs of the tile and a 1-cell halo from
+// them (0 outside the ratio box [1, n-3], where R, which k_ratio fills only
+// there, holds the zeros it was cleared to), and runs update_cell with the
+// pairs read from LDS: bit for bit k_ratio followed by k_update, without R
+// in HBM.  The pairs along i are formed in runs of U2_RL cells of a row,
+// those along j per column of the box.
+// ---------------------------------------------------------------------------
+constexpr int U2_TI = 64, U2_TJ = 4;
+constexpr int U2_RI = U2_TI + 2, U2_RJ = U2_TJ + 2, U2_RN = U2_RI * U2_RJ; // ratio box
+constexpr int U2_QI = U2_TI + 7, U2_QJ = U2_TJ + 7, U2_QN = U2_QI * U2_QJ; // window box
+constexpr int U2_RL = 11, U2_NRUN = U2_RI / U2_RL;
+static_assert(U2_RL * U2_NRUN == U2_RI, "the runs cover the rows of the ratio box");
+
+// pair a (0..4) of the pairs along d: rho', e_int, v_0, v_1, b'_f (f != d):
+// its window quantity in the staged box and its ratio pair
+__device__ __forceinline__ int u2_field(int d, int a) { return a < 4 ? a : 5 - d; }
+__device__ __forceinline__ int u2_pair(int d, int a)
+{
+    typedef Lay<2> LY;
+    return a == 0 ? LY::prho(d)
+                  : (a == 1 ? LY::pe(d) : (a < 4 ? LY::pv(a - 2, d) : LY::pb(1 - d, d)));
+}
+
+// one 64 x 4 tile (tx, ty) of the interior: the window quantities staged,
+// the ratio pairs formed, update_cell run (FOLD: the cfast maxima into mx);
+// every thread of the block calls it (two block barriers)
+template <typename T, bool UNI, bool FOLD>
+__device__ __forceinline__ void update2r_tile(const Geo &g, const Phys &ph, const StageArgs &sa,
+                                              const T *__restrict__ uin,
+                                              const T *__restrict__ un, T *__restrict__ uout,
+                                              T *__restrict__ acc, const T *__restrict__ aux,
+                                              const double *__restrict__ bg,
+                                              const unsigned long long *__restrict__ cmb,
+                                              const int tx, const int ty, T (*sQ)[U2_QN], T *sR,
+                                              T *__restrict__ aux_out, double *mx)
+{
+    typedef Lay<2> LY;
+    typedef Aux<2> AX;
+    const int n0 = g.n[0], n1 = g.n[1];
+    const size_t nc = (size_t)g.nc;
+    const int i0 = 2 + tx * U2_TI, j0 = 2 + ty * U2_TJ;
+    if (ph.hyperdif) {
+        // window box cell (qi, qj) = cell (i0 - 3 + qi, j0 - 3 + qj)
+        for (int q = threadIdx.x; q < U2_QN; q += 256) {
+            const int qj = q / U2_QI, qi = q - qj * U2_QI;
+            const ci_t c = (ci_t)pad_map(i0 - 3 + qi, n0) + (ci_t)g.s1 * (ci_t)pad_map(j0 - 3 + qj, n1);
+            sQ[0][q] = ld(uin, c);
+            sQ[1][q] = ld(aux + AX::EINT * nc, c);
+            sQ[2][q] = ld(aux + AX::V0 * nc, c);
+            sQ[3][q] = ld(aux + (AX::V0 + 1) * nc, c);
+            sQ[4][q] = ld(uin + LY::B0 * nc, c);
+            sQ[5][q] = ld(uin + (LY::B0 + 1) * nc, c);
+        }
+        __syncthreads();
+        // ratio box cell (bi, bj) = cell (i0 - 1 + bi, j0 - 1 + bj); its
+        // window along i starts at window-box column bi of row bj + 2, along
+        // j at row bj of column bi + 2
+        constexpr int NI = 5 * U2_RJ * U2_NRUN, NJ = 5 * U2_RI;
+        for (int u = threadIdx.x; u < NI + NJ; u += 256) {
+            if (u < NI) {
+                const int a = u / (U2_RJ * U2_NRUN);
+                const int y = u - a * (U2_RJ * U2_NRUN);
+                const int bj = y / U2_NRUN, bi = U2_RL * (y - bj * U2_NRUN);
+                T r[U2_RL];
+                ratio_run<U2_RL, T>(sQ[u2_field(0, a)] + (bj + 2) * U2_QI + bi, 1, r);
+                const int gj = j0 - 1 + bj;
+                const int lim = gj > n1 - 3 ? -1 : n0 - 3 - (i0 - 1 + bi);
+                ratio_store<U2_RL, T>(sR + u2_pair(0, a) * U2_RN + bj * U2_RI + bi, 1, r, lim);
+            } else {
+                const int x = u - NI;
+                const int a = x / U2_RI, bi = x - a * U2_RI;
+                T r[U2_RJ];
+                ratio_run<U2_RJ, T>(sQ[u2_field(1, a)] + bi + 2, U2_QI, r);
+                const int gi = i0 - 1 + bi;
+                const int lim = gi > n0 - 3 ? -1 : n1 - 3 - (j0 - 1);
+                ratio_store<U2_RJ, T>(sR + u2_pair(1, a) * U2_RN + bi, U2_RI, r, lim);
+            }
+        }
+        __syncthreads();
+    }
+    const int ti = threadIdx.x & 63, tj = threadIdx.x >> 6;
+    const int x0 = i0 + ti, x1 = j0 + tj;
+    if (x0 > n0 - 3 || x1 > n1 - 3) return;
+    RBox rb;
+    rb.sR = sR;
+    rb.nb = U2_RN;
+    rb.own = (tj + 1) * U2_RI + ti + 1;
+    rb.bi = U2_RI;
+    update_cell<2, T, UNI, FOLD, true>(g, ph, sa, uin, un, uout, acc, nullptr, aux, bg, cmb, x0,
+                                       x1, 0, aux_out, mx, rb);
+}
+
+template <typename T, bool UNI = false, bool FOLD = false>
+__global__ __launch_bounds__(256) void k_update2r(Geo g, TileMap tm, Phys ph, StageArgs sa,
+                                                   const T *__restrict__ uin,
+                                                   const T *__restrict__ un,
+                                                   T *__restrict__ uout, T *__restrict__ acc,
+                                                   const T *__restrict__ aux,
+                                                   const double *__restrict__ bg,
+                                                   const unsigned long long *__restrict__ cmb,
+                                                   T *__restrict__ aux_out = nullptr,
+                                                   double *part = nullptr)
+{
+    __shared__ T sQ[6][U2_QN];
+    __shared__ T sR[Lay<2>::NPAIR * U2_RN];
+    int tx, ty, tk;
+    if (!map_block(tm, tx, ty, tk)) {
+        if (FOLD && threadIdx.x < 2) part[4 * blockIdx.x + threadIdx.x] = 0.0;
+        return;
+    }
+    if constexpr (FOLD) {
+        __shared__ double red[3][4];
+        double mx[3] = {0.0, 0.0, 0.0};
+        update2r_tile<T, UNI, true>(g, ph, sa, uin, un, uout, acc, aux, bg, cmb, tx, ty, sQ, sR,
+                                    aux_out, mx);
+        block_max_part<2>(mx, red, part + 4 * blockIdx.x);
+    } else {
+        update2r_tile<T, UNI, false>(g, ph, sa, uin, un, uout, acc, aux, bg, cmb, tx, ty, sQ, sR,
+                                     nullptr, nullptr);
     }
 }
 

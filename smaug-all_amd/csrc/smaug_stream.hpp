@@ -567,7 +567,6 @@ constexpr int RB_BI = RZ_TI + 5, RB_BJ = RZ_TJ + 5, RB_NB = RB_BI * RB_BJ;
 constexpr int RB_NQ = 8;                           // rho', e_int, v_0..2, b'_0..2
 constexpr int RB_PER = (RB_NB + RZ_NT - 1) / RZ_NT; // box cells per thread
 
-__device__ __forceinline__ int pad_map(int x, int n) { return x < 0 ? 5 : (x >= n ? n - 6 : x); }
 
 template <typename T>
 __device__ __forceinline__ void ratio_zs_body(const Geo &g, const ZMap &zm,
@@ -1060,48 +1059,6 @@ __device__ __forceinline__ int rin_slot(int d, int a)
 // runs of the items along i: the 33 box cells of a row in 3 runs of 11
 constexpr int HZ_RL = 11, HZ_NRUN = (HZ_BI - 1) / HZ_RL;
 static_assert(HZ_RL * HZ_NRUN == HZ_BI - 1, "the runs cover box columns 0..HZ_BI-2");
-
-// R of the L windows q[c..c+5] (c = 0..L-1) of the L + 5 values q[x st],
-// sharing their differences and partial maxima; each equals ratio6 of its
-// window.  The maxima are over the same values as ratio6's, in another
-// order: fmax ignores a NaN operand and the values are non-negative, so the
-// order does not change the result.  The D3 maxima keep ratio6's starting
-// 0 (as fmax(t, 0) per difference); for the D1 maxima ratio_fin's clamp
-// (FP32) or its test m1 > 0 (FP64) gives what the starting 0 does.
-template <int L, typename T>
-__device__ __forceinline__ void ratio_run(const T *q, int st, T r[L])
-{
-    T v[L + 5], d[L + 4], t[L + 3], a[L + 2];
-#pragma unroll
-    for (int x = 0; x < L + 5; x++) v[x] = q[x * st];
-#pragma unroll
-    for (int x = 0; x < L + 4; x++) d[x] = v[x + 1] - v[x];
-    // D3 at the centre differences 1..L+2
-#pragma unroll
-    for (int x = 1; x < L + 3; x++) t[x] = fmax(d3abs(d[x], v[x + 2], v[x - 1]), (T)0);
-    // max |d[x..x+2]|
-#pragma unroll
-    for (int x = 0; x < L + 2; x++) a[x] = fmax(fmax(fabs(d[x]), fabs(d[x + 1])), fabs(d[x + 2]));
-#pragma unroll
-    for (int c = 0; c < L; c++) {
-        const T m1 = fmax(fmax(a[c], fabs(d[c + 3])), fabs(d[c + 4]));
-        const T m3 = fmax(fmax(t[c + 1], t[c + 2]), t[c + 3]);
-        r[c] = ratio_fin(m1, m3);
-    }
-}
-
-// r[c] to o[c st] for c <= lim, 0 beyond
-template <int L, typename T>
-__device__ __forceinline__ void ratio_store(T *o, int st, const T r[L], int lim)
-{
-    if (lim >= L - 1) {
-#pragma unroll
-        for (int c = 0; c < L; c++) o[c * st] = r[c];
-    } else {
-#pragma unroll
-        for (int c = 0; c < L; c++) o[c * st] = c <= lim ? r[c] : (T)0;
-    }
-}
 
 // all 21 ratio pairs for the own column (full = true); a halo column needs
 // only the in-plane pairs and pv/pb(f, 2), f in-plane.  BGP: the background

@@ -31,6 +31,7 @@ EXPORTS = [
     "smaug_sums", "smaug_sums_noalloc", "smaug_ens_sums", "smaug_ens_time_passes",
     "smaug_ens_create", "smaug_ens_destroy", "smaug_ens_nrun", "smaug_ens_ninterior",
     "smaug_ens_set_state", "smaug_ens_get_state", "smaug_ens_rk4", "smaug_ens_time_rk4",
+    "smaug_ens_set_ratio_form", "smaug_ens_get_ratio_form",
 ]
 
 
@@ -112,6 +113,8 @@ def load_library(path: str = LIB_PATH):
         lib.smaug_ens_get_state.argtypes = [vp, ctypes.c_int, dp]
         lib.smaug_ens_rk4.argtypes = [vp, dp, dp, ctypes.c_int]
         lib.smaug_ens_time_rk4.argtypes = [vp, dp, dp, ctypes.c_int, dp]
+        lib.smaug_ens_set_ratio_form.argtypes = [vp, ctypes.c_int]
+        lib.smaug_ens_get_ratio_form.argtypes = [vp, ip]
         _lib = lib
     return _lib
 
@@ -260,7 +263,8 @@ class Engine:
     def set_ratio_form(self, zstreamed, kz=0):
         """3-D: hyperviscosity ratios z-streamed (1), per cell (0), or
         z-streamed along k with the in-plane pairs formed by the z-streamed
-        source kernel (2, the default)"""
+        source kernel (2, the default).  2-D: every pair formed by the update
+        kernel (2, the default) or by the per-cell ratio pass (0)"""
         self._check(self.lib.smaug_set_ratio_form(self.h, int(zstreamed), kz))
 
     def get_ratio_form(self):
@@ -383,6 +387,16 @@ class Ensemble:
         self._check(self.lib.smaug_ens_time_rk4(self.h, _dptr(t0), _dptr(dt), nsteps,
                                                 ctypes.byref(ms)))
         return ms.value
+
+    def set_ratio_form(self, form):
+        """2-D runs: the stage update forms the ratios itself (2, the default)
+        or reads them from the ratio pass (0)"""
+        self._check(self.lib.smaug_ens_set_ratio_form(self.h, int(form)))
+
+    def get_ratio_form(self):
+        f = ctypes.c_int()
+        self._check(self.lib.smaug_ens_get_ratio_form(self.h, ctypes.byref(f)))
+        return f.value
 
 
 def stream_probe(ncell=1 << 24, reps=10):

@@ -81,6 +81,52 @@ def test_member_matches_single_grid(dim, prec):
     ens.close()
 
 
+def periodic_bg_member(side):
+    """an Orszag-Tang-type member with part of its density as a background
+    that varies along x and is periodic (ghost cells copied from their
+    images), so that the update reads it per cell"""
+    s = configs.alfven_member(side, 0.9)
+    n0, n1 = s.n[0], s.n[1]
+    w, bg = s.w.copy(), s.bg.copy()
+    i = np.arange(n0)
+    f = 1.0 + 0.2 * np.sin(2 * np.pi * (i - 2) / (n0 - 4))
+    rb = (0.5 * w[0].reshape(n1, n0)) * f[None, :]
+    rb[:, 0:2] = rb[:, n0 - 4:n0 - 2]
+    rb[:, n0 - 2:n0] = rb[:, 2:4]
+    rb[0:2, :] = rb[n1 - 4:n1 - 2, :]
+    rb[n1 - 2:n1, :] = rb[2:4, :]
+    bg[1] = rb.reshape(-1)
+    w[0] = w[0] - bg[1]
+    s.w, s.bg = w, bg
+    return s
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [MIXED, FP64], ids=["mixed", "fp64"])
+def test_ratio_form2_2d_ensemble_matches_ratio_pass(prec):
+    # the update pulls 64 x 4 tiles and forms their ratios in LDS (form 2)
+    # or reads those of the ratio pass (form 0): the same states bit for bit
+    setups = members_2d() + [periodic_bg_member(70)]
+    print(f"[case] ensemble ratio form 2 vs 0 prec={prec} sizes={[s.n for s in setups]}",
+          flush=True)
+    w0 = states(setups, 1e-2, seed=13)
+    dts = [0.3 / configs.initial_max_courant(s) for s in setups]
+    t0 = [0.0, 1.5, 3.0, 0.5]
+    out = {}
+    for form in (0, 2):
+        ens = Ensemble(setups, prec)
+        ens.set_ratio_form(form)
+        assert ens.get_ratio_form() == form
+        for r, w in enumerate(w0):
+            ens.set_state(r, w, setups[r].bg)
+        ens.rk4(t0, dts, 6)
+        out[form] = [ens.get_state(r) for r in range(len(setups))]
+        ens.close()
+    for r, s in enumerate(setups):
+        assert np.all(np.isfinite(out[2][r]))
+        assert np.array_equal(out[2][r], out[0][r]), (r, rel_err(out[2][r], out[0][r]))
+
+
 @pytest.mark.gpu
 def test_fp64_ensemble_vs_oracle():
     setups = members_2d()[:2]
