@@ -216,7 +216,7 @@ int smaug_ens_time_passes(smaug_ensemble *e, const double *t0, const double *dt,
 int smaug_ens_time_rk4(smaug_ensemble *e, const double *t0, const double *dt, int nsteps,
                        double *total_ms);
 /* 2-D runs with hyperdiffusion: 2 (the default) the stage update pulls
- * 64 x 4-cell tiles and forms their ratio pairs itself, in LDS, and the
+ * tiles of 64 x 8 cells (FP32; 64 x 4 in FP64) and forms their ratio pairs itself, in LDS, and the
  * stage has no ratio pass; 0 the ratio pass writes them to HBM.  Same
  * ratios either way.  smaug_ens_get_ratio_form reports the form in use. */
 int smaug_ens_set_ratio_form(smaug_ensemble *e, int form);

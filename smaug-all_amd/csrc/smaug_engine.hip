@@ -206,14 +206,14 @@ static int launch_cmax(smaug_engine *e, const void *u, int fill = 0)
     return 0;
 }
 
-// tiles of 64 x 4 cells over an extent of e0 x e1 x e2 cells, grouped in
+// tiles of 64 x th cells over an extent of e0 x e1 x e2 cells, grouped in
 // panels of tpx x tpy tiles so that each XCD sweeps its own sub-domain (see
 // map_block); tpx, tpy <= 0 pick the default panel
-static TileMap make_map(int e0, int e1, int e2, int tpx, int tpy, unsigned *nblocks)
+static TileMap make_map(int e0, int e1, int e2, int tpx, int tpy, unsigned *nblocks, int th = 4)
 {
     TileMap m;
     m.ntx = (e0 + 63) / 64;
-    m.nty = (e1 + 3) / 4;
+    m.nty = (e1 + th - 1) / th;
     m.nk = e2;
     if (tpx <= 0 || tpy <= 0) {
         tpx = m.ntx < 8 ? m.ntx : 8;
@@ -260,6 +260,17 @@ static bool ratio_in_update(const smaug_engine *e)
     return e->D == 2 && e->ratio_zs == 2 && e->phys.hyperdif;
 }
 
+// the tiles of the per-cell update pass over the interior: 64 x 4 cells,
+// or 64 x u2_tj for the 2-D update that forms its ratios.  The ghost fill
+// that follows a folded update places its cfast partials after the
+// update's, so both take the block count from here.
+static TileMap update_map(const smaug_engine *e, unsigned *nb)
+{
+    const int th = !ratio_in_update(e) ? 4 : (e->prec == SMAUG_FP64 ? u2_tj<double>() : u2_tj<float>());
+    return make_map(e->n[0] - 4, e->n[1] - 4, e->D == 3 ? e->n[2] - 4 : 1, e->panel[1][0],
+                    e->panel[1][1], nb, th);
+}
+
 template <int D, typename T>
 static int launch_ratio(smaug_engine *e, const void *u)
 {
@@ -299,8 +310,7 @@ static int launch_update(smaug_engine *e, StageArgs sa, const void *uin, const v
                          void *uout, void *aux_out = nullptr)
 {
     unsigned nb;
-    TileMap m = make_map(e->n[0] - 4, e->n[1] - 4, D == 3 ? e->n[2] - 4 : 1, e->panel[1][0],
-                         e->panel[1][1], &nb);
+    TileMap m = update_map(e, &nb);
     const double *bg;
     const Geo g = pc_geo(e, &bg);
     // a background that is the same in every cell (bgm == 0) is read once
@@ -313,14 +323,14 @@ static int launch_update(smaug_engine *e, StageArgs sa, const void *uin, const v
                 if ((idx_t)nb + gb > (idx_t)e->ncpart)
                     return fail(SMAUG_EINVAL, "cfast partials overflow");
                 auto kern = uni ? k_update2r<T, true, true> : k_update2r<T, false, true>;
-                hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, e->stream, g, m, e->phys, sa,
+                hipLaunchKernelGGL(kern, dim3(nb), dim3(U2<T>::NT), 0, e->stream, g, m, e->phys, sa,
                                    (const T *)uin, (const T *)un, (T *)uout, (T *)e->ACC,
                                    (const T *)e->aux_in, bg,
                                    (const unsigned long long *)e->d_cmax, (T *)aux_out,
                                    e->d_cpart);
             } else {
                 auto kern = uni ? k_update2r<T, true> : k_update2r<T, false>;
-                hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, e->stream, g, m, e->phys, sa,
+                hipLaunchKernelGGL(kern, dim3(nb), dim3(U2<T>::NT), 0, e->stream, g, m, e->phys, sa,
                                    (const T *)uin, (const T *)un, (T *)uout, (T *)e->ACC,
                                    (const T *)e->aux_in, bg,
                                    (const unsigned long long *)e->d_cmax, nullptr, nullptr);
@@ -498,7 +508,7 @@ static int launch_ghost_fold(smaug_engine *e, void *uout, void *aux_out)
                            (T *)uout, (const double *)e->bgp, (T *)e->AUX,
                            e->d_cpart + (size_t)4 * nb, 0);
     } else {
-        make_map(e->n[0] - 4, e->n[1] - 4, 1, e->panel[1][0], e->panel[1][1], &nb);
+        update_map(e, &nb);
         gb = (unsigned)((ghost_count2(e->n[0], e->n[1]) + 255) / 256);
         if ((idx_t)nb + gb > (idx_t)e->ncpart) return fail(SMAUG_EINVAL, "cfast partials overflow");
         const double *bg;
@@ -1196,9 +1206,9 @@ extern "C" int smaug_get_panel(smaug_engine *e, int pass, int *tpx, int *tpy)
     if (!e || pass < 0 || pass > 1 || !tpx || !tpy) return fail(SMAUG_EINVAL, "bad argument");
     unsigned nb;
     const int D = e->D;
-    const int c = pass == 0 ? 3 : 4;
-    TileMap m = make_map(e->n[0] - c, e->n[1] - c, D == 3 ? e->n[2] - c : 1, e->panel[pass][0],
-                         e->panel[pass][1], &nb);
+    TileMap m = pass == 1 ? update_map(e, &nb)
+                          : make_map(e->n[0] - 3, e->n[1] - 3, D == 3 ? e->n[2] - 3 : 1,
+                                     e->panel[0][0], e->panel[0][1], &nb);
     *tpx = m.tpx;
     *tpy = m.tpy;
     return 0;

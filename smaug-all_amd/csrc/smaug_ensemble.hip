@@ -58,7 +58,7 @@ __device__ __forceinline__ unsigned fdiv(unsigned t, const FastDiv &f)
     return (unsigned)(((unsigned long long)t * f.m) >> f.s);
 }
 constexpr int NPASS = 4;         // prep, ratio, update, ghost fill
-constexpr int NLIST = NPASS + 1; // their item lists, and the 64 x 4 tiles of the 2-D
+constexpr int NLIST = NPASS + 1; // their item lists, and the 64 x u2_tj tiles of the 2-D
                                  // update that forms the ratios itself (ke_update2r,
                                  // one block per item, no queue)
 constexpr int NQ = 8;            // queue slices, one per XCD
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(256) void ke_update(const RunDesc<T> *__restrict__ 
     }
 }
 
-// 2-D ratio form 2: the update of the 64 x 4-cell tiles of every run, each
+// 2-D ratio form 2: the update of the 64 x u2_tj-cell tiles of every run, each
 // forming the ratio pairs of its tile and a 1-cell halo in LDS as
 // k_update2r does (the stage then has no ratio pass).  One block per tile
 // item rather than a persistent queue (1-2% faster here: the items are
@@ -261,20 +261,20 @@ __global__ __launch_bounds__(256) void ke_update(const RunDesc<T> *__restrict__ 
 // contiguous slices and block b takes item b / 8 of slice b mod 8 (its
 // XCD's), so the tiles of a run mostly share one L2
 template <typename T>
-__global__ __launch_bounds__(256) void ke_update2r(const RunDesc<T> *__restrict__ runs,
+__global__ __launch_bounds__(U2<T>::NT) void ke_update2r(const RunDesc<T> *__restrict__ runs,
                                                       const int *__restrict__ base,
                                                       const int *__restrict__ irun, int nitems,
                                                       Phys ph, int stage, double ct, int src,
                                                       int dst)
 {
-    __shared__ T sQ[6][U2_QN];
-    __shared__ T sR[Lay<2>::NPAIR * U2_RN];
+    __shared__ T sQ[6][U2<T>::QN];
+    __shared__ T sR[Lay<2>::NPAIR * U2<T>::RN];
     const int per = (nitems + 7) >> 3;
     const int it = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if ((int)(blockIdx.x >> 3) >= per || it >= nitems) return;
     const int r = irun[it];
     const RunDesc<T> &rd = runs[r];
-    const int ntx = (rd.geo.n[0] - 4 + U2_TI - 1) / U2_TI;
+    const int ntx = (rd.geo.n[0] - 4 + U2<T>::TI - 1) / U2<T>::TI;
     const int t = it - base[r];
     const int ty = t / ntx, tx = t - ty * ntx;
     StageArgs sa;
@@ -421,7 +421,8 @@ static int setup_typed(smaug_ensemble *e)
             e->phys.hyperdif ? (nbox[0] + RATIO_CELLS - 1) / RATIO_CELLS : 0,
             (nbox[1] + UPDATE_CELLS - 1) / UPDATE_CELLS,
             (long long)((bc_count<D>(g.n[0], g.n[1], n2) + BC_CELLS - 1) / BC_CELLS),
-            D == 2 ? (long long)((g.n[0] - 4 + U2_TI - 1) / U2_TI) * ((g.n[1] - 4 + U2_TJ - 1) / U2_TJ)
+            D == 2 ? (long long)((g.n[0] - 4 + U2<T>::TI - 1) / U2<T>::TI) *
+                         ((g.n[1] - 4 + u2_tj<T>() - 1) / u2_tj<T>())
                    : 0};
         for (int k = 0; k < NLIST; k++) {
             long long nb = (long long)base[k][r] + items[k];
@@ -720,7 +721,7 @@ static int ens_step(smaug_ensemble *e, const hipEvent_t *evk = nullptr)
         if (evk) HIPCHK(hipEventRecord(evk[5 * s + 2], e->stream));
         if constexpr (D == 2) {
             if (r2)
-                hipLaunchKernelGGL(ke_update2r<T>, dim3(8 * ((e->nitems[4] + 7) / 8)), dim3(256),
+                hipLaunchKernelGGL(ke_update2r<T>, dim3(8 * ((e->nitems[4] + 7) / 8)), dim3(U2<T>::NT),
                                    0, e->stream, rd, e->base[4], e->irun[4], e->nitems[4],
                                    e->phys, s, ct[s], in[s], out[s]);
         }

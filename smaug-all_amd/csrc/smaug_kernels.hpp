@@ -104,13 +104,13 @@ __device__ __forceinline__ void block_max_to(double *mx, double (*red)[4], unsig
     }
 }
 
-// block-wide max of mx[] (blockDim.x == 256) written to part[0..D-1]: the
-// single-grid passes leave one partial maximum per block, and k_cmax_reduce
-// folds them.  (One 64-bit atomic max per block on the same few addresses
-// serialises: thousands of short blocks then spend more time in the
-// atomics than in their cells.)
-template <int D>
-__device__ __forceinline__ void block_max_part(double *mx, double (*red)[4], double *part)
+// block-wide max of mx[] (blockDim.x == 64 NW) written to part[0..D-1]:
+// the single-grid passes leave one partial maximum per block, and
+// k_cmax_reduce folds them.  (One 64-bit atomic max per block on the same
+// few addresses serialises: thousands of short blocks then spend more time
+// in the atomics than in their cells.)
+template <int D, int NW = 4>
+__device__ __forceinline__ void block_max_part(double *mx, double (*red)[NW], double *part)
 {
     int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
@@ -122,7 +122,7 @@ __device__ __forceinline__ void block_max_part(double *mx, double (*red)[4], dou
     if (threadIdx.x < D) {
         int d = threadIdx.x;
         double v = red[d][0];
-        for (int k = 1; k < 4; k++) v = red[d][k] > v ? red[d][k] : v;
+        for (int k = 1; k < NW; k++) v = red[d][k] > v ? red[d][k] : v;
         part[d] = v;
     }
 }
@@ -1210,19 +1210,33 @@ __global__ __launch_bounds__(256) void k_update(Geo g, TileMap tm, Phys ph, Stag
 // ---------------------------------------------------------------------------
 // 2-D stage update with the hyperviscosity ratios formed in the kernel
 // (ratio form 2 of 2-D grids).  A block stages rho', e_int, v_0, v_1, b'_0
-// and b'_1 of its 64 x 4 tile and a 3-cell halo in LDS (pads mirrored as in
+// and b'_1 of its 64 x TJ tile and a 3-cell halo in LDS (pads mirrored as in
 // ratio_cell), forms the ten ratio pairs of the tile and a 1-cell halo from
 // them (0 outside the ratio box [1, n-3], where R, which k_ratio fills only
 // there, holds the zeros it was cleared to), and runs update_cell with the
 // pairs read from LDS: bit for bit k_ratio followed by k_update, without R
-// in HBM.  The pairs along i are formed in runs of U2_RL cells of a row,
+// in HBM.  The pairs along i are formed in runs of 11 cells of a row,
 // those along j per column of the box.
 // ---------------------------------------------------------------------------
-constexpr int U2_TI = 64, U2_TJ = 4;
-constexpr int U2_RI = U2_TI + 2, U2_RJ = U2_TJ + 2, U2_RN = U2_RI * U2_RJ; // ratio box
-constexpr int U2_QI = U2_TI + 7, U2_QJ = U2_TJ + 7, U2_QN = U2_QI * U2_QJ; // window box
-constexpr int U2_RL = 11, U2_NRUN = U2_RI / U2_RL;
-static_assert(U2_RL * U2_NRUN == U2_RI, "the runs cover the rows of the ratio box");
+// tiles of 64 x TJ cells, 64 TJ threads
+template <int TJ>
+struct U2Shape {
+    static constexpr int TI = 64, NT = TI * TJ;
+    static constexpr int RI = TI + 2, RJ = TJ + 2, RN = RI * RJ; // ratio box
+    static constexpr int QI = TI + 7, QJ = TJ + 7, QN = QI * QJ; // window box
+    static constexpr int RL = 11, NRUN = RI / RL;
+    static_assert(RL * NRUN == RI, "the runs cover the rows of the ratio box");
+};
+// the tile height of k_update2r and ke_update2r: 8 rows in FP32 (77 VGPRs,
+// 52 KB of LDS, six waves per SIMD; 4 rows: four), 4 in FP64 (8 rows would
+// take 104 KB of LDS, one block per CU)
+template <typename T>
+__host__ __device__ constexpr int u2_tj()
+{
+    return sizeof(T) == 4 ? 8 : 4;
+}
+template <typename T>
+using U2 = U2Shape<u2_tj<T>()>;
 
 // pair a (0..4) of the pairs along d: rho', e_int, v_0, v_1, b'_f (f != d):
 // its window quantity in the staged box and its ratio pair
@@ -1234,28 +1248,30 @@ __device__ __forceinline__ int u2_pair(int d, int a)
                   : (a == 1 ? LY::pe(d) : (a < 4 ? LY::pv(a - 2, d) : LY::pb(1 - d, d)));
 }
 
-// one 64 x 4 tile (tx, ty) of the interior: the window quantities staged,
+// one 64 x TJ tile (tx, ty) of the interior: the window quantities staged,
 // the ratio pairs formed, update_cell run (FOLD: the cfast maxima into mx);
 // every thread of the block calls it (two block barriers)
-template <typename T, bool UNI, bool FOLD>
+template <typename T, bool UNI, bool FOLD, int TJ = u2_tj<T>()>
 __device__ __forceinline__ void update2r_tile(const Geo &g, const Phys &ph, const StageArgs &sa,
                                               const T *__restrict__ uin,
                                               const T *__restrict__ un, T *__restrict__ uout,
                                               T *__restrict__ acc, const T *__restrict__ aux,
                                               const double *__restrict__ bg,
                                               const unsigned long long *__restrict__ cmb,
-                                              const int tx, const int ty, T (*sQ)[U2_QN], T *sR,
+                                              const int tx, const int ty,
+                                              T (*sQ)[U2Shape<TJ>::QN], T *sR,
                                               T *__restrict__ aux_out, double *mx)
 {
     typedef Lay<2> LY;
     typedef Aux<2> AX;
+    typedef U2Shape<TJ> SH;
     const int n0 = g.n[0], n1 = g.n[1];
     const size_t nc = (size_t)g.nc;
-    const int i0 = 2 + tx * U2_TI, j0 = 2 + ty * U2_TJ;
+    const int i0 = 2 + tx * SH::TI, j0 = 2 + ty * TJ;
     if (ph.hyperdif) {
         // window box cell (qi, qj) = cell (i0 - 3 + qi, j0 - 3 + qj)
-        for (int q = threadIdx.x; q < U2_QN; q += 256) {
-            const int qj = q / U2_QI, qi = q - qj * U2_QI;
+        for (int q = threadIdx.x; q < SH::QN; q += SH::NT) {
+            const int qj = q / SH::QI, qi = q - qj * SH::QI;
             const ci_t c = (ci_t)pad_map(i0 - 3 + qi, n0) + (ci_t)g.s1 * (ci_t)pad_map(j0 - 3 + qj, n1);
             sQ[0][q] = ld(uin, c);
             sQ[1][q] = ld(aux + AX::EINT * nc, c);
@@ -1268,25 +1284,25 @@ __device__ __forceinline__ void update2r_tile(const Geo &g, const Phys &ph, cons
         // ratio box cell (bi, bj) = cell (i0 - 1 + bi, j0 - 1 + bj); its
         // window along i starts at window-box column bi of row bj + 2, along
         // j at row bj of column bi + 2
-        constexpr int NI = 5 * U2_RJ * U2_NRUN, NJ = 5 * U2_RI;
-        for (int u = threadIdx.x; u < NI + NJ; u += 256) {
+        constexpr int NI = 5 * SH::RJ * SH::NRUN, NJ = 5 * SH::RI;
+        for (int u = threadIdx.x; u < NI + NJ; u += SH::NT) {
             if (u < NI) {
-                const int a = u / (U2_RJ * U2_NRUN);
-                const int y = u - a * (U2_RJ * U2_NRUN);
-                const int bj = y / U2_NRUN, bi = U2_RL * (y - bj * U2_NRUN);
-                T r[U2_RL];
-                ratio_run<U2_RL, T>(sQ[u2_field(0, a)] + (bj + 2) * U2_QI + bi, 1, r);
+                const int a = u / (SH::RJ * SH::NRUN);
+                const int y = u - a * (SH::RJ * SH::NRUN);
+                const int bj = y / SH::NRUN, bi = SH::RL * (y - bj * SH::NRUN);
+                T r[SH::RL];
+                ratio_run<SH::RL, T>(sQ[u2_field(0, a)] + (bj + 2) * SH::QI + bi, 1, r);
                 const int gj = j0 - 1 + bj;
                 const int lim = gj > n1 - 3 ? -1 : n0 - 3 - (i0 - 1 + bi);
-                ratio_store<U2_RL, T>(sR + u2_pair(0, a) * U2_RN + bj * U2_RI + bi, 1, r, lim);
+                ratio_store<SH::RL, T>(sR + u2_pair(0, a) * SH::RN + bj * SH::RI + bi, 1, r, lim);
             } else {
                 const int x = u - NI;
-                const int a = x / U2_RI, bi = x - a * U2_RI;
-                T r[U2_RJ];
-                ratio_run<U2_RJ, T>(sQ[u2_field(1, a)] + bi + 2, U2_QI, r);
+                const int a = x / SH::RI, bi = x - a * SH::RI;
+                T r[SH::RJ];
+                ratio_run<SH::RJ, T>(sQ[u2_field(1, a)] + bi + 2, SH::QI, r);
                 const int gi = i0 - 1 + bi;
                 const int lim = gi > n0 - 3 ? -1 : n1 - 3 - (j0 - 1);
-                ratio_store<U2_RJ, T>(sR + u2_pair(1, a) * U2_RN + bi, U2_RI, r, lim);
+                ratio_store<SH::RJ, T>(sR + u2_pair(1, a) * SH::RN + bi, SH::RI, r, lim);
             }
         }
         __syncthreads();
@@ -1296,15 +1312,15 @@ __device__ __forceinline__ void update2r_tile(const Geo &g, const Phys &ph, cons
     if (x0 > n0 - 3 || x1 > n1 - 3) return;
     RBox rb;
     rb.sR = sR;
-    rb.nb = U2_RN;
-    rb.own = (tj + 1) * U2_RI + ti + 1;
-    rb.bi = U2_RI;
+    rb.nb = SH::RN;
+    rb.own = (tj + 1) * SH::RI + ti + 1;
+    rb.bi = SH::RI;
     update_cell<2, T, UNI, FOLD, true>(g, ph, sa, uin, un, uout, acc, nullptr, aux, bg, cmb, x0,
                                        x1, 0, aux_out, mx, rb);
 }
 
 template <typename T, bool UNI = false, bool FOLD = false>
-__global__ __launch_bounds__(256) void k_update2r(Geo g, TileMap tm, Phys ph, StageArgs sa,
+__global__ __launch_bounds__(U2<T>::NT) void k_update2r(Geo g, TileMap tm, Phys ph, StageArgs sa,
                                                    const T *__restrict__ uin,
                                                    const T *__restrict__ un,
                                                    T *__restrict__ uout, T *__restrict__ acc,
@@ -1314,19 +1330,19 @@ __global__ __launch_bounds__(256) void k_update2r(Geo g, TileMap tm, Phys ph, St
                                                    T *__restrict__ aux_out = nullptr,
                                                    double *part = nullptr)
 {
-    __shared__ T sQ[6][U2_QN];
-    __shared__ T sR[Lay<2>::NPAIR * U2_RN];
+    __shared__ T sQ[6][U2<T>::QN];
+    __shared__ T sR[Lay<2>::NPAIR * U2<T>::RN];
     int tx, ty, tk;
     if (!map_block(tm, tx, ty, tk)) {
         if (FOLD && threadIdx.x < 2) part[4 * blockIdx.x + threadIdx.x] = 0.0;
         return;
     }
     if constexpr (FOLD) {
-        __shared__ double red[3][4];
+        __shared__ double red[3][U2<T>::NT / 64];
         double mx[3] = {0.0, 0.0, 0.0};
         update2r_tile<T, UNI, true>(g, ph, sa, uin, un, uout, acc, aux, bg, cmb, tx, ty, sQ, sR,
                                     aux_out, mx);
-        block_max_part<2>(mx, red, part + 4 * blockIdx.x);
+        block_max_part<2, U2<T>::NT / 64>(mx, red, part + 4 * blockIdx.x);
     } else {
         update2r_tile<T, UNI, false>(g, ph, sa, uin, un, uout, acc, aux, bg, cmb, tx, ty, sQ, sR,
                                      nullptr, nullptr);
