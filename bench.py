@@ -26,6 +26,7 @@ Printed JSON line (rank 0): the contract fields plus
                 runs stepped one grid at a time for comparison
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -145,7 +146,8 @@ def accuracy_leg(setup, nsteps):
     by 1e-9 (a smooth perturbation), and the state rounded to FP32 once,
     after the first step, the FP64 path continuing from the rounded state
     (the perturbation that storing the state in FP32 makes, once instead of
-    at every stage)."""
+    at every stage); and mixed against FP64 once more with the hyperdiffusion
+    switched off, the term whose ratio switch carries the amplification."""
     import numpy as np
 
     from smaugamd.engine import FP64, MIXED, Engine
@@ -156,9 +158,11 @@ def accuracy_leg(setup, nsteps):
     m[2:-2, 2:-2, 2:-2] = True
     m = m.reshape(-1)
     out = {}
+    nohd = dataclasses.replace(setup, hyperdif=0)
     for key, prec, scale in (("mixed", MIXED, 1.0), ("fp64", FP64, 1.0),
-                             ("fp64_dt", FP64, 1.0 + 1e-9), ("fp64_round1", FP64, 1.0)):
-        e = Engine(setup, prec)
+                             ("fp64_dt", FP64, 1.0 + 1e-9), ("fp64_round1", FP64, 1.0),
+                             ("mixed_nohd", MIXED, 1.0), ("fp64_nohd", FP64, 1.0)):
+        e = Engine(nohd if key.endswith("_nohd") else setup, prec)
         e.set_state()
         dt = setup.dt * scale
         if key == "fp64_round1":
@@ -174,13 +178,15 @@ def accuracy_leg(setup, nsteps):
     l2, linf = _field_errors(out["mixed"], out["fp64"], m, names)
     sens, _ = _field_errors(out["fp64_dt"], out["fp64"], m, names)
     rnd, _ = _field_errors(out["fp64_round1"], out["fp64"], m, names)
+    nhd, _ = _field_errors(out["mixed_nohd"], out["fp64_nohd"], m, names)
     return {"reference": "FP64 GPU path (parity-tested against the FP64 oracle)",
             "steps": nsteps, "rel_l2": l2, "rel_linf": linf,
             "target_rel_l2": 1e-5,
             "meets_target": bool(all(v <= 1e-5 for v in l2.values())),
             "fp64_rel_l2_under_1e-9_dt_perturbation": sens,
             "fp64_rel_l2_after_one_fp32_rounding_of_the_state_at_step_1": rnd,
-            "target_reachable_with_fp32_state": bool(all(v <= 1e-5 for v in rnd.values()))}
+            "target_reachable_with_fp32_state": bool(all(v <= 1e-5 for v in rnd.values())),
+            "rel_l2_without_hyperdiffusion": nhd}
 
 
 def pass_kernels(form, bg_plane, fp32=True, rform=1):

@@ -461,25 +461,30 @@ static int launch_zstream(smaug_engine *e, StageArgs sa, const void *uin, const 
     const bool bp = bg_planar(e);
     const double *bg = bp ? e->bgp : e->bg;
     if (fold && !bp) return fail(SMAUG_EINVAL, "folded aux pass needs the k-plane background");
+    // Every lane of the update issues its loads of the sources, at cell 0
+    // where it has no use for them (no output, or no hyperdiffusion).  HYP
+    // is only allocated with hyperdiffusion; without it the update is given
+    // uin, 8 fields of nc cells like HYP, so that those loads stay in bounds.
+    const void *hyp = e->HYP ? e->HYP : uin;
     if constexpr (sizeof(T) == 4) {
         if (fold)
             hipLaunchKernelGGL((k_zstream_f<true, true>), dim3(nb), dim3(ZS_NT), 0, e->stream,
                                e->geo, e->phys, sa, m, (const T *)uin, (const T *)un, (T *)uout,
-                               (T *)e->ACC, (const T *)e->HYP, bg, (T *)e->AUX, e->d_cpart);
+                               (T *)e->ACC, (const T *)hyp, bg, (T *)e->AUX, e->d_cpart);
         else
             hipLaunchKernelGGL(bp ? k_zstream_f<true> : k_zstream_f<false>, dim3(nb), dim3(ZS_NT),
                                0, e->stream, e->geo, e->phys, sa, m, (const T *)uin,
-                               (const T *)un, (T *)uout, (T *)e->ACC, (const T *)e->HYP, bg,
+                               (const T *)un, (T *)uout, (T *)e->ACC, (const T *)hyp, bg,
                                nullptr, nullptr);
     } else {
         if (fold)
             hipLaunchKernelGGL((k_zstream_d<true, true>), dim3(nb), dim3(ZS_NT), 0, e->stream,
                                e->geo, e->phys, sa, m, (const T *)uin, (const T *)un, (T *)uout,
-                               (T *)e->ACC, (const T *)e->HYP, bg, (T *)e->AUX, e->d_cpart);
+                               (T *)e->ACC, (const T *)hyp, bg, (T *)e->AUX, e->d_cpart);
         else
             hipLaunchKernelGGL(bp ? k_zstream_d<true> : k_zstream_d<false>, dim3(nb), dim3(ZS_NT),
                                0, e->stream, e->geo, e->phys, sa, m, (const T *)uin,
-                               (const T *)un, (T *)uout, (T *)e->ACC, (const T *)e->HYP, bg,
+                               (const T *)un, (T *)uout, (T *)e->ACC, (const T *)hyp, bg,
                                nullptr, nullptr);
     }
     HIPCHK(hipGetLastError());

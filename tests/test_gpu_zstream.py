@@ -274,3 +274,38 @@ def test_timed_shapes_mixed_matches_per_cell(kz, kzr):
     err = rel_err(got, ref, interior_mask(THIN))
     print(f"kz={kz} kzr={kzr} mixed rhs rel err vs per-cell", err)
     assert np.all(err < TOL_RHS[MIXED]), err
+
+
+# without hyperdiffusion the engine allocates no source buffer; the
+# z-streamed update, whose lanes issue their source loads on every path,
+# then reads a stand-in buffer (a grid with hyperdif = 0 once faulted here)
+@pytest.mark.parametrize("prec", [FP64, MIXED], ids=["fp64", "mixed"])
+def test_zstream_without_hyperdiffusion(prec):
+    s = dataclasses.replace(strat(20, 12, 16), hyperdif=0)
+    print(f"[case] zstream without hyperdiffusion prec={prec} n={s.n}", flush=True)
+    o = Oracle(s)
+    w0 = o.boundary(perturbed(s, amp=1e-3, seed=41))
+    if prec == MIXED:
+        w0 = o.boundary(w0.astype(np.float32).astype(np.float64))
+    dt = 0.5 * configs.hyperviscous_dt(s)
+    out = {}
+    for form in ("zstream", "per-cell"):
+        e = Engine(s, prec)
+        if form == "per-cell":
+            e.set_zstream(False)
+        else:
+            e.set_zstream(True, 5)
+            assert e.get_zstream() == (1, 5)
+        e.set_state(w0)
+        e.rk4(0.5, dt, 3)
+        out[form] = e.get_state()
+        e.close()
+    assert np.all(np.isfinite(out["zstream"]))
+    if prec == FP64:
+        ref, _ = o.run(w0, 0.5, dt, 3, rk4=True)
+        err = rel_err(out["zstream"], ref)
+        print("no-hyperdiffusion zstream rk4 rel err vs oracle", err)
+        assert np.all(err < TOL_ORACLE_FP64), err
+    err = rel_err(out["zstream"], out["per-cell"], interior_mask(s))
+    print("no-hyperdiffusion zstream vs per-cell rel err", err)
+    assert np.all(err < TOL_PLANE[prec] * 10), err
