@@ -165,3 +165,37 @@ def test_repeated_calls_continue_time():
         np.testing.assert_array_equal(a.get_state(r), b.get_state(r))
     a.close()
     b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim", [2, 3])
+def test_member_matches_single_grid_without_hyperdiffusion(dim):
+    """without hyperdiffusion the ensemble holds no ratio buffer and runs no
+    ratio pass; its members still evolve bit for bit like the single grid"""
+    setups = members_2d() if dim == 2 else members_3d()
+    for s in setups:
+        s.hyperdif = 0
+    print(f"[case] ensemble without hyperdiffusion dim={dim} sizes={[s.n for s in setups]}",
+          flush=True)
+    w0 = states(setups, 1e-2 if dim == 2 else 1e-3, seed=23)
+    dts = [0.3 / configs.initial_max_courant(s) for s in setups]
+    if dim == 3:
+        dts = [0.5 * configs.hyperviscous_dt(s) for s in setups]
+    t0 = [0.0, 1.5, 3.0][:len(setups)]
+    ens = Ensemble(setups, MIXED)
+    for r, w in enumerate(w0):
+        ens.set_state(r, w)
+    ens.rk4(t0, dts, 4)
+    for r, s in enumerate(setups):
+        e = Engine(s, MIXED)
+        if dim == 3:
+            e.set_zstream(False)
+        e.set_state(w0[r])
+        e.rk4(t0[r], dts[r], 4)
+        ref = e.get_state()
+        got = ens.get_state(r)
+        e.close()
+        print(f"run {r} n={s.n} bitwise={np.array_equal(got, ref)}")
+        assert np.all(np.isfinite(got))
+        assert np.array_equal(got, ref)
+    ens.close()
