@@ -280,14 +280,23 @@ static int launch_ratio(smaug_engine *e, const void *u)
             const bool kz = ratio_in_sources(e);
             ZMap m = make_rmap(e, &nb, kz ? KZ_TJ : RZ_TJ);
             const int nt = kz ? KZ_NT : RZ_NT;
+            if (kz) {
+                hipError_t he;
+                if constexpr (sizeof(T) == 4)
+                    he = launch_ratio_kz_f(nb, e->stream, e->geo, m, (const T *)u,
+                                           (const T *)e->aux_in, (T *)e->R);
+                else
+                    he = launch_ratio_kz_d(nb, e->stream, e->geo, m, (const T *)u,
+                                           (const T *)e->aux_in, (T *)e->R);
+                HIPCHK(he);
+                return 0;
+            }
             if constexpr (sizeof(T) == 4)
-                hipLaunchKernelGGL(kz ? k_ratio_kz_f : k_ratio_zs_f, dim3(nb), dim3(nt), 0,
-                                   e->stream, e->geo, m, (const T *)u, (const T *)e->aux_in,
-                                   (T *)e->R);
+                hipLaunchKernelGGL(k_ratio_zs_f, dim3(nb), dim3(nt), 0, e->stream, e->geo, m,
+                                   (const T *)u, (const T *)e->aux_in, (T *)e->R);
             else
-                hipLaunchKernelGGL(kz ? k_ratio_kz_d : k_ratio_zs_d, dim3(nb), dim3(nt), 0,
-                                   e->stream, e->geo, m, (const T *)u, (const T *)e->aux_in,
-                                   (T *)e->R);
+                hipLaunchKernelGGL(k_ratio_zs_d, dim3(nb), dim3(nt), 0, e->stream, e->geo, m,
+                                   (const T *)u, (const T *)e->aux_in, (T *)e->R);
             HIPCHK(hipGetLastError());
             return 0;
         }
@@ -405,8 +414,13 @@ static int launch_hyper_sources(smaug_engine *e, const void *uin)
         const double *bg = bp ? e->bgp : e->bg;
         const bool rin = ratio_in_sources(e);
         if constexpr (sizeof(T) == 4) {
-            auto kern = rin ? (bp ? k_hyper_zsr_f<true> : k_hyper_zsr_f<false>)
-                            : (bp ? k_hyper_zs_f<true> : k_hyper_zs_f<false>);
+            if (rin) {
+                HIPCHK(launch_hyper_zsr_f(bp, nb, e->stream, e->geo, m, e->phys, (const T *)uin,
+                                          (const T *)e->R, (const T *)e->aux_in, bg,
+                                          (const unsigned long long *)e->d_cmax, (T *)e->HYP));
+                return 0;
+            }
+            auto kern = bp ? k_hyper_zs_f<true> : k_hyper_zs_f<false>;
             hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, e->stream, e->geo, m, e->phys,
                                (const T *)uin, (const T *)e->R, (const T *)e->aux_in, bg,
                                (const unsigned long long *)e->d_cmax, (T *)e->HYP);

@@ -694,6 +694,7 @@ __device__ __forceinline__ void ratio_zs_body(const Geo &g, const ZMap &zm,
     }
 }
 
+#ifndef SMAUG_ILP_TU // built in smaug_engine.hip
 __global__ __launch_bounds__(RZ_NT) void k_ratio_zs_f(Geo g, ZMap zm, const float *__restrict__ u,
                                                        const float *__restrict__ aux,
                                                        float *__restrict__ R)
@@ -709,6 +710,7 @@ __global__ __launch_bounds__(RZ_NT) void k_ratio_zs_d(Geo g, ZMap zm, const doub
     __shared__ double sQ[RB_NQ][RB_NB];
     ratio_zs_body<double>(g, zm, u, aux, R, sQ);
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // The seven ratio pairs along k alone (prho, pe, pv(0..2), pb(0..1) of
@@ -789,6 +791,8 @@ __device__ __forceinline__ void ratio_kz_body(const Geo &g, const ZMap &zm,
     }
 }
 
+// built in smaug_ilp.hip alone (under its scheduler), which defines SMAUG_ILP_TU
+#ifdef SMAUG_ILP_TU
 __global__ __launch_bounds__(KZ_NT) void k_ratio_kz_f(Geo g, ZMap zm, const float *__restrict__ u,
                                                        const float *__restrict__ aux,
                                                        float *__restrict__ R)
@@ -802,6 +806,7 @@ __global__ __launch_bounds__(KZ_NT) void k_ratio_kz_d(Geo g, ZMap zm, const doub
 {
     ratio_kz_body<double>(g, zm, u, aux, R);
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // Hyperdiffusive sources with the plane staged in LDS.  A block takes a
@@ -935,6 +940,7 @@ __device__ __forceinline__ void hyper_lds_body(const Geo &g, const TileMap &tm, 
     for (int f = 0; f < 8; f++) st(dUh + f * nc, vw.gm.c, dU[f]);
 }
 
+#ifndef SMAUG_ILP_TU // built in smaug_engine.hip
 __global__ __launch_bounds__(256) void k_hyper_lds_f(Geo g, TileMap tm, Phys ph,
                                                       const float *__restrict__ uin,
                                                       const float *__restrict__ R,
@@ -960,6 +966,7 @@ __global__ __launch_bounds__(256) void k_hyper_lds_d(Geo g, TileMap tm, Phys ph,
     __shared__ double sRb[HB_NB];
     hyper_lds_body<double>(g, tm, ph, uin, R, aux, bg, cmb, dUh, sU, sA, sRb, sR);
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // Hyperdiffusive sources, z-streamed.  A block owns 32 x 8 columns and walks
@@ -1856,5 +1863,17 @@ __global__ __launch_bounds__(256) void k_hyper(Geo g, TileMap tm, Phys ph,
 #pragma unroll
     for (int f = 0; f < LY::NVE; f++) st(dUh + f * nc, c, dU[f]);
 }
+
+// Launchers of the kernels that smaug_ilp.hip builds under the max-ILP
+// machine scheduler (defined there; the other translation units only call
+// them, so that these kernels have one build): the FP32 z-streamed sources
+// with the in-plane ratios, and the ratio pairs along k in both precisions.
+hipError_t launch_hyper_zsr_f(bool bgp, unsigned nb, hipStream_t st, const Geo &g, const ZMap &zm,
+                              const Phys &ph, const float *uin, const float *R, const float *aux,
+                              const double *bg, const unsigned long long *cmb, float *dUh);
+hipError_t launch_ratio_kz_f(unsigned nb, hipStream_t st, const Geo &g, const ZMap &zm,
+                             const float *u, const float *aux, float *R);
+hipError_t launch_ratio_kz_d(unsigned nb, hipStream_t st, const Geo &g, const ZMap &zm,
+                             const double *u, const double *aux, double *R);
 
 } // namespace smaug
