@@ -227,15 +227,16 @@ def test_steps_split_across_calls_match(prec):
 
 
 # the launch shapes the timed runs use: update and source z-stream depths 32,
-# 64 and 128, ratio depths 4, 8, 16 and 32 (the autotuner's picks at 256^3
+# 64 and 128, ratio depths 4, 6, 8, 16 and 32 (the autotuner's picks at 256^3
 # and 768x512x512, mixed and FP64), on a thin grid deep enough along k for
 # several chunks
 THIN = strat(12, 12, 260)
 
 
-@pytest.mark.parametrize("kz,kzr", [(64, 8), (128, 16), (128, 8), (64, 16), (128, 4), (32, 32)],
+@pytest.mark.parametrize("kz,kzr", [(64, 8), (128, 16), (128, 8), (64, 16), (128, 4), (32, 32),
+                                     (128, 6)],
                          ids=["kz64_r8", "kz128_r16", "kz128_r8", "kz64_r16", "kz128_r4",
-                              "kz32_r32"])
+                              "kz32_r32", "kz128_r6"])
 def test_timed_shapes_rk4_fp64_vs_oracle(kz, kzr):
     print(f"[case] timed shape kz={kz} kzr={kzr} n={THIN.n}", flush=True)
     o = Oracle(THIN)
@@ -255,8 +256,8 @@ def test_timed_shapes_rk4_fp64_vs_oracle(kz, kzr):
     assert np.all(err < TOL_ORACLE_FP64), err
 
 
-@pytest.mark.parametrize("kz,kzr", [(64, 8), (128, 8), (128, 4), (32, 8)],
-                         ids=["kz64_r8", "kz128_r8", "kz128_r4", "kz32_r8"])
+@pytest.mark.parametrize("kz,kzr", [(64, 8), (128, 8), (128, 4), (32, 8), (128, 6)],
+                         ids=["kz64_r8", "kz128_r8", "kz128_r4", "kz32_r8", "kz128_r6"])
 def test_timed_shapes_mixed_matches_per_cell(kz, kzr):
     print(f"[case] timed shape mixed kz={kz} kzr={kzr} n={THIN.n}", flush=True)
     o = Oracle(THIN)
